@@ -1,0 +1,209 @@
+"""ctypes bindings of the FP64 CPU oracle (oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Nothing in nukexc_amd/ imports
+this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_radial.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
+        L.orc_radial_scale.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_radial_scale.restype = ctypes.c_double
+        L.orc_becke.argtypes = [ctypes.c_int, _d, ctypes.c_long, _d, _i, _d, ctypes.c_int]
+        L.orc_make_flat_grid.argtypes = [ctypes.c_int, _d, _i, ctypes.c_int, ctypes.c_int, _d,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, _d, _d, _i]
+        L.orc_make_flat_grid.restype = ctypes.c_long
+        L.orc_sto_norm_cutoff.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
+        L.orc_solid_harmonic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _d]
+        L.orc_collocation.argtypes = [ctypes.c_int, _i, _i, _i, _d, _d, _d, ctypes.c_long, _d,
+                                      _d, _d, _d, _d]
+        L.orc_density_P.argtypes = [ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d, _d, _d, _d]
+        L.orc_density_C.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d,
+                                    _d, _d, _d, _d]
+        L.orc_xc_eval.argtypes = [ctypes.c_int, _i, ctypes.c_double, ctypes.c_long, _d, _d, _d, _d, _d]
+        L.orc_vxc_from_collocation.argtypes = [ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d, _d,
+                                               ctypes.c_int, _d, _d, ctypes.c_int, _i,
+                                               ctypes.c_double, _d, _d, _d]
+        L.orc_overlap.argtypes = [ctypes.c_int, ctypes.c_long, _d, _d, _d]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_d):
+    if a is None:
+        return ctypes.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def radial(scheme, n, R, alpha=0.6):
+    r = np.empty(n)
+    w = np.empty(n)
+    if lib().orc_radial(scheme, n, R, alpha, _p(r), _p(w)):
+        raise ValueError("bad radial scheme")
+    return r, w
+
+
+def radial_scale(scheme, Z):
+    return lib().orc_radial_scale(scheme, Z)
+
+
+def becke(centers, pts, owner, w, capped=True):
+    centers = _f64(centers)
+    pts = _f64(pts)
+    owner = _i32(owner)
+    w = _f64(w).copy()
+    lib().orc_becke(len(centers), _p(centers), len(w), _p(pts), _p(owner, _i), _p(w), int(capped))
+    return w
+
+
+def make_flat_grid(xyz, Z, nrad, ang, scheme=0, alpha=0.6, w_thr=1e-30, capped=True,
+                   pyscf_sizing=False):
+    """ang: (nang, 4) array of x, y, z, weight (sum 4 pi)."""
+    xyz = _f64(xyz)
+    Z = _i32(Z)
+    ang = _f64(ang)
+    nmax = len(Z) * int(nrad * (2.1 if pyscf_sizing else 1) + 1) * len(ang)
+    P = np.empty((nmax, 3))
+    W = np.empty(nmax)
+    O = np.empty(nmax, dtype=np.int32)
+    n = lib().orc_make_flat_grid(len(Z), _p(xyz), _p(Z, _i), nrad, len(ang), _p(ang), scheme,
+                                 alpha, w_thr, int(capped), int(pyscf_sizing), _p(P), _p(W),
+                                 _p(O, _i))
+    if n < 0:
+        raise RuntimeError("Detected negative weights aborting")
+    return P[:n].copy(), W[:n].copy(), O[:n].copy()
+
+
+def sto_norm_cutoff(n, zeta, tol=1e-10):
+    a = ctypes.c_double()
+    b = ctypes.c_double()
+    lib().orc_sto_norm_cutoff(n, zeta, tol, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def solid_harmonic(l, m, x, y, z):
+    out = np.empty(4)
+    if lib().orc_solid_harmonic(l, m, x, y, z, _p(out)):
+        raise ValueError("bad (l, m)")
+    return out
+
+
+def collocation(basis, pts, grad=True):
+    """basis: dict with n, l, m, zeta, norm, centers (nbf, 3).  Returns
+    phi (nbf, npts) and, if grad, gx, gy, gz of the same shape."""
+    n, l, m = _i32(basis["n"]), _i32(basis["l"]), _i32(basis["m"])
+    zeta, norm, O = _f64(basis["zeta"]), _f64(basis["norm"]), _f64(basis["centers"])
+    pts = _f64(pts)
+    nbf, npts = len(n), len(pts)
+    phi = np.empty((nbf, npts))
+    gx = np.empty((nbf, npts)) if grad else None
+    gy = np.empty((nbf, npts)) if grad else None
+    gz = np.empty((nbf, npts)) if grad else None
+    lib().orc_collocation(nbf, _p(n, _i), _p(l, _i), _p(m, _i), _p(zeta), _p(norm), _p(O), npts,
+                          _p(pts), _p(phi), _p(gx), _p(gy), _p(gz))
+    return (phi, gx, gy, gz) if grad else phi
+
+
+def density_P(phi, P, gx=None, gy=None, gz=None):
+    nbf, npts = phi.shape
+    rho = np.empty(npts)
+    grad = np.empty((npts, 3)) if gx is not None else None
+    sig = np.empty(npts) if gx is not None else None
+    lib().orc_density_P(nbf, npts, _p(_f64(phi)), _p(gx), _p(gy), _p(gz), _p(_f64(P)), _p(rho),
+                        _p(grad), _p(sig))
+    return rho, grad, sig
+
+
+def density_C(phi, C, f, gx=None, gy=None, gz=None):
+    nbf, npts = phi.shape
+    C = _f64(C)
+    f = _f64(f)
+    rho = np.empty(npts)
+    grad = np.empty((npts, 3)) if gx is not None else None
+    sig = np.empty(npts) if gx is not None else None
+    lib().orc_density_C(nbf, C.shape[1], npts, _p(_f64(phi)), _p(gx), _p(gy), _p(gz), _p(C),
+                        _p(f), _p(rho), _p(grad), _p(sig))
+    return rho, grad, sig
+
+
+def xc_eval(ids, rho, sigma=None, dens_thr=0.0):
+    ids = _i32(ids)
+    rho = _f64(rho)
+    sigma = None if sigma is None else _f64(sigma)
+    n = len(rho)
+    exc, vr, vs = np.empty(n), np.empty(n), np.empty(n)
+    if lib().orc_xc_eval(len(ids), _p(ids, _i), dens_thr, n, _p(rho), _p(sigma), _p(exc), _p(vr),
+                         _p(vs)):
+        raise ValueError(f"unknown functional id in {list(ids)}")
+    return exc, vr, vs
+
+
+def vxc_from_collocation(phi, w, ids, P=None, C=None, f=None, gx=None, gy=None, gz=None,
+                         dens_thr=0.0, chunk=None):
+    """E_xc, N_el, V_xc from collocation data (compute_lda / compute_gga).
+    Either P (nbf, nbf) or (C (nbf, nocc), f (nocc,)).  Columns (points) may be
+    processed in chunks; results are accumulated in FP64."""
+    nbf, npts = phi.shape
+    ids = _i32(ids)
+    E = ctypes.c_double(0.0)
+    N = ctypes.c_double(0.0)
+    V = np.zeros((nbf, nbf))
+    C = None if C is None else _f64(C)
+    f = None if f is None else _f64(f)
+    P = None if P is None else _f64(P)
+    nocc = 0 if C is None else C.shape[1]
+    step = chunk or npts
+    for s in range(0, npts, step):
+        e = slice(s, min(npts, s + step))
+        a = [_f64(x[:, e]) if x is not None else None for x in (phi, gx, gy, gz)]
+        ww = _f64(w[e])
+        rc = lib().orc_vxc_from_collocation(nbf, ww.shape[0], _p(a[0]), _p(a[1]), _p(a[2]),
+                                            _p(a[3]), _p(ww), _p(P), nocc, _p(C), _p(f), len(ids),
+                                            _p(ids, _i), dens_thr, ctypes.byref(E),
+                                            ctypes.byref(N), _p(V))
+        if rc:
+            raise ValueError(f"oracle vxc failed (code {rc})")
+    return E.value, N.value, V
+
+
+def overlap(phi, w):
+    nbf, npts = phi.shape
+    S = np.zeros((nbf, nbf))
+    lib().orc_overlap(nbf, npts, _p(_f64(phi)), _p(_f64(w)), _p(S))
+    return S
