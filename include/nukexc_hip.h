@@ -1,0 +1,206 @@
+/*
+ * nukexc_hip.h -- C ABI of the MI355X-native NuKEXC exchange-correlation path.
+ *
+ * The reference is a header-only C++ library over Kokkos views; it has no ABI.
+ * This header is the stable boundary a host driver binds instead (C++ through
+ * the headers under include/nukexc/, or any FFI: ctypes, cgo, JNI).  Each entry point names
+ * the reference interface it replaces (paths relative to the NuKEXC tree).
+ *
+ * Conventions
+ *  - Every function returns an nkx_status; no C++ exception crosses the ABI.
+ *    nkx_last_error(ctx) gives the message of the last failure on that ctx
+ *    (ctx == NULL: the calling thread's last failure, e.g. from nkx_create).
+ *  - "_dev" arguments are device pointers on the ctx's GPU; "_host" arguments
+ *    are host pointers.  Device work is ordered on the ctx's stream; calls
+ *    that return host scalars (energies, point counts) synchronise that stream.
+ *  - A ctx is bound to one device and one stream and owns all workspaces, so
+ *    repeated V_xc builds (one per SCF iteration) reuse memory and launch
+ *    plans.  One ctx per host thread.
+ *  - Matrices are row-major unless stated.  Collocation arrays passed to the
+ *    reference-shaped entry points use the reference's LayoutLeft (N_bf, N_pts)
+ *    view: element (mu, g) at mu + g * N_bf.
+ *  - Functional ids follow libxc numbering: LDA_X = 1, LDA_C_PW = 12,
+ *    LDA_C_PW_MOD = 13, GGA_X_PBE = 101, GGA_C_PBE = 130
+ *    (tests/standalone_helpers.hpp:184-225).
+ */
+#ifndef NUKEXC_HIP_H
+#define NUKEXC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nkx_ctx nkx_ctx;
+
+typedef enum nkx_status {
+  NKX_OK = 0,
+  NKX_ERR_INVALID_ARGUMENT = 1,
+  NKX_ERR_HIP = 2,
+  NKX_ERR_OUT_OF_MEMORY = 3,
+  NKX_ERR_FUNCTIONAL_FAMILY = 4, /* xc_integrals.hpp:81-84, :434-438 */
+  NKX_ERR_NSPIN = 5,             /* xc_integrals.hpp:86-90 */
+  NKX_ERR_NEGATIVE_WEIGHT = 6,   /* grid.hpp:284-286 */
+  NKX_ERR_UNKNOWN_FUNCTIONAL = 7,
+  NKX_ERR_STATE = 8,             /* call-order errors (no grid / basis yet) */
+  NKX_ERR_UNSUPPORTED = 9
+} nkx_status;
+
+typedef enum nkx_precision {
+  NKX_PREC_FP64 = 0,  /* FP64 collocation and FP64 MFMA contractions */
+  NKX_PREC_MIXED = 1  /* FP32 collocation, split-BF16 MFMA contractions with
+                         FP32 accumulation; FP64 functional and reductions */
+} nkx_precision;
+
+typedef enum nkx_radial_scheme {
+  NKX_RADIAL_TREUTLER_AHLRICHS = 0, /* grid.hpp:104-108, TA_XI :98-102 */
+  NKX_RADIAL_BECKE = 1              /* half Bragg-Slater radius, :71-96 */
+} nkx_radial_scheme;
+
+typedef enum nkx_radial_sizing {
+  NKX_SIZING_UNIFORM = 0, /* grid.hpp:129-133 */
+  NKX_SIZING_PYSCF = 1    /* grid.hpp:135-148 */
+} nkx_radial_sizing;
+
+typedef enum nkx_family { NKX_FAMILY_LDA = 1, NKX_FAMILY_GGA = 2 } nkx_family;
+
+/* ---------------------------------------------------------------- context */
+/* hip_stream == NULL: the ctx creates its own blocking stream, which is
+   ordered with the legacy default stream. */
+int nkx_create(int device, void *hip_stream, nkx_ctx **out);
+int nkx_destroy(nkx_ctx *ctx);
+int nkx_sync(nkx_ctx *ctx);
+const char *nkx_last_error(const nkx_ctx *ctx);
+int nkx_version(int *major, int *minor, int *patch);
+
+/* Device memory helpers so C/C++ callers need no HIP headers
+   (the reference's Kokkos::View allocation / deep_copy). */
+int nkx_malloc(nkx_ctx *ctx, size_t bytes, void **ptr_dev);
+int nkx_free(nkx_ctx *ctx, void *ptr_dev);
+int nkx_memcpy_h2d(nkx_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int nkx_memcpy_d2h(nkx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+int nkx_memset(nkx_ctx *ctx, void *dst_dev, int value, size_t bytes);
+
+/* ------------------------------------------------------- angular / radial */
+/* Lebedev rule for the smallest tabulated algebraic order >= order
+   (IntegratorXX next_algebraic_order, grid.hpp:186-187). */
+int nkx_lebedev_npts(int order, int *npts, int *actual_order);
+/* Fills xyzw_host (npts rows of x, y, z, w; sum w = 4 pi). */
+int nkx_lebedev_rule(int order, double *xyzw_host);
+/* Radial rule with r^2 and the Jacobian folded into w (IntegratorXX radial
+   quadratures called at grid.hpp:247-251). */
+int nkx_radial_rule(int scheme, int n, double R, double alpha, double *r_host,
+                    double *w_host);
+/* Per-element radial scale (grid.hpp:227-235). */
+int nkx_radial_scale(int scheme, int Z, double *R);
+
+/* ------------------------------------------------------------------- grid */
+/* make_flat_grid<TA|Becke, LebedevLaikov>(mol, nrad, nang_order, w_thr,
+   ta_alpha, Unpruned, sizing)  (grid.hpp:172-348).  The grid is kept in the
+   ctx: atom-major, radial, angular order; Becke-partitioned with the capped
+   partition (partition_becke_team); points with w <= w_thr removed by an
+   order-preserving compaction.  Returns the number of points. */
+int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz_host,
+                       const int *Z_host, int nrad, int nang_order, int radial_scheme,
+                       double ta_alpha, double weight_threshold, int radial_sizing,
+                       int64_t *npts);
+/* Replace the ctx grid with caller data (device pointers, copied). */
+int nkx_set_grid(nkx_ctx *ctx, int natoms, const double *centers_dev, int64_t npts,
+                 const double *points_dev, const double *weights_dev,
+                 const int *owner_dev);
+int nkx_grid_npts(nkx_ctx *ctx, int64_t *npts);
+/* Copy the ctx grid out: points (npts x 3), weights, owners (any may be NULL). */
+int nkx_grid_export(nkx_ctx *ctx, double *points_dev, double *weights_dev,
+                    int *owner_dev);
+
+/* partition_becke (capped == 0, partitioning.hpp:63-120) and
+   partition_becke_team (capped != 0, :126-213): weights_dev[g] *=
+   P_owner(g) / sum_i P_i on caller arrays.  points_dev is npts x 3. */
+int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers_dev, int64_t npts,
+              const double *points_dev, const int *owner_dev, double *weights_dev,
+              int capped);
+
+/* ------------------------------------------------------------------ basis */
+/* make_manual_basis normalisation and compute_cutoff radius
+   (stobasis.hpp:90-173, :200-201). */
+int nkx_sto_norm_cutoff(int n, double zeta, double tol, double *norm, double *cutoff);
+/* STOBasisSet (stobasis.hpp:54-64) for the ctx; host arrays, copied.
+   l <= 7. centers_host is nbf x 3. */
+int nkx_set_basis(nkx_ctx *ctx, int nbf, const int *n, const int *l, const int *m,
+                  const double *zeta, const double *norm, const double *centers_host,
+                  const double *cutoff);
+int nkx_basis_nbf(nkx_ctx *ctx, int *nbf);
+/* Real solid harmonic S_lm = r^l Y_lm and its gradient at (x, y, z), l <= 7
+   (real_solid_harmonic_cart_and_grad_precomputed, spherical_harmonics.hpp:
+   750-1608); host.  out4 = {S, dS/dx, dS/dy, dS/dz}. */
+int nkx_solid_harmonic(int l, int m, double x, double y, double z, double *out4);
+
+/* ------------------------------------------------------------ collocation */
+/* fill_collocation / fill_grad_collocation (stobasis.hpp:494-584) on caller
+   points (npts x 3): LayoutLeft (N_bf, N_pts) outputs; grad pointers may be
+   NULL.  FP64. */
+int nkx_collocation(nkx_ctx *ctx, int64_t npts, const double *points_dev,
+                    double *phi_dev, double *gx_dev, double *gy_dev, double *gz_dev);
+
+/* --------------------------------------------------------------- density */
+/* compute_density / compute_density_and_sigma (density.hpp:150-255) on
+   caller collocation arrays (LayoutLeft (N_bf, N_pts)) and orbitals
+   C (LayoutLeft (N_bf, N_occ)), occupations f.  gx/gy/gz NULL: rho only. */
+int nkx_density(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi_dev,
+                const double *gx_dev, const double *gy_dev, const double *gz_dev,
+                int nocc, const double *C_dev, const double *f_dev, double *rho_dev,
+                double *grx_dev, double *gry_dev, double *grz_dev, double *sigma_dev);
+
+/* ------------------------------------------------------------- functional */
+/* libxc family of an id (func.info->family); kind 0 exchange, 1 correlation. */
+int nkx_functional_info(int id, int *family, int *kind);
+/* Sets the density threshold (libxc xc_func_set_dens_threshold, used by
+   standalone.cxx:336).  thr <= 0 restores each functional's default. */
+int nkx_set_dens_threshold(nkx_ctx *ctx, double thr);
+/* Sum over the listed functionals of exc (per particle), vrho, vsigma on
+   device arrays (the host libxc calls at xc_integrals.hpp:113, :471-472, done
+   on the GPU).  sigma/vsigma may be NULL for an LDA-only list. */
+int nkx_functional_eval(nkx_ctx *ctx, int nfunc, const int *func_ids, int64_t npts,
+                        const double *rho_dev, const double *sigma_dev,
+                        double *exc_dev, double *vrho_dev, double *vsigma_dev);
+
+/* ----------------------------------------------------- reference-shaped XC */
+/* compute_lda (gx == NULL, xc_integrals.hpp:73-166) and compute_gga
+   (:424-541): E_xc and V_xc (row-major N_bf x N_bf, device) from caller
+   collocation arrays, weights, orbitals and occupations.  family_check:
+   0 = no check, 1 = require LDA functionals, 2 = require GGA functionals. */
+int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi_dev,
+                            const double *gx_dev, const double *gy_dev,
+                            const double *gz_dev, const double *w_dev, int nocc,
+                            const double *C_dev, const double *f_dev, int nfunc,
+                            const int *func_ids, int family_check, double *E_xc,
+                            double *V_dev);
+
+/* ---------------------------------------------------------------- hot path */
+/* One V_xc build on the ctx grid and basis (the SCF-iteration XC call,
+   tests/standalone.cxx:724-741 -> compute_gga): density from the density
+   matrix P (row-major N_bf x N_bf, symmetric, device), on-device functional,
+   E_xc = sum_g w rho eps, N_el = sum_g w rho, and
+   V_xc = Phi^T diag(w v) Phi (+ GGA terms), row-major N_bf x N_bf (device).
+   Points are processed in spatially compact batches against screened
+   function lists; precision selects the FP64 or the mixed form. */
+int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
+            int precision, double *E_xc, double *N_el, double *V_dev);
+/* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
+   and the point-batch size (64 or 128; 0 keeps the current value). */
+int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);
+/* Statistics of the current plan: number of points, padded points, batches,
+   screened (function-block, batch) pairs kept, total pairs. */
+int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npts_padded, int64_t *nbatches,
+                   int64_t *pairs_kept, int64_t *pairs_total);
+/* Numerical overlap S = Phi diag(w) Phi^T on the ctx grid (FP64, row-major,
+   device); core_hamiltonian.hpp:62-119 overlap_integral. */
+int nkx_overlap(nkx_ctx *ctx, double *S_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUKEXC_HIP_H */

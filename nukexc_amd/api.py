@@ -1,0 +1,378 @@
+"""Host-side mirror of the reference's nukexc/ API over the C ABI.
+
+Names, argument meaning and errors follow the reference headers
+(grid.hpp, partitioning.hpp, stobasis.hpp, density.hpp, xc_integrals.hpp);
+device arrays are torch tensors on the ctx's GPU (torch is only the
+allocator and stream provider here).  Reference errors raise NukexcError
+whose status names the failure (e.g. NKX_ERR_FUNCTIONAL_FAMILY where the
+reference throws std::runtime_error at xc_integrals.hpp:81-84).
+
+Layout: the reference's LayoutLeft (N_bf, N_pts) collocation view is a torch
+tensor of shape (N_bf, N_pts) with strides (1, N_bf) -- the transpose of a
+contiguous (N_pts, N_bf) tensor -- so phi[mu, g] indexes like the reference.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from ._capi import NukexcError, check, lib
+
+try:  # torch provides device memory and the stream
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+FUNCTIONALS = {"LDA_X": 1, "LDA_C_PW": 12, "LDA_C_PW_MOD": 13, "GGA_X_PBE": 101,
+               "GGA_C_PBE": 130}
+PBE = (101, 130)
+TA_M4 = 0.6
+TA_M3 = 0.0
+
+
+def _ids(funcs):
+    if isinstance(funcs, (int, np.integer, str)):
+        funcs = [funcs]
+    out = [FUNCTIONALS[f] if isinstance(f, str) else int(f) for f in funcs]
+    return (ctypes.c_int * len(out))(*out), len(out)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _dev_f64(x, device):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=device)
+
+
+# ------------------------------------------------------------------ types ----
+@dataclass
+class STOBasisSet:
+    """stobasis.hpp:54-64 (host arrays; uploaded by Context.set_basis)."""
+    n: np.ndarray
+    l: np.ndarray
+    m: np.ndarray
+    norm: np.ndarray
+    zeta: np.ndarray
+    O: np.ndarray  # (nbf, 3) centres
+    cutoff_radii: np.ndarray
+
+    def nbf(self):
+        return len(self.n)
+
+
+@dataclass
+class FlatGrid:
+    """grid.hpp:150-160: device arrays."""
+    quad_points: "torch.Tensor"  # (npts, 3)
+    weights: "torch.Tensor"
+    atom_centers: "torch.Tensor"  # (natoms, 3)
+    Z: np.ndarray
+    point_owner: "torch.Tensor"
+
+
+@dataclass
+class XC_result:
+    """xc_integrals.hpp:62-65."""
+    energy: float
+    potential: "torch.Tensor"  # (N_bf, N_bf)
+    n_el: float = float("nan")
+
+
+def sto_norm_cutoff(n, zeta, tol=1e-10):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    check(lib().nkx_sto_norm_cutoff(int(n), float(zeta), float(tol), ctypes.byref(a),
+                                    ctypes.byref(b)))
+    return a.value, b.value
+
+
+def make_manual_basis(funcs, cutoff_tol=1e-10):
+    """make_manual_basis (stobasis.hpp:175-225).  funcs: iterable of
+    (n, l, m, zeta, ox, oy, oz)."""
+    funcs = list(funcs)
+    n = np.array([f[0] for f in funcs], dtype=np.int32)
+    l = np.array([f[1] for f in funcs], dtype=np.int32)
+    m = np.array([f[2] for f in funcs], dtype=np.int32)
+    zeta = np.array([f[3] for f in funcs], dtype=np.float64)
+    O = np.array([[f[4], f[5], f[6]] for f in funcs], dtype=np.float64).reshape(-1, 3)
+    nc = [sto_norm_cutoff(a, z, cutoff_tol) for a, z in zip(n, zeta)]
+    norm = np.array([x[0] for x in nc])
+    cut = np.array([x[1] for x in nc])
+    return STOBasisSet(n, l, m, norm, zeta, O, cut)
+
+
+def lebedev_rule(order):
+    npts, actual = ctypes.c_int(), ctypes.c_int()
+    check(lib().nkx_lebedev_npts(int(order), ctypes.byref(npts), ctypes.byref(actual)))
+    out = np.empty((npts.value, 4))
+    check(lib().nkx_lebedev_rule(int(order), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def radial_rule(scheme, n, R, alpha=TA_M4):
+    r, w = np.empty(n), np.empty(n)
+    pd = ctypes.POINTER(ctypes.c_double)
+    check(lib().nkx_radial_rule(int(scheme), int(n), float(R), float(alpha),
+                                r.ctypes.data_as(pd), w.ctypes.data_as(pd)))
+    return r, w
+
+
+def radial_scale(scheme, Z):
+    R = ctypes.c_double()
+    check(lib().nkx_radial_scale(int(scheme), int(Z), ctypes.byref(R)))
+    return R.value
+
+
+def solid_harmonic(l, m, x, y, z):
+    """S_lm = r^l Y_lm and gradient (spherical_harmonics.hpp:750-1608)."""
+    out = np.empty(4)
+    check(lib().nkx_solid_harmonic(int(l), int(m), float(x), float(y), float(z),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def functional_info(fid):
+    fam, kind = ctypes.c_int(), ctypes.c_int()
+    check(lib().nkx_functional_info(int(fid), ctypes.byref(fam), ctypes.byref(kind)))
+    return fam.value, kind.value
+
+
+# ---------------------------------------------------------------- context ----
+class Context:
+    """An nkx_ctx bound to one GPU and torch's current stream on it."""
+
+    def __init__(self, device=0):
+        if torch is None or not torch.cuda.is_available():
+            raise RuntimeError("nukexc_amd needs a ROCm GPU (torch.cuda is unavailable)")
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        h = ctypes.c_void_p()
+        # stream 0 (torch's legacy default stream): the ctx makes a blocking
+        # stream of its own, which is ordered with it.
+        check(lib().nkx_create(device, ctypes.c_void_p(stream) if stream else None,
+                               ctypes.byref(h)))
+        self.h = h
+        self.basis = None
+        self.grid = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nkx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        return check(rc, self.h)
+
+    def _done(self, rc):
+        """Check a call that leaves device outputs and wait for them."""
+        check(rc, self.h)
+        self.sync()
+
+    def sync(self):
+        self._chk(lib().nkx_sync(self.h))
+
+    # -- grid -----------------------------------------------------------------
+    def make_flat_grid(self, xyz, Z, nrad=50, nang_order=30, weight_threshold=1e-30,
+                       ta_alpha=TA_M4, radial="ta", sizing="uniform"):
+        """make_flat_grid<TA|Becke, LebedevLaikov> (grid.hpp:172-348)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        Zc = np.ascontiguousarray(Z, dtype=np.int32)
+        n = ctypes.c_int64()
+        scheme = {"ta": _capi.RADIAL_TA, "becke": _capi.RADIAL_BECKE}[radial]
+        self._chk(lib().nkx_make_flat_grid(
+            self.h, len(Zc), xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            Zc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(nrad), int(nang_order), scheme,
+            float(ta_alpha), float(weight_threshold), {"uniform": 0, "pyscf": 1}[sizing],
+            ctypes.byref(n)))
+        npts = n.value
+        pts = torch.empty((npts, 3), dtype=torch.float64, device=self.device)
+        w = torch.empty(npts, dtype=torch.float64, device=self.device)
+        own = torch.empty(npts, dtype=torch.int32, device=self.device)
+        self._chk(lib().nkx_grid_export(self.h, _ptr(pts), _ptr(w), _ptr(own)))
+        self.sync()
+        self.grid = FlatGrid(pts, w, torch.as_tensor(xyz, device=self.device), Zc, own)
+        return self.grid
+
+    def set_grid(self, grid):
+        pts = grid.quad_points.contiguous()
+        w = grid.weights.contiguous()
+        own = grid.point_owner.to(torch.int32).contiguous()
+        ac = grid.atom_centers.to(torch.float64).contiguous()
+        self._done(lib().nkx_set_grid(self.h, ac.shape[0], _ptr(ac), pts.shape[0], _ptr(pts),
+                                     _ptr(w), _ptr(own)))
+        self.grid = grid
+
+    # -- basis ----------------------------------------------------------------
+    def set_basis(self, basis):
+        pi = ctypes.POINTER(ctypes.c_int)
+        pd = ctypes.POINTER(ctypes.c_double)
+        n = np.ascontiguousarray(basis.n, np.int32)
+        l = np.ascontiguousarray(basis.l, np.int32)
+        m = np.ascontiguousarray(basis.m, np.int32)
+        z = np.ascontiguousarray(basis.zeta, np.float64)
+        nr = np.ascontiguousarray(basis.norm, np.float64)
+        O = np.ascontiguousarray(basis.O, np.float64)
+        cut = np.ascontiguousarray(basis.cutoff_radii, np.float64)
+        self._chk(lib().nkx_set_basis(self.h, len(n), n.ctypes.data_as(pi), l.ctypes.data_as(pi),
+                                      m.ctypes.data_as(pi), z.ctypes.data_as(pd),
+                                      nr.ctypes.data_as(pd), O.ctypes.data_as(pd),
+                                      cut.ctypes.data_as(pd)))
+        self.basis = basis
+
+    def set_dens_threshold(self, thr):
+        self._chk(lib().nkx_set_dens_threshold(self.h, float(thr)))
+
+    def set_screening(self, tol, batch_points=0):
+        self._chk(lib().nkx_set_screening(self.h, float(tol), int(batch_points)))
+
+    def plan_stats(self):
+        v = [ctypes.c_int64() for _ in range(5)]
+        self._chk(lib().nkx_plan_stats(self.h, *[ctypes.byref(x) for x in v]))
+        keys = ("npts", "npts_padded", "nbatches", "pairs_kept", "pairs_total")
+        return dict(zip(keys, [x.value for x in v]))
+
+    # -- hot path -------------------------------------------------------------
+    def vxc(self, P, funcs=PBE, precision="fp64", out=None):
+        """One V_xc build on the ctx grid and basis from a density matrix P.
+        Returns XC_result(energy, potential, n_el)."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        P = _dev_f64(P, self.device).contiguous()
+        nbf = P.shape[0]
+        V = out if out is not None else torch.empty((nbf, nbf), dtype=torch.float64,
+                                                    device=self.device)
+        ids, nf = _ids(funcs)
+        E, N = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().nkx_vxc(self.h, _ptr(P), nf, ids, prec, ctypes.byref(E), ctypes.byref(N),
+                                _ptr(V)))
+        return XC_result(E.value, V, N.value)
+
+    def overlap(self):
+        """Numerical overlap on the ctx grid (core_hamiltonian.hpp:62-119)."""
+        nbf = self.basis.nbf()
+        S = torch.empty((nbf, nbf), dtype=torch.float64, device=self.device)
+        self._chk(lib().nkx_overlap(self.h, _ptr(S)))
+        self.sync()
+        return S
+
+
+# ------------------------------------------------- reference-shaped calls ----
+def partition_becke(ctx, atom_centers, quad_points, point_owner, weights, capped=False):
+    """partition_becke (partitioning.hpp:63-120); in place on weights."""
+    ac = atom_centers.contiguous()
+    qp = quad_points.contiguous()
+    own = point_owner.to(torch.int32).contiguous()
+    assert weights.is_contiguous() and weights.dtype == torch.float64
+    ctx._done(lib().nkx_becke(ctx.h, ac.shape[0], _ptr(ac), qp.shape[0], _ptr(qp), _ptr(own),
+                             _ptr(weights), int(bool(capped))))
+    return weights
+
+
+def partition_becke_team(ctx, atom_centers, quad_points, point_owner, weights):
+    """partition_becke_team (partitioning.hpp:126-213): R_ij capped at 5."""
+    return partition_becke(ctx, atom_centers, quad_points, point_owner, weights, capped=True)
+
+
+def _colloc_out(nbf, npts, device):
+    # LayoutLeft (N_bf, N_pts): transpose view of a contiguous (N_pts, N_bf)
+    return torch.empty((npts, nbf), dtype=torch.float64, device=device).t()
+
+
+def fill_collocation(ctx, basis, points):
+    """fill_collocation (stobasis.hpp:494-532)."""
+    if ctx.basis is not basis:
+        ctx.set_basis(basis)
+    pts = points.contiguous()
+    phi = _colloc_out(basis.nbf(), pts.shape[0], ctx.device)
+    ctx._done(lib().nkx_collocation(ctx.h, pts.shape[0], _ptr(pts), _ptr(phi), None, None, None))
+    return phi
+
+
+def fill_grad_collocation(ctx, basis, points):
+    """fill_grad_collocation (stobasis.hpp:534-584): returns gx, gy, gz."""
+    if ctx.basis is not basis:
+        ctx.set_basis(basis)
+    pts = points.contiguous()
+    g = [_colloc_out(basis.nbf(), pts.shape[0], ctx.device) for _ in range(3)]
+    ctx._done(lib().nkx_collocation(ctx.h, pts.shape[0], _ptr(pts), None, _ptr(g[0]), _ptr(g[1]),
+                                   _ptr(g[2])))
+    return tuple(g)
+
+
+def _left(t):
+    """Check a (rows, cols) LayoutLeft view: the transpose of a contiguous tensor."""
+    if not t.t().is_contiguous():
+        t = t.t().contiguous().t()
+    return t
+
+
+def compute_density(ctx, phi, mo_orbitals, mo_coeff):
+    """compute_density (density.hpp:150-186)."""
+    phi = _left(phi)
+    C = _left(_dev_f64(mo_orbitals, ctx.device).reshape(phi.shape[0], -1))
+    f = _dev_f64(mo_coeff, ctx.device).contiguous()
+    rho = torch.empty(phi.shape[1], dtype=torch.float64, device=ctx.device)
+    ctx._done(lib().nkx_density(ctx.h, phi.shape[0], phi.shape[1], _ptr(phi), None, None, None,
+                               C.shape[1], _ptr(C), _ptr(f), _ptr(rho), None, None, None, None))
+    return rho
+
+
+def compute_density_and_sigma(ctx, phi, gx, gy, gz, mo_orbitals, mo_coeff):
+    """compute_density_and_sigma (density.hpp:188-255): rho, grad rho (3), sigma."""
+    phi, gx, gy, gz = (_left(x) for x in (phi, gx, gy, gz))
+    C = _left(_dev_f64(mo_orbitals, ctx.device).reshape(phi.shape[0], -1))
+    f = _dev_f64(mo_coeff, ctx.device).contiguous()
+    n = phi.shape[1]
+    out = [torch.empty(n, dtype=torch.float64, device=ctx.device) for _ in range(5)]
+    ctx._done(lib().nkx_density(ctx.h, phi.shape[0], n, _ptr(phi), _ptr(gx), _ptr(gy), _ptr(gz),
+                               C.shape[1], _ptr(C), _ptr(f), *[_ptr(o) for o in out]))
+    return out[0], (out[1], out[2], out[3]), out[4]
+
+
+def functional_eval(ctx, funcs, rho, sigma=None):
+    """exc, vrho, vsigma on device (the libxc calls of xc_integrals.hpp)."""
+    ids, nf = _ids(funcs)
+    n = rho.shape[0]
+    exc, vr = (torch.empty(n, dtype=torch.float64, device=ctx.device) for _ in range(2))
+    vs = torch.empty(n, dtype=torch.float64, device=ctx.device) if sigma is not None else None
+    ctx._done(lib().nkx_functional_eval(ctx.h, nf, ids, n, _ptr(rho.contiguous()),
+                                       _ptr(None if sigma is None else sigma.contiguous()),
+                                       _ptr(exc), _ptr(vr), _ptr(vs)))
+    return exc, vr, vs
+
+
+def _xc(ctx, phi, grads, weights, mo_orbitals, mo_coeff, funcs, family):
+    phi = _left(phi)
+    nbf, n = phi.shape
+    g = [None, None, None] if grads is None else [_left(x) for x in grads]
+    C = _left(_dev_f64(mo_orbitals, ctx.device).reshape(nbf, -1))
+    f = _dev_f64(mo_coeff, ctx.device).contiguous()
+    w = weights.contiguous()
+    V = torch.empty((nbf, nbf), dtype=torch.float64, device=ctx.device)
+    E = ctypes.c_double()
+    ids, nf = _ids(funcs)
+    ctx._chk(lib().nkx_xc_from_collocation(ctx.h, nbf, n, _ptr(phi), _ptr(g[0]), _ptr(g[1]),
+                                           _ptr(g[2]), _ptr(w), C.shape[1], _ptr(C), _ptr(f), nf,
+                                           ids, family, ctypes.byref(E), _ptr(V)))
+    return XC_result(E.value, V)
+
+
+def compute_lda(ctx, collocation_values, weights, mo_orbitals, mo_coeff, funcs):
+    """compute_lda (xc_integrals.hpp:73-166)."""
+    return _xc(ctx, collocation_values, None, weights, mo_orbitals, mo_coeff, funcs,
+               _capi.FAMILY_LDA)
+
+
+def compute_gga(ctx, collocation_values, gx, gy, gz, weights, mo_orbitals, mo_coeff, funcs):
+    """compute_gga (xc_integrals.hpp:424-541)."""
+    return _xc(ctx, collocation_values, (gx, gy, gz), weights, mo_orbitals, mo_coeff, funcs,
+               _capi.FAMILY_GGA)

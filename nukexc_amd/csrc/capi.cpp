@@ -1,0 +1,566 @@
+// extern "C" entry points of libnukexc_hip.so (include/nukexc_hip.h).
+// Every call converts internal errors to an nkx_status and records the
+// message on the ctx; no exception leaves this file.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "nkx_internal.hpp"
+
+struct nkx_ctx {
+  nkx::Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+template <typename F>
+int guarded(nkx_ctx *ctx, F &&fn) {
+  try {
+    if (ctx) NKX_HIP_CHECK(hipSetDevice(ctx->c.device));
+    fn();
+    return NKX_OK;
+  } catch (const nkx::Error &e) {
+    (ctx ? ctx->c.last_error : g_thread_error) = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    (ctx ? ctx->c.last_error : g_thread_error) = "host allocation failed";
+    return NKX_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    (ctx ? ctx->c.last_error : g_thread_error) = e.what();
+    return NKX_ERR_INVALID_ARGUMENT;
+  }
+}
+
+#define NKX_CTX_CHECK(ctx)                                                           \
+  do {                                                                               \
+    if (!(ctx)) {                                                                    \
+      g_thread_error = "null nkx_ctx";                                               \
+      return NKX_ERR_INVALID_ARGUMENT;                                               \
+    }                                                                                \
+  } while (0)
+
+int functional_info(int id, int *family, int *kind) {
+  switch (id) {
+    case 1: *family = NKX_FAMILY_LDA; *kind = 0; return 0;
+    case 12: case 13: *family = NKX_FAMILY_LDA; *kind = 1; return 0;
+    case 101: *family = NKX_FAMILY_GGA; *kind = 0; return 0;
+    case 130: *family = NKX_FAMILY_GGA; *kind = 1; return 0;
+    default: return 1;
+  }
+}
+
+double default_threshold(int id) { return id == 130 ? 1e-12 : 1e-15; }
+
+}  // namespace
+
+namespace nkx {
+bool xc_list_make(int nfunc, const int *ids, double thr_override, XcList *out, bool *any_gga) {
+  if (nfunc < 1 || nfunc > 4 || !ids) return false;
+  out->n = nfunc;
+  *any_gga = false;
+  for (int k = 0; k < nfunc; ++k) {
+    int fam, kind;
+    if (functional_info(ids[k], &fam, &kind)) return false;
+    out->ids[k] = ids[k];
+    out->thr[k] = thr_override > 0 ? thr_override : default_threshold(ids[k]);
+    if (fam == NKX_FAMILY_GGA) *any_gga = true;
+  }
+  for (int k = nfunc; k < 4; ++k) {
+    out->ids[k] = 0;
+    out->thr[k] = 1.0;
+  }
+  return true;
+}
+}  // namespace nkx
+
+using nkx::Error;
+
+extern "C" {
+
+int nkx_version(int *major, int *minor, int *patch) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  if (patch) *patch = 0;
+  return NKX_OK;
+}
+
+int nkx_create(int device, void *hip_stream, nkx_ctx **out) {
+  if (!out) {
+    g_thread_error = "nkx_create: out is NULL";
+    return NKX_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  nkx_ctx *ctx = nullptr;
+  int rc = guarded(nullptr, [&] {
+    int n = 0;
+    NKX_HIP_CHECK(hipGetDeviceCount(&n));
+    NKX_REQUIRE(device >= 0 && device < n, NKX_ERR_INVALID_ARGUMENT, "no such device");
+    NKX_HIP_CHECK(hipSetDevice(device));
+    ctx = new nkx_ctx();
+    ctx->c.device = device;
+    if (hip_stream) {
+      ctx->c.stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+      // A blocking stream: ordered with the legacy default stream that
+      // callers (and torch) use for their own copies.
+      NKX_HIP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamDefault));
+      ctx->c.own_stream = true;
+    }
+  });
+  if (rc != NKX_OK) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return NKX_OK;
+}
+
+int nkx_destroy(nkx_ctx *ctx) {
+  if (!ctx) return NKX_OK;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return NKX_OK;
+}
+
+int nkx_sync(nkx_ctx *ctx) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] { NKX_HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+const char *nkx_last_error(const nkx_ctx *ctx) {
+  return ctx ? ctx->c.last_error.c_str() : g_thread_error.c_str();
+}
+
+int nkx_malloc(nkx_ctx *ctx, size_t bytes, void **ptr) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(ptr, NKX_ERR_INVALID_ARGUMENT, "nkx_malloc: ptr is NULL");
+    *ptr = nullptr;
+    if (bytes) NKX_HIP_CHECK(hipMalloc(ptr, bytes));
+  });
+}
+
+int nkx_free(nkx_ctx *ctx, void *ptr) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    if (ptr) {
+      NKX_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+      NKX_HIP_CHECK(hipFree(ptr));
+    }
+  });
+}
+
+int nkx_memcpy_h2d(nkx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    if (!bytes) return;
+    NKX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    NKX_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int nkx_memcpy_d2h(nkx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    if (!bytes) return;
+    NKX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    NKX_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int nkx_memset(nkx_ctx *ctx, void *dst, int value, size_t bytes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    if (bytes) NKX_HIP_CHECK(hipMemsetAsync(dst, value, bytes, ctx->c.stream));
+  });
+}
+
+int nkx_lebedev_npts(int order, int *npts, int *actual) {
+  return guarded(nullptr, [&] { nkx::lebedev_lookup(order, npts, actual); });
+}
+
+int nkx_lebedev_rule(int order, double *xyzw) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(xyzw, NKX_ERR_INVALID_ARGUMENT, "xyzw is NULL");
+    int actual = 0;
+    nkx::lebedev_lookup(order, nullptr, &actual);
+    nkx::lebedev_expand(actual, xyzw);
+  });
+}
+
+int nkx_radial_rule(int scheme, int n, double R, double alpha, double *r, double *w) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(r && w, NKX_ERR_INVALID_ARGUMENT, "output is NULL");
+    nkx::radial_rule(scheme, n, R, alpha, r, w);
+  });
+}
+
+int nkx_radial_scale(int scheme, int Z, double *R) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(R, NKX_ERR_INVALID_ARGUMENT, "R is NULL");
+    NKX_REQUIRE(scheme == 0 || scheme == 1, NKX_ERR_INVALID_ARGUMENT, "unknown radial scheme");
+    *R = nkx::radial_scale(scheme, Z);
+  });
+}
+
+int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z, int nrad,
+                       int nang_order, int radial_scheme, double ta_alpha, double w_thr,
+                       int radial_sizing, int64_t *npts_out) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(natoms > 0 && xyz && Z && nrad > 0, NKX_ERR_INVALID_ARGUMENT,
+                "make_flat_grid: need atoms, coordinates, charges and nrad > 0");
+    NKX_REQUIRE(radial_sizing == 0 || radial_sizing == 1, NKX_ERR_INVALID_ARGUMENT,
+                "unknown radial sizing");
+    int nang = 0, actual = 0;
+    nkx::lebedev_lookup(nang_order, &nang, &actual);
+    std::vector<double> ang(4 * (size_t)nang);
+    nkx::lebedev_expand(actual, ang.data());
+    std::vector<double> P, W;
+    std::vector<int> O;
+    for (int a = 0; a < natoms; ++a) {
+      int nr = nrad;
+      if (radial_sizing == NKX_SIZING_PYSCF) {
+        static const double ratio[7] = {1.0, 1.5, 1.6, 1.8, 1.9, 2.0, 2.1};
+        nr = (int)(nrad * ratio[nkx::periodic_row(Z[a]) - 1] + 0.5);
+      }
+      std::vector<double> r(nr), wr(nr);
+      nkx::radial_rule(radial_scheme, nr, nkx::radial_scale(radial_scheme, Z[a]), ta_alpha,
+                       r.data(), wr.data());
+      for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < nang; ++j) {
+          P.push_back(xyz[3 * a] + r[i] * ang[4 * j]);
+          P.push_back(xyz[3 * a + 1] + r[i] * ang[4 * j + 1]);
+          P.push_back(xyz[3 * a + 2] + r[i] * ang[4 * j + 2]);
+          const double ww = wr[i] * ang[4 * j + 3];
+          NKX_REQUIRE(ww >= 0, NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
+          W.push_back(ww);
+          O.push_back(a);
+        }
+    }
+    const int64_t n = (int64_t)W.size();
+    hipStream_t s = c.stream;
+    nkx::DevBuf<double> p0, w0;
+    nkx::DevBuf<int> o0;
+    nkx::DevBuf<int64_t> cnt;
+    p0.upload(P.data(), P.size(), s);
+    w0.upload(W.data(), W.size(), s);
+    o0.upload(O.data(), O.size(), s);
+    c.grid.centers_host.assign(xyz, xyz + 3 * natoms);
+    c.grid.centers.upload(xyz, 3 * (size_t)natoms, s);
+    c.grid.natoms = natoms;
+    nkx::launch_becke(s, natoms, c.grid.centers.p, n, p0.p, o0.p, w0.p, 1);
+    cnt.reserve((n + 1023) / 1024 + 1);
+    c.grid.pts.reserve(3 * n);
+    c.grid.w.reserve(n);
+    c.grid.owner.reserve(n);
+    const int64_t kept = nkx::compact_points(s, n, w_thr, p0.p, w0.p, o0.p, c.grid.pts.p,
+                                             c.grid.w.p, c.grid.owner.p, nullptr, cnt.p);
+    c.grid.npts = kept;
+    c.grid.version++;
+    if (npts_out) *npts_out = kept;
+  });
+}
+
+int nkx_set_grid(nkx_ctx *ctx, int natoms, const double *centers, int64_t npts,
+                 const double *pts, const double *w, const int *owner) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(natoms > 0 && centers && npts >= 0 && (npts == 0 || (pts && w)),
+                NKX_ERR_INVALID_ARGUMENT, "set_grid: bad arguments");
+    hipStream_t s = c.stream;
+    c.grid.natoms = natoms;
+    c.grid.centers.reserve(3 * (size_t)natoms);
+    NKX_HIP_CHECK(hipMemcpyAsync(c.grid.centers.p, centers, 3 * natoms * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+    c.grid.centers_host.resize(3 * natoms);
+    NKX_HIP_CHECK(hipMemcpyAsync(c.grid.centers_host.data(), centers,
+                                 3 * natoms * sizeof(double), hipMemcpyDeviceToHost, s));
+    c.grid.pts.reserve(3 * npts);
+    c.grid.w.reserve(npts);
+    c.grid.owner.reserve(npts);
+    if (npts) {
+      NKX_HIP_CHECK(hipMemcpyAsync(c.grid.pts.p, pts, 3 * npts * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      NKX_HIP_CHECK(hipMemcpyAsync(c.grid.w.p, w, npts * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      if (owner)
+        NKX_HIP_CHECK(hipMemcpyAsync(c.grid.owner.p, owner, npts * sizeof(int),
+                                     hipMemcpyDeviceToDevice, s));
+      else
+        NKX_HIP_CHECK(hipMemsetAsync(c.grid.owner.p, 0, npts * sizeof(int), s));
+    }
+    NKX_HIP_CHECK(hipStreamSynchronize(s));
+    c.grid.npts = npts;
+    c.grid.version++;
+  });
+}
+
+int nkx_grid_npts(nkx_ctx *ctx, int64_t *npts) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(npts, NKX_ERR_INVALID_ARGUMENT, "npts is NULL");
+    *npts = ctx->c.grid.npts;
+  });
+}
+
+int nkx_grid_export(nkx_ctx *ctx, double *pts, double *w, int *owner) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    const int64_t n = c.grid.npts;
+    hipStream_t s = c.stream;
+    if (n == 0) return;
+    if (pts)
+      NKX_HIP_CHECK(hipMemcpyAsync(pts, c.grid.pts.p, 3 * n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+    if (w)
+      NKX_HIP_CHECK(hipMemcpyAsync(w, c.grid.w.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (owner)
+      NKX_HIP_CHECK(hipMemcpyAsync(owner, c.grid.owner.p, n * sizeof(int),
+                                   hipMemcpyDeviceToDevice, s));
+  });
+}
+
+int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers, int64_t npts, const double *pts,
+              const int *owner, double *w, int capped) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(natoms > 0 && centers && (npts == 0 || (pts && owner && w)),
+                NKX_ERR_INVALID_ARGUMENT, "becke: bad arguments");
+    nkx::launch_becke(ctx->c.stream, natoms, centers, npts, pts, owner, w, capped ? 1 : 0);
+  });
+}
+
+int nkx_sto_norm_cutoff(int n, double zeta, double tol, double *norm, double *cutoff) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(norm && cutoff, NKX_ERR_INVALID_ARGUMENT, "output is NULL");
+    nkx::sto_norm_cutoff(n, zeta, tol, norm, cutoff);
+  });
+}
+
+int nkx_set_basis(nkx_ctx *ctx, int nbf, const int *n, const int *l, const int *m,
+                  const double *zeta, const double *norm, const double *centers,
+                  const double *cutoff) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nbf > 0 && n && l && m && zeta && norm && centers, NKX_ERR_INVALID_ARGUMENT,
+                "set_basis: bad arguments");
+    nkx::BasisHost B;
+    B.nbf = nbf;
+    for (int i = 0; i < nbf; ++i) {
+      NKX_REQUIRE(n[i] >= 1 && l[i] >= 0 && l[i] <= 7 && l[i] < n[i] && std::abs(m[i]) <= l[i] &&
+                      zeta[i] > 0,
+                  NKX_ERR_INVALID_ARGUMENT,
+                  "set_basis: need n >= 1, 0 <= l < n, l <= 7, |m| <= l, zeta > 0");
+      B.n.push_back(n[i]);
+      B.l.push_back(l[i]);
+      B.m.push_back(m[i]);
+      B.zeta.push_back(zeta[i]);
+      B.norm.push_back(norm[i]);
+      B.cx.push_back(centers[3 * i]);
+      B.cy.push_back(centers[3 * i + 1]);
+      B.cz.push_back(centers[3 * i + 2]);
+      B.cutoff.push_back(cutoff ? cutoff[i] : 0.0);
+      (void)nkx::solid_harmonic_terms(l[i], m[i]);
+    }
+    ctx->c.basis = std::move(B);
+    ctx->c.basis_version++;
+  });
+}
+
+int nkx_basis_nbf(nkx_ctx *ctx, int *nbf) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nbf, NKX_ERR_INVALID_ARGUMENT, "nbf is NULL");
+    *nbf = ctx->c.basis.nbf;
+  });
+}
+
+int nkx_solid_harmonic(int l, int m, double x, double y, double z, double *out4) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(out4, NKX_ERR_INVALID_ARGUMENT, "out4 is NULL");
+    const auto &t = nkx::solid_harmonic_terms(l, m);
+    double v = 0, gx = 0, gy = 0, gz = 0;
+    auto pw = [](double b, int e) {
+      double r = 1.0;
+      for (int i = 0; i < e; ++i) r *= b;
+      return r;
+    };
+    for (const auto &h : t) {
+      v += h.coef * pw(x, h.a) * pw(y, h.b) * pw(z, h.c);
+      if (h.a) gx += h.coef * h.a * pw(x, h.a - 1) * pw(y, h.b) * pw(z, h.c);
+      if (h.b) gy += h.coef * h.b * pw(x, h.a) * pw(y, h.b - 1) * pw(z, h.c);
+      if (h.c) gz += h.coef * h.c * pw(x, h.a) * pw(y, h.b) * pw(z, h.c - 1);
+    }
+    out4[0] = v;
+    out4[1] = gx;
+    out4[2] = gy;
+    out4[3] = gz;
+  });
+}
+
+int nkx_collocation(nkx_ctx *ctx, int64_t npts, const double *pts, double *phi, double *gx,
+                    double *gy, double *gz) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(c.basis.nbf > 0, NKX_ERR_STATE, "no basis: call nkx_set_basis");
+    NKX_REQUIRE(npts >= 0 && (npts == 0 || (pts && (phi || gx))), NKX_ERR_INVALID_ARGUMENT,
+                "collocation: bad arguments");
+    NKX_REQUIRE((!gx && !gy && !gz) || (gx && gy && gz), NKX_ERR_INVALID_ARGUMENT,
+                "collocation: pass all three gradient arrays or none");
+    // Function records in basis order (no permutation).
+    const nkx::BasisHost &B = c.basis;
+    std::vector<nkx::FuncRec> fr(B.nbf);
+    std::vector<nkx::HarmTerm> terms;
+    for (int i = 0; i < B.nbf; ++i) {
+      const auto &t = nkx::solid_harmonic_terms(B.l[i], B.m[i]);
+      fr[i] = nkx::FuncRec{B.cx[i], B.cy[i], B.cz[i], B.zeta[i], B.norm[i],
+                           B.n[i] - B.l[i] - 1, (int)terms.size(), (int)t.size(), 0};
+      terms.insert(terms.end(), t.begin(), t.end());
+    }
+    nkx::DevBuf<nkx::FuncRec> fd;
+    nkx::DevBuf<nkx::HarmTerm> td;
+    fd.upload(fr.data(), fr.size(), c.stream);
+    td.upload(terms.data(), terms.size(), c.stream);
+    nkx::launch_collocation_ref(c.stream, B.nbf, fd.p, td.p, npts, pts, phi, gx, gy, gz);
+    NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int nkx_density(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi, const double *gx,
+                const double *gy, const double *gz, int nocc, const double *C, const double *f,
+                double *rho, double *grx, double *gry, double *grz, double *sigma) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nbf > 0 && npts >= 0 && nocc >= 0 && phi && C && f && rho,
+                NKX_ERR_INVALID_ARGUMENT, "density: bad arguments");
+    NKX_REQUIRE((!gx && !gy && !gz) || (gx && gy && gz), NKX_ERR_INVALID_ARGUMENT,
+                "density: pass all three gradient arrays or none");
+    nkx::launch_density_ref(ctx->c.stream, nbf, npts, phi, gx, gy, gz, nocc, C, f, rho,
+                            gx ? grx : nullptr, gry, grz, gx ? sigma : nullptr);
+  });
+}
+
+int nkx_functional_info(int id, int *family, int *kind) {
+  int fam = 0, k = 0;
+  if (functional_info(id, &fam, &k)) {
+    g_thread_error = "unknown functional id " + std::to_string(id);
+    return NKX_ERR_UNKNOWN_FUNCTIONAL;
+  }
+  if (family) *family = fam;
+  if (kind) *kind = k;
+  return NKX_OK;
+}
+
+int nkx_set_dens_threshold(nkx_ctx *ctx, double thr) {
+  NKX_CTX_CHECK(ctx);
+  ctx->c.opt.dens_threshold = thr;
+  return NKX_OK;
+}
+
+int nkx_functional_eval(nkx_ctx *ctx, int nfunc, const int *ids, int64_t npts,
+                        const double *rho, const double *sigma, double *exc, double *vrho,
+                        double *vsigma) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(!gga || (sigma && vsigma), NKX_ERR_FUNCTIONAL_FAMILY,
+                "GGA functionals need sigma and vsigma");
+    NKX_REQUIRE(npts >= 0 && (npts == 0 || (rho && exc && vrho)), NKX_ERR_INVALID_ARGUMENT,
+                "functional_eval: bad arguments");
+    nkx::launch_functional_eval(ctx->c.stream, xl, npts, rho, sigma, exc, vrho, vsigma);
+  });
+}
+
+int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi,
+                            const double *gx, const double *gy, const double *gz,
+                            const double *w, int nocc, const double *C, const double *f,
+                            int nfunc, const int *ids, int family_check, double *E, double *V) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    if (family_check == NKX_FAMILY_LDA)
+      NKX_REQUIRE(!gga, NKX_ERR_FUNCTIONAL_FAMILY,
+                  "Provided funtional is not a part of the LDA family");
+    if (family_check == NKX_FAMILY_GGA) {
+      bool all_gga = true;
+      for (int k = 0; k < nfunc; ++k) {
+        int fam, kind;
+        functional_info(ids[k], &fam, &kind);
+        all_gga = all_gga && fam == NKX_FAMILY_GGA;
+      }
+      NKX_REQUIRE(all_gga, NKX_ERR_FUNCTIONAL_FAMILY,
+                  "Provided funtional is not a part of the GGA or Hybrid GGA family");
+    }
+    NKX_REQUIRE(!gga || (gx && gy && gz), NKX_ERR_INVALID_ARGUMENT,
+                "GGA functionals need the collocation gradients");
+    NKX_REQUIRE(nbf > 0 && npts >= 0 && phi && w && C && f && E && V,
+                NKX_ERR_INVALID_ARGUMENT, "xc_from_collocation: bad arguments");
+    nkx::xc_from_collocation(ctx->c, nbf, npts, phi, gx, gy, gz, w, nocc, C, f, xl, gga, E, V);
+  });
+}
+
+int nkx_vxc(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision, double *E,
+            double *N, double *V) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_INVALID_ARGUMENT, "unknown precision");
+    NKX_REQUIRE(P && E && N && V, NKX_ERR_INVALID_ARGUMENT, "vxc: bad arguments");
+    nkx::vxc_build(ctx->c, P, xl, gga, precision, E, N, V);
+  });
+}
+
+int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(tol > 0 && tol < 1, NKX_ERR_INVALID_ARGUMENT, "screening tolerance in (0, 1)");
+    NKX_REQUIRE(batch_points == 0 || batch_points == 64, NKX_ERR_UNSUPPORTED,
+                "batch size 64 is the only one built");
+    ctx->c.opt.screen_tol = tol;
+  });
+}
+
+int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npad, int64_t *nb, int64_t *kept,
+                   int64_t *total) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::ensure_plan(ctx->c);
+    const nkx::Plan &p = ctx->c.plan;
+    if (npts) *npts = p.npts;
+    if (npad) *npad = p.npts_pad;
+    if (nb) *nb = p.nb;
+    if (kept) *kept = p.pairs_kept;
+    if (total) *total = p.nb * p.nfb;
+  });
+}
+
+int nkx_overlap(nkx_ctx *ctx, double *S) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(S, NKX_ERR_INVALID_ARGUMENT, "S is NULL");
+    nkx::overlap_build(ctx->c, S);
+  });
+}
+
+}  // extern "C"

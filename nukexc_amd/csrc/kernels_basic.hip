@@ -1,0 +1,388 @@
+// Grid partition, compaction, reference-shaped collocation / density / XC.
+#include <cmath>
+#include <vector>
+
+#include "nkx_internal.hpp"
+#include "sto_eval.hpp"
+#include "xc_functionals.hpp"
+
+namespace nkx {
+
+// ---------------------------------------------------------------------------
+// Becke partition (partitioning.hpp:44-55, :63-120, :126-213), one thread per
+// point, atoms visited in index order as in the reference.
+//
+// Capped form, exact screening: with R_ij <= 5 every atom i whose distance
+// exceeds the nearest atom's by >= 5 bohr has mu_{i,nearest} >= 1, so
+// s = 0 and P_i = 0 exactly; every j farther than r_i + 5 has mu_ij <= -1 and
+// contributes the factor s(-1) = 1 exactly (p(+-1) = +-1 in FP64).  Skipping
+// those factors and terms leaves every product and sum bit-identical.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double becke_p(double x) { return (1.5 * x) - (0.5 * (x * x * x)); }
+
+__device__ __forceinline__ double bdist(double px, double py, double pz, const double *c) {
+  const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__global__ void __launch_bounds__(256) becke_kernel(int natoms, const double *__restrict__ C,
+                                                    int64_t npts, const double *__restrict__ pts,
+                                                    const int *__restrict__ owner,
+                                                    double *__restrict__ w, int capped) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts) return;
+  const double px = pts[3 * g], py = pts[3 * g + 1], pz = pts[3 * g + 2];
+  const int own = owner[g];
+  double rmin = 1e300;
+  if (capped)
+    for (int j = 0; j < natoms; ++j) rmin = fmin(rmin, bdist(px, py, pz, C + 3 * j));
+  double norm = 0.0, pown = 0.0;
+  for (int i = 0; i < natoms; ++i) {
+    const double ri = bdist(px, py, pz, C + 3 * i);
+    if (capped && ri - rmin >= 5.0) continue;  // P_i == 0 exactly
+    double Pi = 1.0;
+    for (int j = 0; j < natoms; ++j) {
+      if (j == i) continue;
+      const double rj = bdist(px, py, pz, C + 3 * j);
+      if (capped && ri - rj <= -5.0) continue;  // factor == 1 exactly
+      double Rij = bdist(C[3 * i], C[3 * i + 1], C[3 * i + 2], C + 3 * j);
+      if (capped) Rij = fmin(Rij, 5.0);
+      double mu = (ri - rj) / Rij;
+      mu = mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
+      Pi *= 0.5 * (1.0 - becke_p(becke_p(becke_p(mu))));
+      if (Pi == 0.0) break;  // stays exactly zero
+    }
+    if (i == own) pown = Pi;
+    norm += Pi;
+  }
+  w[g] *= pown / norm;
+}
+
+void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
+                  const double *pts, const int *owner, double *w, int capped) {
+  if (npts == 0) return;
+  const int64_t blocks = (npts + 255) / 256;
+  hipLaunchKernelGGL(becke_kernel, dim3((unsigned)blocks), dim3(256), 0, s, natoms, centers,
+                     npts, pts, owner, w, capped);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Order-preserving compaction of (point, weight, owner) with w > thr.  The
+// reference compacts with an atomic counter, so its surviving order is
+// nondeterministic (grid.hpp:329-338); this keeps generation order.
+// ---------------------------------------------------------------------------
+constexpr int kCompactBlock = 1024;
+
+__global__ void __launch_bounds__(kCompactBlock) compact_count(int64_t n, double thr,
+                                                               const double *__restrict__ w,
+                                                               int64_t *__restrict__ counts) {
+  __shared__ int part[kCompactBlock / 64];
+  const int64_t g = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool keep = g < n && w[g] > thr;
+  const unsigned long long mask = __ballot(keep);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int i = 0; i < kCompactBlock / 64; ++i) s += part[i];
+    counts[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kCompactBlock) compact_scatter(
+    int64_t n, double thr, const double *__restrict__ pin, const double *__restrict__ win,
+    const int *__restrict__ oin, const int64_t *__restrict__ offs, double *__restrict__ pout,
+    double *__restrict__ wout, int *__restrict__ oout) {
+  __shared__ int part[kCompactBlock / 64];
+  const int64_t g = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool keep = g < n && win[g] > thr;
+  const unsigned long long mask = __ballot(keep);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) part[wv] = __popcll(mask);
+  __syncthreads();
+  int before = 0;
+  for (int i = 0; i < wv; ++i) before += part[i];
+  const unsigned long long lower = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+  if (keep) {
+    const int64_t d = offs[blockIdx.x] + before + __popcll(lower);
+    pout[3 * d] = pin[3 * g];
+    pout[3 * d + 1] = pin[3 * g + 1];
+    pout[3 * d + 2] = pin[3 * g + 2];
+    wout[d] = win[g];
+    oout[d] = oin[g];
+  }
+}
+
+int64_t compact_points(hipStream_t s, int64_t n, double thr, const double *pin,
+                       const double *win, const int *oin, double *pout, double *wout,
+                       int *oout, int * /*unused*/, int64_t *counts) {
+  if (n == 0) return 0;
+  const int64_t nb = (n + kCompactBlock - 1) / kCompactBlock;
+  hipLaunchKernelGGL(compact_count, dim3((unsigned)nb), dim3(kCompactBlock), 0, s, n, thr, win,
+                     counts);
+  NKX_HIP_CHECK(hipGetLastError());
+  std::vector<int64_t> h(nb);
+  NKX_HIP_CHECK(hipMemcpyAsync(h.data(), counts, nb * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
+  int64_t tot = 0;
+  for (int64_t i = 0; i < nb; ++i) {
+    const int64_t c = h[i];
+    h[i] = tot;
+    tot += c;
+  }
+  NKX_HIP_CHECK(hipMemcpyAsync(counts, h.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(compact_scatter, dim3((unsigned)nb), dim3(kCompactBlock), 0, s, n, thr, pin,
+                     win, oin, counts, pout, wout, oout);
+  NKX_HIP_CHECK(hipGetLastError());
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
+  return tot;
+}
+
+// ---------------------------------------------------------------------------
+// Collocation in the reference layout: LayoutLeft (N_bf, N_pts), element
+// (mu, g) at mu + g * N_bf (stobasis.hpp:494-584).  One thread per element,
+// mu fastest so the stores are contiguous.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) colloc_ref_kernel(int nbf, const FuncRec *__restrict__ F,
+                                                         const HarmTerm *__restrict__ T,
+                                                         int64_t npts,
+                                                         const double *__restrict__ pts,
+                                                         double *phi, double *gx, double *gy,
+                                                         double *gz) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nbf * npts) return;
+  const int mu = (int)(e % nbf);
+  const int64_t g = e / nbf;
+  const FuncRec f = F[mu];
+  const double x = pts[3 * g], y = pts[3 * g + 1], z = pts[3 * g + 2];
+  if (gx) {
+    double v, a, b, c;
+    sto_value_grad<double>(f, T, x, y, z, v, a, b, c);
+    if (phi) phi[e] = v;
+    gx[e] = a;
+    gy[e] = b;
+    gz[e] = c;
+  } else {
+    phi[e] = sto_value<double>(f, T, x, y, z);
+  }
+}
+
+void launch_collocation_ref(hipStream_t s, int nbf, const FuncRec *f, const HarmTerm *t,
+                            int64_t npts, const double *pts, double *phi, double *gx,
+                            double *gy, double *gz) {
+  const int64_t n = (int64_t)nbf * npts;
+  if (n == 0) return;
+  hipLaunchKernelGGL(colloc_ref_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nbf,
+                     f, t, npts, pts, phi, gx, gy, gz);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Functional evaluation on device arrays.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) functional_kernel(XcList xl, int64_t npts,
+                                                         const double *__restrict__ rho,
+                                                         const double *__restrict__ sigma,
+                                                         double *exc, double *vrho,
+                                                         double *vsigma) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts) return;
+  const double r = rho[g];
+  const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, r, sigma ? sigma[g] : 0.0);
+  exc[g] = r > 0.0 ? o.e / r : 0.0;
+  vrho[g] = o.vrho;
+  if (vsigma) vsigma[g] = o.vsigma;
+}
+
+void launch_functional_eval(hipStream_t s, XcList xl, int64_t npts, const double *rho,
+                            const double *sigma, double *exc, double *vrho, double *vsigma) {
+  if (npts == 0) return;
+  hipLaunchKernelGGL(functional_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, xl,
+                     npts, rho, sigma, exc, vrho, vsigma);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Reference-shaped density (density.hpp:150-255): psi_i = sum_mu C(mu,i)
+// Phi(mu,g), rho = sum_i f_i psi_i^2, grad rho = 2 sum_i f_i psi_i grad psi_i.
+// One thread per point; C and f staged through LDS in chunks.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) density_ref_kernel(
+    int nbf, int64_t npts, const double *__restrict__ phi, const double *__restrict__ gx,
+    const double *__restrict__ gy, const double *__restrict__ gz, int nocc,
+    const double *__restrict__ C, const double *__restrict__ f, double *rho, double *grx,
+    double *gry, double *grz, double *sigma) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = g < npts;
+  double r = 0, a = 0, b = 0, c = 0;
+  for (int i = 0; i < nocc; ++i) {
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    if (live) {
+      const double *col = C + (size_t)i * nbf;  // LayoutLeft (N_bf, N_occ)
+      const size_t base = (size_t)g * nbf;
+      for (int mu = 0; mu < nbf; ++mu) {
+        const double cm = col[mu];
+        p0 += cm * phi[base + mu];
+        if (gx) {
+          p1 += cm * gx[base + mu];
+          p2 += cm * gy[base + mu];
+          p3 += cm * gz[base + mu];
+        }
+      }
+    }
+    const double fi = f[i];
+    r += fi * p0 * p0;
+    a += fi * p1 * p0;
+    b += fi * p2 * p0;
+    c += fi * p3 * p0;
+  }
+  if (!live) return;
+  rho[g] = r;
+  if (grx) {
+    grx[g] = 2.0 * a;
+    gry[g] = 2.0 * b;
+    grz[g] = 2.0 * c;
+  }
+  if (sigma) sigma[g] = 4.0 * (a * a + b * b + c * c);
+}
+
+void launch_density_ref(hipStream_t s, int nbf, int64_t npts, const double *phi,
+                        const double *gx, const double *gy, const double *gz, int nocc,
+                        const double *C, const double *f, double *rho, double *grx,
+                        double *gry, double *grz, double *sigma) {
+  if (npts == 0) return;
+  hipLaunchKernelGGL(density_ref_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
+                     nbf, npts, phi, gx, gy, gz, nocc, C, f, rho, grx, gry, grz, sigma);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Reference-shaped XC assembly (compute_lda xc_integrals.hpp:73-166,
+// compute_gga :424-541): Z_mu,g = w_g (1/2 Phi v_rho + 2 (grad rho . grad Phi)
+// v_sigma) written over Phi's storage pattern, then V = Phi Z^T + (Phi Z^T)^T
+// by a split-K LDS-tiled FP64 kernel, and E = sum_g w rho eps.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) zmat_ref_kernel(int nbf, int64_t npts,
+                                                       const double *__restrict__ phi,
+                                                       const double *__restrict__ gx,
+                                                       const double *__restrict__ gy,
+                                                       const double *__restrict__ gz,
+                                                       const double *__restrict__ w,
+                                                       const double *__restrict__ rho,
+                                                       const double *__restrict__ grx,
+                                                       const double *__restrict__ gry,
+                                                       const double *__restrict__ grz,
+                                                       const double *__restrict__ vrho,
+                                                       const double *__restrict__ vsig,
+                                                       const double *__restrict__ exc,
+                                                       double *Z, double *EN) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double econ = 0.0;
+  if (e < (int64_t)nbf * npts) {
+    const int mu = (int)(e % nbf);
+    const int64_t g = e / nbf;
+    double z = 0.5 * phi[e] * vrho[g];
+    if (gx) z += 2.0 * vsig[g] * (grx[g] * gx[e] + gry[g] * gy[e] + grz[g] * gz[e]);
+    Z[e] = w[g] * z;
+    if (mu == 0) econ = w[g] * rho[g] * exc[g];
+  }
+  // block reduction of the energy terms
+  for (int o = 32; o > 0; o >>= 1) econ += __shfl_down(econ, o);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = econ;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(EN, part[0] + part[1] + part[2] + part[3]);
+}
+
+// V(mu, nu) += sum_g A(mu, g) B(nu, g) with A, B LayoutLeft (nbf, npts):
+// 64 x 64 output tile per block, 16 x 16 threads with 4 x 4 micro-tiles,
+// K split over blockIdx.z, partial tiles added with FP64 atomics.
+__global__ void __launch_bounds__(256) gemm_nt_ref_kernel(int nbf, int64_t npts,
+                                                          const double *__restrict__ A,
+                                                          const double *__restrict__ B,
+                                                          double *V, int64_t kchunk) {
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int64_t k0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t k1 = k0 + kchunk < npts ? k0 + kchunk : npts;
+  double acc[4][4] = {};
+  for (int64_t kk = k0; kk < k1; kk += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kr = e / 64, mc = e % 64;
+      const int64_t g = kk + kr;
+      const bool ok = g < k1;
+      As[kr][mc] = (ok && m0 + mc < nbf) ? A[g * nbf + m0 + mc] : 0.0;
+      Bs[kr][mc] = (ok && n0 + mc < nbf) ? B[g * nbf + n0 + mc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      double a[4], b[4];
+      for (int i = 0; i < 4; ++i) a[i] = As[k][ty * 4 + i];
+      for (int j = 0; j < 4; ++j) b[j] = Bs[k][tx * 4 + j];
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int mu = m0 + ty * 4 + i, nu = n0 + tx * 4 + j;
+      if (mu < nbf && nu < nbf) atomicAdd(&V[(size_t)mu * nbf + nu], acc[i][j]);
+    }
+}
+
+__global__ void symmetrize_kernel(int nbf, const double *__restrict__ Vb, double *V) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nbf * nbf) return;
+  const int i = (int)(e / nbf), j = (int)(e % nbf);
+  V[e] = Vb[(size_t)i * nbf + j] + Vb[(size_t)j * nbf + i];
+}
+
+void xc_from_collocation(Ctx &c, int nbf, int64_t npts, const double *phi, const double *gx,
+                         const double *gy, const double *gz, const double *w, int nocc,
+                         const double *C, const double *f, const XcList &xl, bool gga,
+                         double *E, double *V) {
+  hipStream_t s = c.stream;
+  const bool grad = gga && gx;
+  // per-point scratch: rho, grx, gry, grz, sigma, exc, vrho, vsigma; Z; Vbar; E
+  const size_t per = 8 * (size_t)npts;
+  c.ws.scratch.reserve(per + (size_t)nbf * npts + (size_t)nbf * nbf + 1);
+  double *rho = c.ws.scratch.p, *grx = rho + npts, *gry = grx + npts, *grz = gry + npts,
+         *sig = grz + npts, *exc = sig + npts, *vr = exc + npts, *vs = vr + npts;
+  double *Z = c.ws.scratch.p + per;
+  double *Vb = Z + (size_t)nbf * npts;
+  double *EN = Vb + (size_t)nbf * nbf;
+  launch_density_ref(s, nbf, npts, phi, grad ? gx : nullptr, gy, gz, nocc, C, f, rho,
+                     grad ? grx : nullptr, gry, grz, grad ? sig : nullptr);
+  launch_functional_eval(s, xl, npts, rho, grad ? sig : nullptr, exc, vr, grad ? vs : nullptr);
+  NKX_HIP_CHECK(hipMemsetAsync(Vb, 0, ((size_t)nbf * nbf + 1) * sizeof(double), s));
+  const int64_t ne = (int64_t)nbf * npts;
+  if (ne > 0) {
+    hipLaunchKernelGGL(zmat_ref_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, nbf,
+                       npts, phi, grad ? gx : nullptr, gy, gz, w, rho, grx, gry, grz, vr, vs,
+                       exc, Z, EN);
+    NKX_HIP_CHECK(hipGetLastError());
+    const int tiles = (nbf + 63) / 64;
+    int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(512 / (tiles * tiles) + 1,
+                                                            (npts + 1023) / 1024));
+    const int64_t kchunk = ((npts + nsplit - 1) / nsplit + 15) / 16 * 16;
+    nsplit = (npts + kchunk - 1) / kchunk;
+    hipLaunchKernelGGL(gemm_nt_ref_kernel, dim3(tiles, tiles, (unsigned)nsplit), dim3(256), 0, s,
+                       nbf, npts, phi, Z, Vb, kchunk);
+    NKX_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t nn = (int64_t)nbf * nbf;
+  if (nn > 0) {
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                       nbf, Vb, V);
+    NKX_HIP_CHECK(hipGetLastError());
+  }
+  NKX_HIP_CHECK(hipMemcpyAsync(E, EN, sizeof(double), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace nkx

@@ -1,0 +1,202 @@
+// Internal declarations shared by the host sources and HIP kernels of
+// libnukexc_hip.so.  Nothing here is part of the ABI (include/nukexc_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nukexc_hip.h"
+
+namespace nkx {
+
+// Errors raised inside the library; converted to nkx_status at the ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define NKX_HIP_CHECK(call)                                                   \
+  do {                                                                        \
+    hipError_t nkx_e_ = (call);                                               \
+    if (nkx_e_ != hipSuccess)                                                 \
+      throw ::nkx::Error{nkx_e_ == hipErrorOutOfMemory ? NKX_ERR_OUT_OF_MEMORY \
+                                                       : NKX_ERR_HIP,         \
+                         std::string(#call) + ": " + hipGetErrorString(nkx_e_)}; \
+  } while (0)
+
+#define NKX_REQUIRE(cond, code, msg)                                          \
+  do {                                                                        \
+    if (!(cond)) throw ::nkx::Error{(code), (msg)};                           \
+  } while (0)
+
+// Owning device buffer; grows on demand and never shrinks.
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    release();
+    if (count == 0) return;
+    NKX_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T *host, size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) NKX_HIP_CHECK(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// STO basis on the device, one record per function.
+struct BasisHost {
+  int nbf = 0;
+  std::vector<int> n, l, m;
+  std::vector<double> zeta, norm, cx, cy, cz, cutoff;
+};
+
+// Solid-harmonic polynomial terms: S_lm = sum_t coef_t x^a y^b z^c.
+struct HarmTerm {
+  double coef;
+  int a, b, c, pad;
+};
+
+// Per-function record consumed by the collocation kernels.
+struct FuncRec {
+  double cx, cy, cz, zeta, norm;
+  int k;          // n - l - 1, power of r in the radial part
+  int term_off;   // first HarmTerm of this function
+  int term_cnt;   // number of HarmTerms
+  int pad;
+};
+
+struct GridState {
+  int natoms = 0;
+  std::vector<double> centers_host;  // natoms x 3
+  DevBuf<double> centers;            // natoms x 3
+  int64_t npts = 0;
+  DevBuf<double> pts;                // npts x 3
+  DevBuf<double> w;
+  DevBuf<int> owner;
+  uint64_t version = 0;              // bumped on every change
+};
+
+// Launch plan of the batched V_xc build; rebuilt when grid, basis or the
+// screening options change.
+struct Plan {
+  uint64_t grid_version = ~0ull, basis_version = ~0ull;
+  double screen_tol = -1;
+  int pt = 0;                 // points per batch
+  int fb = 64;                // functions per function block
+  int nbf = 0, nbf_pad = 0, nfb = 0;
+  int64_t npts = 0, npts_pad = 0, nb = 0;  // nb = number of point batches
+  std::vector<int> fperm;     // sorted position -> original function index
+  DevBuf<int> fperm_d;
+  DevBuf<FuncRec> funcs;      // nbf_pad records in sorted order (padding: norm 0)
+  DevBuf<HarmTerm> terms;
+  DevBuf<double> px, py, pz, pw;  // sorted, padded point SoA
+  DevBuf<int> pidx;               // sorted position -> original point (-1: pad)
+  // batch -> active function blocks (CSR)
+  std::vector<int> b_off_h;
+  DevBuf<int> b_off, b_fbl, slot_b;  // slot s = CSR position; slot_b[s] = its batch
+  // function-block pair (I <= J) -> batches where both are active (CSR)
+  int npairs = 0;
+  std::vector<int> pair_I_h, pair_J_h;
+  DevBuf<int> pair_I, pair_J, pair_off, pair_bl;
+  DevBuf<int> tile_slot;          // (fb, batch) -> slot in the tile store or -1
+  int64_t nslots = 0;
+  int64_t pairs_kept = 0;
+};
+
+struct Options {
+  double screen_tol = 1e-12;
+  int batch_points = 64;
+  double dens_threshold = 0.0;  // <= 0: per-functional libxc default
+};
+
+struct Workspace {
+  DevBuf<double> Pd;               // permuted padded density matrix (FP64)
+  DevBuf<float> Pf;                // same, FP32 (mixed)
+  DevBuf<double> tiles64;          // Phi, gx, gy, gz, Z tiles (FP64)
+  DevBuf<float> tiles32;           // same (mixed)
+  DevBuf<double> coef;             // per padded point: a, bx, by, bz
+  DevBuf<double> blockEN;          // per batch: E, N partial sums
+  DevBuf<double> vpart;            // split-K partial V tiles
+  DevBuf<double> red;              // small reduction outputs
+  DevBuf<double> scratch;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+  Options opt;
+  GridState grid;
+  BasisHost basis;
+  uint64_t basis_version = 0;
+  Plan plan;
+  Workspace ws;
+};
+
+// ---- host helpers (basis.cpp, lebedev.cpp, grid.cpp) --------------------
+void sto_norm_cutoff(int n, double zeta, double tol, double *norm, double *cut);
+// Monomial expansion of the real solid harmonic (l, m), l <= 7.
+const std::vector<HarmTerm> &solid_harmonic_terms(int l, int m);
+int lebedev_lookup(int order, int *npts, int *actual_order);
+void lebedev_expand(int actual_order, double *xyzw);
+void radial_rule(int scheme, int n, double R, double alpha, double *r, double *w);
+double radial_scale(int scheme, int Z);
+int periodic_row(int Z);
+
+// ---- kernels (launchers) -------------------------------------------------
+void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
+                  const double *pts, const int *owner, double *w, int capped);
+int64_t compact_points(hipStream_t s, int64_t n, double thr, const double *pts_in,
+                       const double *w_in, const int *own_in, double *pts_out,
+                       double *w_out, int *own_out, int *scratch_flags,
+                       int64_t *scratch_scan);
+
+void launch_collocation_ref(hipStream_t s, int nbf, const FuncRec *f,
+                            const HarmTerm *t, int64_t npts, const double *pts,
+                            double *phi, double *gx, double *gy, double *gz);
+
+// Functionals: kind bitmask of ids (device constant table).
+struct XcList {
+  int n;
+  int ids[4];
+  double thr[4];
+};
+bool xc_list_make(int nfunc, const int *ids, double dens_thr_override, XcList *out,
+                  bool *any_gga);
+void launch_functional_eval(hipStream_t s, XcList xl, int64_t npts, const double *rho,
+                            const double *sigma, double *exc, double *vrho,
+                            double *vsigma);
+
+void launch_density_ref(hipStream_t s, int nbf, int64_t npts, const double *phi,
+                        const double *gx, const double *gy, const double *gz, int nocc,
+                        const double *C, const double *f, double *rho, double *grx,
+                        double *gry, double *grz, double *sigma);
+void xc_from_collocation(Ctx &c, int nbf, int64_t npts, const double *phi,
+                         const double *gx, const double *gy, const double *gz,
+                         const double *w, int nocc, const double *C, const double *f,
+                         const XcList &xl, bool gga, double *E, double *V);
+
+// Batched build (vxc.cpp)
+void ensure_plan(Ctx &c);
+void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
+               double *E, double *N, double *V);
+void overlap_build(Ctx &c, double *S);
+
+}  // namespace nkx

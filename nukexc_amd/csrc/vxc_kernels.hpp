@@ -1,0 +1,22 @@
+// Launchers of the batched V_xc kernels (kernels_vxc.hip).
+#pragma once
+
+#include "nkx_internal.hpp"
+
+namespace nkx {
+
+void vk_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const double *P,
+                 double *Pd);
+void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
+               double *tiles);
+void vk_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
+                double *coef, double *blockEN);
+void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr,
+                 double *tiles);
+void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
+              int narr, const double *tiles, double *vpart);
+void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V);
+void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out);
+
+}  // namespace nkx

@@ -1,0 +1,292 @@
+"""GPU parity: every kernel of libnukexc_hip.so against the FP64 CPU oracle on
+the same inputs, through the C ABI.  FP64 bars from the XC-path contract:
+|dE_xc| <= 1e-8 Eh, max|dV_xc| <= 1e-9; element-wise kernels to ~1e-13."""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from conftest import ROOT, lebedev, next_order  # noqa: E402
+from gpu_util import basis_dict, oracle_on_grid, orbitals_on_grid  # noqa: E402
+import nukexc_amd as nk  # noqa: E402
+from nukexc_amd import configs  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_goldens.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = nk.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+# ------------------------------------------------------------ MFMA layout ----
+def test_mfma_f64_lane_layout():
+    """v_mfma_f64_16x16x4f64: lane l feeds A[l%16][l/16] and B[l/16][l%16];
+    result register r of lane l is D[4(l/16)+r][l%16]."""
+    probe = ctypes.CDLL(os.path.join(ROOT, "tests", "gpu_probe", "libmfma_probe.so"))
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 8, size=(16, 4)).astype(np.float64)
+    B = rng.integers(-8, 8, size=(4, 16)).astype(np.float64)
+    a = np.array([A[l % 16, l // 16] for l in range(64)])
+    b = np.array([B[l // 16, l % 16] for l in range(64)])
+    d = np.zeros(256)
+    pd = ctypes.POINTER(ctypes.c_double)
+    assert probe.mfma_probe_f64(a.ctypes.data_as(pd), b.ctypes.data_as(pd), d.ctypes.data_as(pd)) == 0
+    D = A @ B
+    got = np.array([[d[(j + 16 * (i // 4)) * 4 + i % 4] for j in range(16)] for i in range(16)])
+    assert np.array_equal(got, D)
+
+
+def _bf16_bits(x):
+    return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def test_mfma_bf16_lane_layout():
+    """v_mfma_f32_16x16x32_bf16: lane l feeds A[l%16][8(l/16)..+7] and
+    B[8(l/16)..+7][l%16]; result as for the f64 form."""
+    probe = ctypes.CDLL(os.path.join(ROOT, "tests", "gpu_probe", "libmfma_probe.so"))
+    rng = np.random.default_rng(1)
+    A = rng.integers(-4, 4, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-4, 4, size=(32, 16)).astype(np.float32)
+    a = np.array([A[l % 16, 8 * (l // 16) + i] for l in range(64) for i in range(8)], np.float32)
+    b = np.array([B[8 * (l // 16) + i, l % 16] for l in range(64) for i in range(8)], np.float32)
+    ab, bb = _bf16_bits(a), _bf16_bits(b)
+    d = np.zeros(256, np.float32)
+    pu = ctypes.POINTER(ctypes.c_ushort)
+    assert probe.mfma_probe_bf16(ab.ctypes.data_as(pu), bb.ctypes.data_as(pu),
+                                 d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    D = A @ B
+    got = np.array([[d[(j + 16 * (i // 4)) * 4 + i % 4] for j in range(16)] for i in range(16)])
+    assert np.array_equal(got, D)
+
+
+# ------------------------------------------------------------------ Becke ----
+def _raw_points(xyz, Z, nrad, order):
+    ang = nk.lebedev_rule(order)
+    P, W, Ow = [], [], []
+    for a, (c, z) in enumerate(zip(xyz, Z)):
+        r, w = nk.radial_rule(0, nrad, nk.radial_scale(0, z), 0.6)
+        P.append((r[:, None, None] * ang[None, :, :3]).reshape(-1, 3) + c)
+        W.append((w[:, None] * ang[None, :, 3]).reshape(-1))
+        Ow.append(np.full(len(r) * len(ang), a, np.int32))
+    return np.concatenate(P), np.concatenate(W), np.concatenate(Ow)
+
+
+@pytest.mark.parametrize("capped", [True, False])
+def test_becke_matches_oracle(ctx, capped):
+    w = configs.workload("benzene")
+    P, W, Ow = _raw_points(w["xyz"], w["Z"], 30, 17)
+    ref = O.becke(w["xyz"], P, Ow, W, capped=capped)
+    wd = dev(W)
+    nk.partition_becke(ctx, dev(w["xyz"]), dev(P), dev(Ow, torch.int32), wd, capped=capped)
+    got = wd.cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-300)
+    assert np.sum(got == 0.0) == np.sum(ref == 0.0)
+
+
+def test_becke_partition_of_unity_gpu(ctx):
+    """test_partition.cxx:197-258 on the GPU kernel."""
+    centers = np.array([[i, i % 2, i % 3] for i in range(10)], float)
+    jk = np.array([(0.0, j / 10, k / 10) for j in range(10) for k in range(10)])
+    pts = np.tile(jk, (10, 1))
+    owner = np.repeat(np.arange(10), 100).astype(np.int32)
+    wd = dev(np.ones(1000))
+    nk.partition_becke_team(ctx, dev(centers), dev(pts), dev(owner, torch.int32), wd)
+    assert np.abs(wd.cpu().numpy().reshape(10, 100).sum(0) - 1).max() < 1e-10
+
+
+# ------------------------------------------------------------------- grid ----
+@pytest.mark.parametrize("nrad,s_ref", [(10, 0.859122134805326), (20, 0.858383667211934),
+                                        (30, 0.858385391682404)])
+def test_h2plus_grid_and_overlap_gpu(ctx, nrad, s_ref):
+    """latex/Data/convergence_h2plus.csv through the GPU grid build and the
+    GPU overlap build."""
+    H2 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    grid = ctx.make_flat_grid(H2, [1, 1], nrad=nrad, nang_order=59)
+    ctx.set_basis(nk.make_manual_basis([(1, 0, 0, 1.0, 0, 0, 0), (1, 0, 0, 1.0, 1.0, 0, 0)]))
+    S = ctx.overlap().cpu().numpy()
+    assert abs(S[0, 1] - s_ref) < 5e-13
+    P, W, _ = O.make_flat_grid(H2, [1, 1], nrad, lebedev(59), scheme=0)
+    assert abs(len(W) - grid.weights.shape[0]) <= 16
+    assert abs(grid.weights.sum().item() - W.sum()) < 1e-10 * W.sum()
+
+
+def test_make_flat_grid_matches_oracle(ctx):
+    w = configs.workload("water")
+    grid = ctx.make_flat_grid(w["xyz"], w["Z"], nrad=40, nang_order=29)
+    P, W, Ow = O.make_flat_grid(w["xyz"], w["Z"], 40, lebedev(29), scheme=0)
+    from scipy.spatial import cKDTree
+    gp = grid.quad_points.cpu().numpy()
+    gw = grid.weights.cpu().numpy()
+    go = grid.point_owner.cpu().numpy()
+    # Same points up to the ordering of the angular rule: match nearest points.
+    d, idx = cKDTree(P).query(gp)
+    assert d.max() < 1e-12
+    assert np.array_equal(Ow[idx], go)
+    assert np.allclose(gw, W[idx], rtol=1e-12, atol=1e-300)
+    # Points kept by only one side carry noise-level weights (<= 1e-25 here).
+    d2, idx2 = cKDTree(gp).query(P)
+    assert np.all(W[d2 > 1e-12] < 1e-25)
+
+
+# ------------------------------------------------------------ collocation ----
+def test_collocation_matches_oracle_high_l(ctx):
+    rng = np.random.default_rng(5)
+    funcs = []
+    for l in range(8):
+        for m in range(-l, l + 1):
+            n = l + 1 + (m % 2)
+            funcs.append((n, l, m, float(rng.uniform(0.4, 3)), *rng.normal(size=3)))
+    basis = nk.make_manual_basis(funcs)
+    pts = rng.normal(scale=2.0, size=(3000, 3))
+    phi = nk.fill_collocation(ctx, basis, dev(pts)).cpu().numpy()
+    gx, gy, gz = (g.cpu().numpy() for g in nk.fill_grad_collocation(ctx, basis, dev(pts)))
+    r = oracle_on_grid(basis, pts, grad=True)
+    for got, ref in zip((phi, gx, gy, gz), r):
+        scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
+        assert np.abs(got - ref).max() / scale.min() < 1e-11 or np.all(np.abs(got - ref) <= 1e-12 * scale)
+
+
+# --------------------------------------------------------------- functional ----
+@pytest.mark.parametrize("ids", [[1], [12], [13], [101], [130], [101, 130], [1, 12]])
+def test_functional_matches_oracle(ctx, ids):
+    rng = np.random.default_rng(sum(ids))
+    rho = 10 ** rng.uniform(-16, 3, 4000)
+    sig = 10 ** rng.uniform(-20, 4, 4000) * np.maximum(rho, 1e-10) ** (8 / 3)
+    exc, vr, vs = nk.functional_eval(ctx, ids, dev(rho), dev(sig))
+    e0, v0, s0 = O.xc_eval(ids, rho, sig)
+    for got, ref in ((exc, e0), (vr, v0), (vs, s0)):
+        got = got.cpu().numpy()
+        assert np.allclose(got, ref, rtol=1e-11, atol=1e-300)
+
+
+# ------------------------------------------- reference-shaped XC on GPU ----
+def _single_atom(ctx, key):
+    g = GOLD[key]
+    Z = 1 if g["zeta"] == 1.0 else 2
+    order = next_order(g["grid"]["nang_order"]) if g["grid"]["nrad"] < 1000 else 3
+    grid = ctx.make_flat_grid([[0.0, 0.0, 0.0]], [Z], nrad=g["grid"]["nrad"], nang_order=order,
+                              radial="becke")
+    basis = nk.make_manual_basis([(1, 0, 0, g["zeta"], 0.0, 0.0, 0.0)])
+    return g, grid, basis
+
+
+def test_gpu_lda_x_hydrogen_golden(ctx):
+    g, grid, basis = _single_atom(ctx, "h_lda_x")
+    phi = nk.fill_collocation(ctx, basis, grid.quad_points)
+    r = nk.compute_lda(ctx, phi, grid.weights, [[1.0]], [g["occ"]], g["func"])
+    assert abs(r.energy - g["energy"]) < g["rel"] * abs(g["energy"])
+
+
+@pytest.mark.parametrize("key", ["h_pbe_x", "he_pbe_x"])
+def test_gpu_pbe_x_goldens(ctx, key):
+    g, grid, basis = _single_atom(ctx, key)
+    phi = nk.fill_collocation(ctx, basis, grid.quad_points)
+    gx, gy, gz = nk.fill_grad_collocation(ctx, basis, grid.quad_points)
+    r = nk.compute_gga(ctx, phi, gx, gy, gz, grid.weights, [[1.0]], [g["occ"]], g["func"])
+    assert abs(r.energy - g["energy"]) < g["rel"] * abs(g["energy"])
+    assert abs(r.potential[0, 0].item() - g["v00"]) < g["rel"] * abs(g["v00"])
+    # the batched build gives the same numbers from P = f C C^T
+    r2 = ctx.vxc(torch.tensor([[g["occ"]]], dtype=torch.float64), funcs=g["func"])
+    assert abs(r2.energy - g["energy"]) < g["rel"] * abs(g["energy"])
+    assert abs(r2.potential[0, 0].item() - g["v00"]) < g["rel"] * abs(g["v00"])
+
+
+@pytest.mark.parametrize("key", ["h_density", "he_density"])
+def test_gpu_density_integrals(ctx, key):
+    g, grid, basis = _single_atom(ctx, key)
+    phi = nk.fill_collocation(ctx, basis, grid.quad_points)
+    gx, gy, gz = nk.fill_grad_collocation(ctx, basis, grid.quad_points)
+    rho, grad, sig = nk.compute_density_and_sigma(ctx, phi, gx, gy, gz, [[1.0]], [g["occ"]])
+    w = grid.weights
+    z = g["zeta"]
+    ref_sig = 1 / (2 * math.pi) if key == "h_density" else 2 * z ** 5 / math.pi
+    assert abs((w * rho).sum().item() - g["int_rho"]) < g["rel"] * g["int_rho"]
+    assert abs((w * sig).sum().item() - ref_sig) < g["rel"] * ref_sig
+
+
+def test_gga_rejects_lda_functional(ctx):
+    g, grid, basis = _single_atom(ctx, "h_lda_x")
+    phi = nk.fill_collocation(ctx, basis, grid.quad_points)
+    gx, gy, gz = nk.fill_grad_collocation(ctx, basis, grid.quad_points)
+    with pytest.raises(nk.NukexcError) as e:
+        nk.compute_gga(ctx, phi, gx, gy, gz, grid.weights, [[1.0]], [1.0], [1])
+    assert e.value.status == "NKX_ERR_FUNCTIONAL_FAMILY"
+    with pytest.raises(nk.NukexcError) as e:
+        nk.compute_lda(ctx, phi, grid.weights, [[1.0]], [1.0], [101])
+    assert e.value.status == "NKX_ERR_FUNCTIONAL_FAMILY"
+
+
+def test_vxc_needs_basis():
+    c = nk.Context(0)
+    c.make_flat_grid([[0.0, 0, 0]], [1], nrad=10, nang_order=5)
+    with pytest.raises(nk.NukexcError) as e:
+        c.vxc(torch.ones((1, 1), dtype=torch.float64))
+    assert e.value.status == "NKX_ERR_STATE"
+    c.close()
+
+
+# ----------------------------------------------------- batched V_xc (FP64) ----
+def _workload_on_gpu_grid(ctx, name, **grid_kw):
+    w = configs.workload(name)
+    gkw = dict(w["grid"])
+    gkw.update(grid_kw)
+    grid = ctx.make_flat_grid(w["xyz"], w["Z"], **gkw)
+    ctx.set_basis(w["basis"])
+    pts = grid.quad_points.cpu().numpy()
+    wts = grid.weights.cpu().numpy()
+    S, C, P = orbitals_on_grid(w, pts, wts)
+    return w, pts, wts, S, C, P
+
+
+@pytest.mark.parametrize("name,grid_kw", [("water", {}), ("benzene", dict(nrad=50, nang_order=29))])
+def test_vxc_fp64_matches_oracle(ctx, name, grid_kw):
+    w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, name, **grid_kw)
+    gga = any(f > 100 for f in w["funcs"])
+    phi, gx, gy, gz = oracle_on_grid(w["basis"], pts, grad=True)
+    E0, N0, V0 = O.vxc_from_collocation(phi, wts, w["funcs"], P=P,
+                                        gx=gx if gga else None, gy=gy if gga else None,
+                                        gz=gz if gga else None)
+    r = ctx.vxc(dev(P), funcs=w["funcs"], precision="fp64")
+    V = r.potential.cpu().numpy()
+    print(f"{name}: npts={len(wts)} E={r.energy:.12f} dE={r.energy - E0:.2e} "
+          f"max|dV|={np.abs(V - V0).max():.2e} N_el={r.n_el:.12f}")
+    assert abs(r.energy - E0) <= 1e-8
+    assert np.abs(V - V0).max() <= 1e-9
+    assert np.array_equal(V, V.T)
+    # N_el = tr(P S_num) = 2 N_occ exactly for Loewdin orbitals in the same-grid S
+    assert abs(r.n_el - 2 * w["nocc"]) < 1e-9
+    assert abs(N0 - 2 * w["nocc"]) < 1e-9
+
+
+def test_vxc_fp64_matches_reference_shaped_path(ctx):
+    """nkx_vxc (batched, P route) == nkx_xc_from_collocation (compute_gga, MO
+    route) on the same grid: two independent GPU pipelines."""
+    w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, "benzene", nrad=30, nang_order=17)
+    grid = ctx.grid
+    phi = nk.fill_collocation(ctx, w["basis"], grid.quad_points)
+    gx, gy, gz = nk.fill_grad_collocation(ctx, w["basis"], grid.quad_points)
+    r1 = nk.compute_gga(ctx, phi, gx, gy, gz, grid.weights, C, 2.0 * np.ones(C.shape[1]),
+                        w["funcs"])
+    r2 = ctx.vxc(dev(P), funcs=w["funcs"])
+    assert abs(r1.energy - r2.energy) < 1e-9
+    assert (r1.potential - r2.potential).abs().max().item() < 1e-10
+
+
+def test_overlap_matches_oracle(ctx):
+    w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, "water", nrad=40, nang_order=17)
+    Sg = ctx.overlap().cpu().numpy()
+    assert np.abs(Sg - S).max() < 1e-12
