@@ -27,6 +27,10 @@ namespace nkx {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// v_mfma_f64_16x16x4f64 lane layout (checked by tests/gpu_probe): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; result register r of lane l
+// holds D[4r + l/16][l%16].
+
 constexpr int FB = 64;   // functions per block
 constexpr int PT = 64;   // points per batch
 constexpr int TILE = FB * PT;
@@ -128,7 +132,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int mu = 16 * wv + 4 * q + r;
+        const int mu = 16 * wv + 4 * r + q;
         const double x = acc[j][r];
         s[0] += tb[mu * PT + g] * x;
         if (gga) {
@@ -265,7 +269,7 @@ __global__ void __launch_bounds__(256) vgemm_f64(
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int mu = 16 * wv + 4 * q + r, nu = 16 * j + l16;
+      const int mu = 16 * wv + 4 * r + q, nu = 16 * j + l16;
       out[mu * FB + nu] += acc[j][r];
     }
 }

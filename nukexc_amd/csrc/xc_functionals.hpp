@@ -41,9 +41,9 @@ __host__ __device__ inline void slater_x(double rho, double r13, XcPoint &o) {
 __host__ __device__ inline void pw92(double rs, double A, double &eps, double &deps) {
   const double a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
   const double s = sqrt(rs);
-  const double den = 2.0 * A * s * (b1 + s * (b2 + s * (b3 + b4 * s)));
+  const double den = 2.0 * A * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs);
   const double dden = A * (b1 / s + 2.0 * b2 + 3.0 * b3 * s + 4.0 * b4 * rs);
-  const double lg = log1p(1.0 / den);
+  const double lg = log(1.0 + 1.0 / den);  // libxc's form (not log1p)
   const double pre = -2.0 * A * (1.0 + a1 * rs);
   eps = pre * lg;
   deps = -2.0 * A * a1 * lg - pre * dden / (den * (den + 1.0));
@@ -92,7 +92,7 @@ __host__ __device__ inline void pbe_c(double rho, double r13, double sigma, XcPo
   const double dnm = 1.0 + At2 * (1.0 + At2);
   const double inv = 1.0 / dnm;
   const double y = bg * t2 * num * inv;
-  const double H = gam * log1p(y);
+  const double H = gam * log(1.0 + y);
   const double Hy = gam / (1.0 + y);
   // dy/dt2 = bg (1 + 2 A t2) / dnm^2 ;  dy/dA = -bg t2^2 (A t2)(2 + A t2) / dnm^2
   const double dyt = bg * (1.0 + 2.0 * At2) * inv * inv;

@@ -307,10 +307,13 @@ XcOut gga_c_pbe(double rho, double sigma) {
   const double y = beta / gamma * t2 * num / dnm;
   const double H = gamma * std::log(1.0 + y);
   const double dH_dy = gamma / (1.0 + y);
-  // dy/dt2 and dy/dA
+  // dy/dt2 and dy/dA from the quotient rule, simplified so that nothing
+  // cancels when A t^2 >> 1:  with u = A t^2,
+  //   d/dt2 [t2 (1+u)/(1+u+u^2)] = (1 + 2u) / (1+u+u^2)^2
+  //   d/dA  [t2 (1+u)/(1+u+u^2)] = -t2^2 u (2 + u) / (1+u+u^2)^2
   const double bg = beta / gamma;
-  const double dy_dt2 = bg * (num / dnm + t2 * (A * dnm - num * (A + 2.0 * A * At2)) / (dnm * dnm));
-  const double dy_dA = bg * t2 * (t2 * dnm - num * (t2 + 2.0 * At2 * t2)) / (dnm * dnm);
+  const double dy_dt2 = bg * (1.0 + 2.0 * At2) / (dnm * dnm);
+  const double dy_dA = -bg * t2 * t2 * At2 * (2.0 + At2) / (dnm * dnm);
   const double dt2_drho = -7.0 / 3.0 * t2 / rho;
   const double dH_drho = dH_dy * (dy_dA * dA_dec * dec + dy_dt2 * dt2_drho);
   const double e = rho * (ec + H);
