@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the PBE V_xc build on one MI355X.
+
+A step is one V_xc build -- collocation, density and density gradient,
+on-device PBE, Z, V_xc, E_xc and N_el -- over the whole grid of the workload,
+from a density matrix already resident in HBM.  Default workload: the
+100-water cluster (TA 75 x Lebedev 302 per atom, 2500 STO functions, PBE).
+
+Prints one JSON line.  The roofline object is built from HIP events around
+each kernel launch inside the timed region (ctx stream) and the plan's
+algorithmic flop count; cpu_baseline times the FP64 CPU restatement of the
+reference's dense algorithm on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0}
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup(name):
+    import numpy as np
+    import torch
+
+    import nukexc_amd as nk
+    from nukexc_amd import configs
+
+    w = configs.workload(name)
+    ctx = nk.Context(0)
+    t0 = time.perf_counter()
+    grid = ctx.make_flat_grid(w["xyz"], w["Z"], **w["grid"])
+    ctx.set_basis(w["basis"])
+    log(f"{name}: {len(w['Z'])} atoms, {w['basis'].nbf()} functions, "
+        f"{grid.weights.shape[0]} points (grid {time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    S = ctx.overlap().cpu().numpy()
+    C, P = configs.loewdin_orbitals(S, w["nocc"], w["rng"])
+    log(f"overlap + Loewdin orbitals {time.perf_counter() - t0:.1f} s; plan {ctx.plan_stats()}")
+    Pd = torch.as_tensor(P, device="cuda")
+    return w, ctx, grid, C, Pd
+
+
+def cpu_baseline(w, grid, C, budget_s=15.0, chunk=16384):
+    """FP64 CPU restatement of the reference dense algorithm (collocation,
+    MO-route density with GEMMs, functional, Z, Phi Z^T), on chunks of a
+    seeded random sample of the grid until about budget_s of work."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    pts = grid.quad_points.cpu().numpy()
+    wts = grid.weights.cpu().numpy()
+    rng = np.random.default_rng(12345)
+    order = rng.permutation(len(wts))
+    b = w["basis"]
+    basis = dict(n=b.n, l=b.l, m=b.m, zeta=b.zeta, norm=b.norm, centers=b.O)
+    gga = any(f > 100 for f in w["funcs"])
+    done, t_total = 0, 0.0
+    while t_total < budget_s and done < len(order):
+        idx = order[done:done + chunk]
+        p, ww = pts[idx], wts[idx]
+        t0 = time.perf_counter()
+        phi, gx, gy, gz = O.collocation(basis, p, grad=True)
+        psi = C.T @ phi
+        rho = 2.0 * np.einsum("ig,ig->g", psi, psi)
+        if gga:
+            gr = np.stack([4.0 * np.einsum("ig,ig->g", psi, C.T @ g) for g in (gx, gy, gz)])
+            sig = (gr * gr).sum(0)
+        else:
+            sig = None
+        exc, vr, vs = O.xc_eval(list(w["funcs"]), rho, sig)
+        Z = 0.5 * phi * vr
+        if gga:
+            Z += 2.0 * vs * (gr[0] * gx + gr[1] * gy + gr[2] * gz)
+        Z *= ww
+        Vb = phi @ Z.T
+        _ = Vb + Vb.T
+        _ = float(np.sum(ww * rho * exc))
+        t_total += time.perf_counter() - t0
+        done += len(idx)
+    return done, t_total, O.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="water100")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path "
+              "(the per-GPU share of a sharded build is the single-GPU workload).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    w, ctx, grid, C, P = setup(args.workload)
+    funcs = w["funcs"]
+    npts = grid.weights.shape[0]
+
+    ref = None
+    if args.precision != "fp64":
+        ref = ctx.vxc(P, funcs, precision="fp64")
+        log(f"FP64 reference build: E_xc = {ref.energy:.10f}")
+
+    for i in range(args.warmup):
+        r = ctx.vxc(P, funcs, precision=args.precision)
+        log(f"warmup {i}: E_xc = {r.energy:.10f} N_el = {r.n_el:.10f}")
+
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = ctx.vxc(P, funcs, precision=args.precision)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    tm = ctx.timing()
+    work = ctx.plan_work()
+    ms_step = 1e3 * elapsed / args.steps
+    log(f"timed {args.steps} builds: {ms_step:.2f} ms/build; stages {tm}")
+
+    # Dominant kernel and its roofline (MFMA-bound contractions).
+    stage_flops = {"density": work["flops_density"], "vgemm": work["flops_v"]}
+    dom = max(("density", "vgemm", "collocation"), key=lambda k: tm[k][0])
+    dom_ms, dom_launches = tm[dom]
+    if dom == "collocation":
+        elem_bytes = (8 if args.precision == "fp64" else 4) * (4 if len(funcs) and max(funcs) > 100 else 1)
+        traffic_alg = work["colloc_elems"] * elem_bytes
+        achieved = traffic_alg * args.steps / (dom_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None}
+    else:
+        peak = PEAK_TFLOPS["f64" if args.precision == "fp64" else "bf16"]
+        achieved = stage_flops[dom] * args.steps / (dom_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
+    roof.update({"kernel": {"density": "density_fused", "vgemm": "vgemm",
+                            "collocation": "colloc_tiles"}[dom],
+                 "avg_launch_ms": dom_ms / max(dom_launches, 1),
+                 "launches": dom_launches})
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        n_cpu, t_cpu, cores = cpu_baseline(w, grid, C, budget_s=args.cpu_budget)
+        cpu = {"value": n_cpu / t_cpu, "unit": "grid pts/s", "cores": cores, "kind": "port",
+               "sample": f"{n_cpu} of {npts} grid points (seeded random sample, 16384-point "
+                         f"chunks), full basis, FP64 dense algorithm: collocation, "
+                         f"MO-route density by OpenBLAS GEMMs, functional, Z, Phi Z^T"}
+        log(f"cpu baseline: {n_cpu} pts in {t_cpu:.1f} s on {cores} threads")
+
+    out = {
+        "metric": "grid pts/s for PBE Vxc build, mixed vs FP64 on 1 GPU at |dE_xc|<=1e-6 Eh",
+        "value": npts / (ms_step * 1e-3),
+        "unit": "grid pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "fp32+bf16x3 (f64 functional/reductions)",
+        "data": "synthetic (seeded geometry, STO exponents and orbitals)",
+        "config": {"workload": f"{w['name']}: {len(w['Z'])} atoms, {w['basis'].nbf()} STO "
+                               f"functions, TA {w['grid']['nrad']} x Lebedev order "
+                               f"{w['grid']['nang_order']}, {npts} points, "
+                               f"{'PBE' if max(funcs) > 100 else 'LDA_X'}",
+                   "precision": args.precision, "builds_per_s": 1e3 / ms_step,
+                   "E_xc": r.energy, "N_el": r.n_el},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if ref is not None:
+        out["config"]["dE_xc_vs_fp64"] = r.energy - ref.energy
+        out["config"]["max_dV_vs_fp64"] = float((r.potential - ref.potential).abs().max())
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -195,6 +195,21 @@ int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);
    screened (function-block, batch) pairs kept, total pairs. */
 int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npts_padded, int64_t *nbatches,
                    int64_t *pairs_kept, int64_t *pairs_total);
+/* Kernel timing (the reference driver's TimingRegistry,
+   tests/standalone_helpers.hpp:71-156, at kernel granularity): when enabled,
+   nkx_vxc records HIP events on the ctx stream around each launch.  Stages:
+   0 collocation, 1 density contraction + functional, 2 V contraction,
+   3 gather / reductions, 4 whole build.  ms is the summed event time since
+   the last reset, launches the number of timed launches. */
+int nkx_set_timing(nkx_ctx *ctx, int enable);
+int nkx_reset_timing(nkx_ctx *ctx);
+int nkx_get_timing(nkx_ctx *ctx, int stage, double *ms, int64_t *launches);
+/* Algorithmic work of one nkx_vxc build on the current plan, counting true
+   points g_b and true screened functions n_b per batch b:
+   density flops 2 sum_b n_b^2 g_b, V flops 2 sum_b n_b^2 g_b,
+   collocation elements sum_b n_b g_b. */
+int nkx_plan_work(nkx_ctx *ctx, double *flops_density, double *flops_v,
+                  double *colloc_elems);
 /* Numerical overlap S = Phi diag(w) Phi^T on the ctx grid (FP64, row-major,
    device); core_hamiltonian.hpp:62-119 overlap_integral. */
 int nkx_overlap(nkx_ctx *ctx, double *S_dev);

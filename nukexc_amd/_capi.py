@@ -66,6 +66,10 @@ SIGNATURES = {
     "nkx_set_screening": [_vp, _c_double, _c_int],
     "nkx_plan_stats": [_vp, _pi64, _pi64, _pi64, _pi64, _pi64],
     "nkx_overlap": [_vp, _vp],
+    "nkx_set_timing": [_vp, _c_int],
+    "nkx_reset_timing": [_vp],
+    "nkx_get_timing": [_vp, _c_int, _pd, _pi64],
+    "nkx_plan_work": [_vp, _pd, _pd, _pd],
 }
 _RESTYPE = {"nkx_last_error": ctypes.c_char_p}
 

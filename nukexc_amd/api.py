@@ -256,6 +256,28 @@ class Context:
                                 _ptr(V)))
         return XC_result(E.value, V, N.value)
 
+    STAGES = ("collocation", "density", "vgemm", "final", "build")
+
+    def set_timing(self, on=True):
+        self._chk(lib().nkx_set_timing(self.h, int(bool(on))))
+
+    def reset_timing(self):
+        self._chk(lib().nkx_reset_timing(self.h))
+
+    def timing(self):
+        """{stage: (ms summed since reset, launches)} from HIP events."""
+        out = {}
+        for k, name in enumerate(self.STAGES):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            self._chk(lib().nkx_get_timing(self.h, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+    def plan_work(self):
+        v = [ctypes.c_double() for _ in range(3)]
+        self._chk(lib().nkx_plan_work(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("flops_density", "flops_v", "colloc_elems"), [x.value for x in v]))
+
     def overlap(self):
         """Numerical overlap on the ctx grid (core_hamiltonian.hpp:62-119)."""
         nbf = self.basis.nbf()

@@ -555,6 +555,42 @@ int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npad, int64_t *nb, int6
   });
 }
 
+int nkx_set_timing(nkx_ctx *ctx, int enable) {
+  NKX_CTX_CHECK(ctx);
+  ctx->c.timing.on = enable != 0;
+  return NKX_OK;
+}
+
+int nkx_reset_timing(nkx_ctx *ctx) {
+  NKX_CTX_CHECK(ctx);
+  for (int k = 0; k < nkx::kNumStages; ++k) {
+    ctx->c.timing.ms[k] = 0;
+    ctx->c.timing.launches[k] = 0;
+  }
+  return NKX_OK;
+}
+
+int nkx_get_timing(nkx_ctx *ctx, int stage, double *ms, int64_t *launches) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(stage >= 0 && stage < nkx::kNumStages, NKX_ERR_INVALID_ARGUMENT,
+                "unknown timing stage");
+    if (ms) *ms = ctx->c.timing.ms[stage];
+    if (launches) *launches = ctx->c.timing.launches[stage];
+  });
+}
+
+int nkx_plan_work(nkx_ctx *ctx, double *fd, double *fv, double *ce) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::ensure_plan(ctx->c);
+    const nkx::Plan &p = ctx->c.plan;
+    if (fd) *fd = 2.0 * p.work_nb2g;
+    if (fv) *fv = 2.0 * p.work_nb2g;
+    if (ce) *ce = p.work_nbg;
+  });
+}
+
 int nkx_overlap(nkx_ctx *ctx, double *S) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

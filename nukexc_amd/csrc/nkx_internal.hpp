@@ -117,6 +117,7 @@ struct Plan {
   DevBuf<int> tile_slot;          // (fb, batch) -> slot in the tile store or -1
   int64_t nslots = 0;
   int64_t pairs_kept = 0;
+  double work_nb2g = 0, work_nbg = 0;  // sum_b n_b^2 g_b, sum_b n_b g_b
 };
 
 struct Options {
@@ -137,8 +138,36 @@ struct Workspace {
   DevBuf<double> scratch;
 };
 
+// Per-stage kernel timing with HIP events on the ctx stream
+// (the reference's driver-level TimingRegistry, standalone_helpers.hpp:71-156,
+// at kernel granularity).
+enum Stage { kStageColloc = 0, kStageDensity = 1, kStageVgemm = 2, kStageFinal = 3,
+             kStageBuild = 4, kNumStages = 5 };
+struct Timing {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  std::vector<int> pending_stage;  // stage of each recorded pair this build
+  double ms[kNumStages] = {0, 0, 0, 0, 0};
+  int64_t launches[kNumStages] = {0, 0, 0, 0, 0};
+  // Records the start of a timed launch; returns its slot (-1 when off).
+  int begin(int stage, hipStream_t s);
+  void end(int slot, hipStream_t s);
+  void collect();  // after a stream synchronisation
+  void release();
+  ~Timing() { release(); }
+};
+
+// Algorithmic work of one build on the current plan (true functions and true
+// points only): density contraction flops 2 sum_b n_b^2 g_b, V contraction
+// flops 2 sum_b n_b^2 g_b, collocation elements sum_b n_b g_b.
+struct Work {
+  double flops_density = 0, flops_v = 0, colloc_elems = 0;
+};
+
 struct Ctx {
   int device = 0;
+  Timing timing;
+  Work work;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;

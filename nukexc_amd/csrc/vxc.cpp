@@ -203,6 +203,20 @@ void ensure_plan(Ctx &c) {
                 "too many live (block, batch) pairs for 32-bit slot numbers");
   }
   b_off[nb] = (int)b_fbl.size();
+  // algorithmic work: true functions of the live blocks, true points
+  {
+    double nb2g = 0, nbg = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      double nfun = 0;
+      for (int k = b_off[b]; k < b_off[b + 1]; ++k)
+        nfun += std::min(nbf, (b_fbl[k] + 1) * FB) - b_fbl[k] * FB;
+      const double g = (double)std::min<int64_t>(PT, npts - b * PT);
+      nb2g += nfun * nfun * g;
+      nbg += nfun * g;
+    }
+    pl.work_nb2g = nb2g;
+    pl.work_nbg = nbg;
+  }
   if (b_fbl.empty()) {  // keep device buffers non-empty
     b_fbl.push_back(0);
     slot_b.push_back(0);
@@ -320,22 +334,77 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   ws.vpart.reserve(vsz);
   ws.red.reserve(2);
 
+  Timing &tm = c.timing;
+  const int t_build = tm.begin(kStageBuild, s);
+  int t = tm.begin(kStageFinal, s);
   vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
   NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, vsz * sizeof(double), s));
   NKX_HIP_CHECK(hipMemsetAsync(V, 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
+  tm.end(t, s);
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
+    int tt = tm.begin(kStageColloc, s);
     vk_colloc(s, s0, s1, pl, narr, gga ? 1 : 0, ws.tiles64.p);
+    tm.end(tt, s);
+    tt = tm.begin(kStageDensity, s);
     vk_density(s, b0, b1, s0, pl, ws.Pd.p, narr, gga ? 1 : 0, ws.tiles64.p, xl, ws.coef.p,
                ws.blockEN.p);
+    tm.end(tt, s);
+    tt = tm.begin(kStageVgemm, s);
     vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
+    tm.end(tt, s);
   });
+  t = tm.begin(kStageFinal, s);
   vk_vreduce(s, pl, nsplit, ws.vpart.p, V);
   vk_en_reduce(s, pl.nb, ws.blockEN.p, ws.red.p);
+  tm.end(t, s);
+  tm.end(t_build, s);
   double en[2];
   NKX_HIP_CHECK(hipMemcpyAsync(en, ws.red.p, sizeof(en), hipMemcpyDeviceToHost, s));
   NKX_HIP_CHECK(hipStreamSynchronize(s));
+  tm.collect();
+  c.work.flops_density = 2.0 * pl.work_nb2g;
+  c.work.flops_v = 2.0 * pl.work_nb2g;
+  c.work.colloc_elems = pl.work_nbg;
   *E = en[0];
   *N = en[1];
+}
+
+int Timing::begin(int stage, hipStream_t s) {
+  if (!on) return -1;
+  const size_t k = pending_stage.size();
+  if (k >= pool.size()) {
+    hipEvent_t a, b;
+    NKX_HIP_CHECK(hipEventCreate(&a));
+    NKX_HIP_CHECK(hipEventCreate(&b));
+    pool.emplace_back(a, b);
+  }
+  pending_stage.push_back(stage);
+  NKX_HIP_CHECK(hipEventRecord(pool[k].first, s));
+  return (int)k;
+}
+
+void Timing::end(int slot, hipStream_t s) {
+  if (slot < 0) return;
+  NKX_HIP_CHECK(hipEventRecord(pool[slot].second, s));
+}
+
+void Timing::collect() {
+  for (size_t k = 0; k < pending_stage.size(); ++k) {
+    float ms_k = 0.f;
+    NKX_HIP_CHECK(hipEventElapsedTime(&ms_k, pool[k].first, pool[k].second));
+    ms[pending_stage[k]] += ms_k;
+    launches[pending_stage[k]] += 1;
+  }
+  pending_stage.clear();
+}
+
+void Timing::release() {
+  for (auto &e : pool) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  pool.clear();
+  pending_stage.clear();
 }
 
 void overlap_build(Ctx &c, double *S) {
