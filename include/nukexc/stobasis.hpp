@@ -1,0 +1,96 @@
+// nukexc/stobasis.hpp -- STO basis and collocation (stobasis.hpp:54-225,
+// :494-584 of the reference).  The basis lives on the host (it is small) and
+// is handed to the library when a collocation or XC call needs it.
+#pragma once
+
+#include <cmath>
+
+#include "nukexc_config.hpp"
+
+namespace Nukexc {
+
+struct STOBasisSet {
+  std::vector<int> n, l, m;
+  std::vector<double> norm, zeta, cutoff_radii;
+  std::vector<Point> O;
+  size_t nbf() const { return O.size(); }
+};
+
+struct STOFunc {
+  int n, l, m;
+  double zeta;
+  double ox, oy, oz;
+};
+
+// N = (2 zeta)^{n+1/2} / sqrt((2n)!), cutoff radius at cutoff_tol.
+inline STOBasisSet make_manual_basis(const std::vector<STOFunc> &funcs,
+                                     double cutoff_tol = 1e-10) {
+  STOBasisSet b;
+  for (const auto &f : funcs) {
+    double nrm = 0, cut = 0;
+    check(nkx_sto_norm_cutoff(f.n, f.zeta, cutoff_tol, &nrm, &cut));
+    b.n.push_back(f.n);
+    b.l.push_back(f.l);
+    b.m.push_back(f.m);
+    b.zeta.push_back(f.zeta);
+    b.norm.push_back(nrm);
+    b.cutoff_radii.push_back(cut);
+    Point p;
+    p[0] = f.ox;
+    p[1] = f.oy;
+    p[2] = f.oz;
+    b.O.push_back(p);
+  }
+  return b;
+}
+
+inline bool same_basis(const STOBasisSet &a, const STOBasisSet &b) {
+  if (a.nbf() != b.nbf() || a.n != b.n || a.l != b.l || a.m != b.m || a.zeta != b.zeta ||
+      a.norm != b.norm)
+    return false;
+  for (size_t i = 0; i < a.nbf(); ++i)
+    for (int d = 0; d < 3; ++d)
+      if (a.O[i][d] != b.O[i][d]) return false;
+  return true;
+}
+
+// Hands the basis to the thread's ctx; a basis equal to the last one is not
+// re-sent, so the ctx keeps its launch plan.
+inline void set_basis(const STOBasisSet &b) {
+  thread_local STOBasisSet last;
+  thread_local bool have = false;
+  if (have && same_basis(last, b)) return;
+  std::vector<double> c;
+  for (const auto &p : b.O)
+    for (int d = 0; d < 3; ++d) c.push_back(p[d]);
+  check(nkx_set_basis(context().get(), (int)b.nbf(), b.n.data(), b.l.data(), b.m.data(),
+                      b.zeta.data(), b.norm.data(), c.data(), b.cutoff_radii.data()),
+        context().get());
+  last = b;
+  have = true;
+}
+
+// fill_collocation: values(mu, g), LayoutLeft (N_bf, N_pts), allocated by the caller.
+inline void fill_collocation(const STOBasisSet &basis, const DeviceArray<Point> &points,
+                             DeviceArray<double> &values) {
+  set_basis(basis);
+  check(nkx_collocation(context().get(), (int64_t)points.size(),
+                        reinterpret_cast<const double *>(points.data()), values.data(), nullptr,
+                        nullptr, nullptr),
+        context().get());
+  context().fence();
+}
+
+// fill_grad_collocation: d/dx, d/dy, d/dz, same layout.
+inline void fill_grad_collocation(const STOBasisSet &basis, const DeviceArray<Point> &points,
+                                  DeviceArray<double> &gx, DeviceArray<double> &gy,
+                                  DeviceArray<double> &gz) {
+  set_basis(basis);
+  check(nkx_collocation(context().get(), (int64_t)points.size(),
+                        reinterpret_cast<const double *>(points.data()), nullptr, gx.data(),
+                        gy.data(), gz.data()),
+        context().get());
+  context().fence();
+}
+
+}  // namespace Nukexc

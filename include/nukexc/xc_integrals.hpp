@@ -1,0 +1,98 @@
+// nukexc/xc_integrals.hpp -- E_xc and V_xc (xc_integrals.hpp:62-166, :424-541
+// of the reference).  The reference passes a libxc xc_func_type and evaluates
+// the functional on the host; here a small Functional handle carries libxc ids
+// and the functional runs on the GPU.
+#pragma once
+
+#include "density.hpp"
+#include "grid.hpp"
+#include "stobasis.hpp"
+
+namespace Nukexc {
+
+struct XC_result {
+  double energy = 0.0;
+  DeviceView2DLeft potential;  // (N_bf, N_bf); symmetric
+  double n_el = 0.0;           // sum_g w rho (batched build only)
+};
+
+// libxc-numbered functional list (LDA_X = 1, GGA_X_PBE = 101, GGA_C_PBE = 130,
+// LDA_C_PW = 12, LDA_C_PW_MOD = 13); family as func.info->family.
+struct Functional {
+  std::vector<int> ids;
+  int family = NKX_FAMILY_LDA;
+  explicit Functional(std::vector<int> list) : ids(std::move(list)) {
+    for (int id : ids) {
+      int fam = 0;
+      check(nkx_functional_info(id, &fam, nullptr));
+      if (fam == NKX_FAMILY_GGA) family = NKX_FAMILY_GGA;
+    }
+  }
+};
+
+inline XC_result compute_lda(const DeviceView2DLeft &collocation_values,
+                             const DeviceView1D &weights, const DeviceView2DLeft &mo_orbitals,
+                             const DeviceView1D &mo_coeff, const Functional &func) {
+  const size_t nbf = collocation_values.extent(0), npts = collocation_values.extent(1);
+  XC_result r;
+  r.potential.allocate(nbf, nbf);
+  check(nkx_xc_from_collocation(context().get(), (int)nbf, (int64_t)npts,
+                                collocation_values.data(), nullptr, nullptr, nullptr,
+                                weights.data(), (int)mo_orbitals.extent(1), mo_orbitals.data(),
+                                mo_coeff.data(), (int)func.ids.size(), func.ids.data(),
+                                NKX_FAMILY_LDA, &r.energy, r.potential.data()),
+        context().get());
+  return r;
+}
+
+inline XC_result compute_gga(const DeviceView2DLeft &collocation_values,
+                             const DeviceView2DLeft &collocation_gx,
+                             const DeviceView2DLeft &collocation_gy,
+                             const DeviceView2DLeft &collocation_gz,
+                             const DeviceView1D &weights, const DeviceView2DLeft &mo_orbitals,
+                             const DeviceView1D &mo_coeff, const Functional &func) {
+  const size_t nbf = collocation_values.extent(0), npts = collocation_values.extent(1);
+  XC_result r;
+  r.potential.allocate(nbf, nbf);
+  check(nkx_xc_from_collocation(context().get(), (int)nbf, (int64_t)npts,
+                                collocation_values.data(), collocation_gx.data(),
+                                collocation_gy.data(), collocation_gz.data(), weights.data(),
+                                (int)mo_orbitals.extent(1), mo_orbitals.data(), mo_coeff.data(),
+                                (int)func.ids.size(), func.ids.data(), NKX_FAMILY_GGA,
+                                &r.energy, r.potential.data()),
+        context().get());
+  return r;
+}
+
+// Batched build from a density matrix (row-major = LayoutLeft for symmetric
+// P): the per-SCF-iteration XC call, without materialising N_bf x N_pts
+// collocation matrices.
+enum class Precision { FP64 = NKX_PREC_FP64, Mixed = NKX_PREC_MIXED };
+
+inline XC_result compute_vxc(const FlatGrid &grid, const STOBasisSet &basis,
+                             const DeviceView2DLeft &P, const Functional &func,
+                             Precision prec = Precision::FP64) {
+  nkx_ctx *ctx = context().get();
+  // the grid is handed over once; repeated builds on it reuse the launch plan
+  thread_local const void *last_grid = nullptr;
+  thread_local size_t last_npts = 0;
+  if (last_grid != grid.weights.data() || last_npts != grid.weights.size()) {
+    check(nkx_set_grid(ctx, (int)grid.atom_centers.size(),
+                       reinterpret_cast<const double *>(grid.atom_centers.data()),
+                       (int64_t)grid.weights.size(),
+                       reinterpret_cast<const double *>(grid.quad_points.data()),
+                       grid.weights.data(), grid.point_owner.data()),
+          ctx);
+    last_grid = grid.weights.data();
+    last_npts = grid.weights.size();
+  }
+  set_basis(basis);
+  XC_result r;
+  r.potential.allocate(basis.nbf(), basis.nbf());
+  check(nkx_vxc(ctx, P.data(), (int)func.ids.size(), func.ids.data(), (int)prec, &r.energy,
+                &r.n_el, r.potential.data()),
+        ctx);
+  return r;
+}
+
+}  // namespace Nukexc

@@ -191,6 +191,9 @@ int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);
+/* Number of bf16 products per split contraction in the mixed form: 6 (default;
+   all terms down to O(2^-24)) or 3 (a0 b0 + a0 b1 + a1 b0). */
+int nkx_set_mixed_products(nkx_ctx *ctx, int nprod);
 /* Statistics of the current plan: number of points, padded points, batches,
    screened (function-block, batch) pairs kept, total pairs. */
 int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npts_padded, int64_t *nbatches,

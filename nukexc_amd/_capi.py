@@ -64,6 +64,7 @@ SIGNATURES = {
                                 _c_int, _pi, _c_int, _pd, _vp],
     "nkx_vxc": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp],
     "nkx_set_screening": [_vp, _c_double, _c_int],
+    "nkx_set_mixed_products": [_vp, _c_int],
     "nkx_plan_stats": [_vp, _pi64, _pi64, _pi64, _pi64, _pi64],
     "nkx_overlap": [_vp, _vp],
     "nkx_set_timing": [_vp, _c_int],

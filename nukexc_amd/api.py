@@ -235,6 +235,10 @@ class Context:
     def set_screening(self, tol, batch_points=0):
         self._chk(lib().nkx_set_screening(self.h, float(tol), int(batch_points)))
 
+    def set_mixed_products(self, nprod):
+        """bf16 products per split contraction in the mixed form (3 or 6)."""
+        self._chk(lib().nkx_set_mixed_products(self.h, int(nprod)))
+
     def plan_stats(self):
         v = [ctypes.c_int64() for _ in range(5)]
         self._chk(lib().nkx_plan_stats(self.h, *[ctypes.byref(x) for x in v]))

@@ -541,6 +541,14 @@ int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
   });
 }
 
+int nkx_set_mixed_products(nkx_ctx *ctx, int nprod) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nprod == 3 || nprod == 6, NKX_ERR_INVALID_ARGUMENT, "products must be 3 or 6");
+    ctx->c.opt.mixed_products = nprod;
+  });
+}
+
 int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npad, int64_t *nb, int64_t *kept,
                    int64_t *total) {
   NKX_CTX_CHECK(ctx);

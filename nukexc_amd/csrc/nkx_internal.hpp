@@ -124,11 +124,12 @@ struct Options {
   double screen_tol = 1e-12;
   int batch_points = 64;
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
+  int mixed_products = 6;       // bf16 products per split contraction (3 or 6)
 };
 
 struct Workspace {
   DevBuf<double> Pd;               // permuted padded density matrix (FP64)
-  DevBuf<float> Pf;                // same, FP32 (mixed)
+  DevBuf<uint16_t> Ps;             // three bf16 planes of the permuted P (mixed)
   DevBuf<double> tiles64;          // Phi, gx, gy, gz, Z tiles (FP64)
   DevBuf<float> tiles32;           // same (mixed)
   DevBuf<double> coef;             // per padded point: a, bx, by, bz

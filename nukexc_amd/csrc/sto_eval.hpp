@@ -75,4 +75,73 @@ __device__ __forceinline__ void sto_value_grad(const FuncRec &f, const HarmTerm 
   gz = R * Sz + S * (dz * common);
 }
 
+// Mixed-form collocation: FP32 output values, each rounded once from an FP64
+// evaluation.  Rounding is then the only FP32 error, unbiased and independent
+// from point to point.  Evaluating in FP32 instead (zeta r, normalisation or
+// harmonic coefficients rounded to FP32, or an FP32 exponential) shifts every
+// value of a function by a common relative amount of 1e-8 .. 1e-7; such a
+// systematic error survives the sum over millions of grid points and alone
+// exceeds the mixed-form budget (|dE_xc| <= 1e-6 Eh) on large systems.
+// The exponential uses an argument reduction zeta r = k ln2 + f, |f| <= ln2/2,
+// and a degree-11 Taylor polynomial for e^{-f} (truncation < 3e-13).
+__host__ __device__ inline double exp_neg_mixed(double y) {
+  const double k = rint(y * 1.4426950408889634);  // y / ln2
+  double f = fma(-k, 6.93147180369123816490e-01, y);  // ln2 split hi / lo
+  f = fma(-k, 1.90821492927058770002e-10, f);
+  const double x = -f;
+  double p = 1.0 / 39916800.0;
+  p = fma(p, x, 1.0 / 3628800.0);
+  p = fma(p, x, 1.0 / 362880.0);
+  p = fma(p, x, 1.0 / 40320.0);
+  p = fma(p, x, 1.0 / 5040.0);
+  p = fma(p, x, 1.0 / 720.0);
+  p = fma(p, x, 1.0 / 120.0);
+  p = fma(p, x, 1.0 / 24.0);
+  p = fma(p, x, 1.0 / 6.0);
+  p = fma(p, x, 0.5);
+  p = fma(p, x, 1.0);
+  p = fma(p, x, 1.0);
+  return ldexp(p, -(int)k);
+}
+
+__device__ __forceinline__ float sto_value_mixed(const FuncRec &f, const HarmTerm *terms,
+                                                 double x, double y, double z) {
+  const double dx = x - f.cx, dy = y - f.cy, dz = z - f.cz;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double R = f.norm * ipow(r, f.k) * exp_neg_mixed(f.zeta * r);
+  double S = 0.0;
+  for (int t = 0; t < f.term_cnt; ++t) {
+    const HarmTerm h = terms[f.term_off + t];
+    S += h.coef * ipow(dx, h.a) * ipow(dy, h.b) * ipow(dz, h.c);
+  }
+  return (float)(R * S);
+}
+
+__device__ __forceinline__ void sto_value_grad_mixed(const FuncRec &f, const HarmTerm *terms,
+                                                     double x, double y, double z, float &v,
+                                                     float &gx, float &gy, float &gz) {
+  const double dx = x - f.cx, dy = y - f.cy, dz = z - f.cz;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double R = f.norm * ipow(r, f.k) * exp_neg_mixed(f.zeta * r);
+  double S = 0.0, Sx = 0.0, Sy = 0.0, Sz = 0.0;
+  for (int t = 0; t < f.term_cnt; ++t) {
+    const HarmTerm h = terms[f.term_off + t];
+    const double xa1 = ipow(dx, h.a > 0 ? h.a - 1 : 0);
+    const double yb1 = ipow(dy, h.b > 0 ? h.b - 1 : 0);
+    const double zc1 = ipow(dz, h.c > 0 ? h.c - 1 : 0);
+    const double xa = h.a > 0 ? xa1 * dx : 1.0;
+    const double yb = h.b > 0 ? yb1 * dy : 1.0;
+    const double zc = h.c > 0 ? zc1 * dz : 1.0;
+    S += h.coef * xa * yb * zc;
+    if (h.a > 0) Sx += h.coef * h.a * xa1 * yb * zc;
+    if (h.b > 0) Sy += h.coef * h.b * xa * yb1 * zc;
+    if (h.c > 0) Sz += h.coef * h.c * xa * yb * zc1;
+  }
+  const double common = ((double)f.k / r - f.zeta) * R / r;
+  v = (float)(R * S);
+  gx = (float)(R * Sx + S * (dx * common));
+  gy = (float)(R * Sy + S * (dy * common));
+  gz = (float)(R * Sz + S * (dz * common));
+}
+
 }  // namespace nkx

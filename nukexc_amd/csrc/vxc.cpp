@@ -313,21 +313,26 @@ void for_each_chunk(const Plan &pl, int64_t max_slots, F &&fn) {
 
 void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *E,
                double *N, double *V) {
-  NKX_REQUIRE(precision == NKX_PREC_FP64, NKX_ERR_UNSUPPORTED,
-              "this build of the library has the FP64 form only");
   ensure_plan(c);
   Plan &pl = c.plan;
   Workspace &ws = c.ws;
   hipStream_t s = c.stream;
   const int narr = gga ? 5 : 2;
   const int nsplit = choose_nsplit(pl);
-  int64_t max_slots = chunk_slot_budget(narr, sizeof(double));
+  const bool mixed = precision == NKX_PREC_MIXED;
+  const int nprod = c.opt.mixed_products;
+  int64_t max_slots = chunk_slot_budget(narr, mixed ? sizeof(float) : sizeof(double));
   // at least the largest single batch
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
-  ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
-  ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+  if (mixed) {
+    ws.tiles32.reserve((size_t)slots_alloc * narr * FB * PT);
+    ws.Ps.reserve((size_t)3 * pl.nbf_pad * pl.nbf_pad);
+  } else {
+    ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
+    ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+  }
   ws.coef.reserve((size_t)4 * pl.npts_pad);
   ws.blockEN.reserve((size_t)2 * pl.nb);
   const size_t vsz = (size_t)nsplit * std::max(pl.npairs, 1) * FB * FB;
@@ -337,20 +342,34 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   Timing &tm = c.timing;
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
-  vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
+  if (mixed)
+    vkm_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Ps.p);
+  else
+    vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
   NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, vsz * sizeof(double), s));
   NKX_HIP_CHECK(hipMemsetAsync(V, 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
   tm.end(t, s);
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
+    const int ig = gga ? 1 : 0;
     int tt = tm.begin(kStageColloc, s);
-    vk_colloc(s, s0, s1, pl, narr, gga ? 1 : 0, ws.tiles64.p);
+    if (mixed)
+      vkm_colloc(s, s0, s1, pl, narr, ig, ws.tiles32.p);
+    else
+      vk_colloc(s, s0, s1, pl, narr, ig, ws.tiles64.p);
     tm.end(tt, s);
     tt = tm.begin(kStageDensity, s);
-    vk_density(s, b0, b1, s0, pl, ws.Pd.p, narr, gga ? 1 : 0, ws.tiles64.p, xl, ws.coef.p,
-               ws.blockEN.p);
+    if (mixed)
+      vkm_density(s, nprod, b0, b1, s0, pl, ws.Ps.p, narr, ig, ws.tiles32.p, xl, ws.coef.p,
+                  ws.blockEN.p);
+    else
+      vk_density(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+                 ws.blockEN.p);
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
-    vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
+    if (mixed)
+      vkm_vgemm(s, nprod, b0, b1, s0, pl, nsplit, narr, ws.tiles32.p, ws.vpart.p);
+    else
+      vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
     tm.end(tt, s);
   });
   t = tm.begin(kStageFinal, s);

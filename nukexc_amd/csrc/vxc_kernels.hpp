@@ -19,4 +19,15 @@ void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
 void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V);
 void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out);
 
+// mixed form (kernels_vxc_mixed.hip); Ps holds three bf16 planes of P
+void vkm_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const double *P,
+                  uint16_t *Ps);
+void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
+                float *tiles);
+void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                 const uint16_t *Ps, int narr, int gga, float *tiles, const XcList &xl,
+                 double *coef, double *blockEN);
+void vkm_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+               int nsplit, int narr, const float *tiles, double *vpart);
+
 }  // namespace nkx

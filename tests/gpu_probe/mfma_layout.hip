@@ -53,3 +53,33 @@ extern "C" int mfma_probe_bf16(const unsigned short *a_h, const unsigned short *
   (void)hipFree(a); (void)hipFree(b); (void)hipFree(d);
   return rc;
 }
+
+// Accumulation rounding of v_mfma_f32_16x16x32_bf16: every lane's C is c_h[r];
+// A has a single nonzero bf16, A[0][0] = a, and B a single nonzero, B[0][0] =
+// b, so D[0][0] = C[0][0] + a * b and every other D element = C.
+__global__ void probe_round(const float *c, __bf16 a, __bf16 b, float *d) {
+  const int l = threadIdx.x;
+  v8bf av, bv;
+  for (int i = 0; i < 8; ++i) {
+    av[i] = (__bf16)0.0f;
+    bv[i] = (__bf16)0.0f;
+  }
+  if (l == 0) {  // A[0][0] = a (lane 0 holds row 0, k = 0..7), B[0][0] = b
+    av[0] = a;
+    bv[0] = b;
+  }
+  v4f acc = {c[l * 4 + 0], c[l * 4 + 1], c[l * 4 + 2], c[l * 4 + 3]};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+}
+
+extern "C" int mfma_probe_round(const float *c_h, float a_f, float b_f, float *d_h) {
+  float *c, *d;
+  if (hipMalloc(&c, 256 * 4) || hipMalloc(&d, 256 * 4)) return 1;
+  (void)hipMemcpy(c, c_h, 256 * 4, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(probe_round, dim3(1), dim3(64), 0, 0, c, (__bf16)a_f, (__bf16)b_f, d);
+  int rc = hipDeviceSynchronize() != hipSuccess;
+  (void)hipMemcpy(d_h, d, 256 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(c); (void)hipFree(d);
+  return rc;
+}

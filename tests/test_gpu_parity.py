@@ -145,9 +145,8 @@ def test_make_flat_grid_matches_oracle(ctx):
     assert np.all(gw[d > 1e-12] < noise)
     ok = d <= 1e-12
     assert np.array_equal(Ow[idx[ok]], go[ok])
-    rel = np.abs(gw[ok] - W[idx[ok]]) / np.maximum(W[idx[ok]], 1e-300)
-    big = W[idx[ok]] > 1e-12 * W.max()
-    assert rel[big].max() < 1e-11
+    # partition fractions agree to ~1e-16 absolute (see test_becke_matches_oracle)
+    assert np.all(np.abs(gw[ok] - W[idx[ok]]) <= 1e-11 * W[idx[ok]] + 1e-14 * W.max())
     d2, _ = cKDTree(gp).query(P)
     assert np.all(W[d2 > 1e-12] < noise)
     assert abs(gw.sum() - W.sum()) < 1e-12 * W.sum()
@@ -310,3 +309,68 @@ def test_overlap_matches_oracle(ctx):
     w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, "water", nrad=40, nang_order=17)
     Sg = ctx.overlap().cpu().numpy()
     assert np.abs(Sg - S).max() < 1e-12
+
+
+# ------------------------------------------------------ batched V_xc (mixed) ----
+@pytest.mark.parametrize("name,grid_kw", [("water", {}), ("benzene", dict(nrad=50, nang_order=29))])
+def test_vxc_mixed_matches_oracle(ctx, name, grid_kw):
+    """Mixed form (FP32 collocation, 3-term split-bf16 MFMA, 6 products) vs the
+    FP64 oracle: |dE_xc| <= 1e-6 Eh, max|dV_xc| <= 1e-7."""
+    w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, name, **grid_kw)
+    gga = any(f > 100 for f in w["funcs"])
+    phi, gx, gy, gz = oracle_on_grid(w["basis"], pts, grad=True)
+    E0, N0, V0 = O.vxc_from_collocation(phi, wts, w["funcs"], P=P,
+                                        gx=gx if gga else None, gy=gy if gga else None,
+                                        gz=gz if gga else None)
+    ctx.set_mixed_products(6)
+    r = ctx.vxc(dev(P), funcs=w["funcs"], precision="mixed")
+    V = r.potential.cpu().numpy()
+    dE, dV, dN = r.energy - E0, np.abs(V - V0).max(), r.n_el - N0
+    k = np.unravel_index(np.abs(V - V0).argmax(), V.shape)
+    b = w["basis"]
+    print(f"mixed {name}: dE={dE:.2e} max|dV|={dV:.2e} dN={dN:.2e} at V{k}={V0[k]:.4f} "
+          f"zeta=({b.zeta[k[0]]:.2f},{b.zeta[k[1]]:.2f}) l=({b.l[k[0]]},{b.l[k[1]]})")
+    assert abs(dE) <= 1e-6
+    assert dV <= 1e-7
+    assert abs(dN) <= 1e-6
+    assert np.array_equal(V, V.T)
+
+
+def test_vxc_mixed_three_products_reported(ctx):
+    """The 3-product variant runs and is less accurate than the 6-product one
+    (the error of each is reported, not gated)."""
+    w, pts, wts, S, C, P = _workload_on_gpu_grid(ctx, "water", nrad=40, nang_order=17)
+    ref = ctx.vxc(dev(P), funcs=w["funcs"], precision="fp64")
+    out = {}
+    for npr in (3, 6):
+        ctx.set_mixed_products(npr)
+        r = ctx.vxc(dev(P), funcs=w["funcs"], precision="mixed")
+        out[npr] = (abs(r.energy - ref.energy), (r.potential - ref.potential).abs().max().item())
+    ctx.set_mixed_products(6)
+    print(f"mixed products 3: dE={out[3][0]:.2e} dV={out[3][1]:.2e}; 6: dE={out[6][0]:.2e} "
+          f"dV={out[6][1]:.2e}")
+    assert out[6][1] <= out[3][1]
+
+
+def test_mfma_bf16_accumulate_rounding():
+    """Records how v_mfma_f32_16x16x32_bf16 rounds C + a*b into FP32 (the
+    mixed form bounds FP32 accumulation to 64-element blocks either way)."""
+    probe = ctypes.CDLL(os.path.join(ROOT, "tests", "gpu_probe", "libmfma_probe.so"))
+    probe.mfma_probe_round.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_float,
+                                       ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+    ulp = 2.0 ** -23
+    c = np.ones(256, np.float32)
+    res = {}
+    for frac in (0.25, 0.5, 0.75, -0.25, -0.75):
+        d = np.zeros(256, np.float32)
+        # a * b = frac ulp(1) exactly: a = frac * 2^-23 (a power of two times
+        # 0.25/0.5/0.75, exact in bf16), b = 1
+        assert probe.mfma_probe_round(c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                      frac * ulp, 1.0,
+                                      d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+        got = float(d[0 * 4 + 0])  # lane 0, register 0 = D[0][0]
+        res[frac] = (got - 1.0) / ulp
+        assert np.all(d[4:] == 1.0)  # untouched elements
+    print("MFMA bf16 accumulation, (result - 1)/ulp for a*b = frac ulp:", res)
+    # round to nearest even: 0.25 -> 0, 0.75 -> 1, -0.25 -> 0, -0.75 -> -0.5 (ulp below 1 is half)
+    assert res[0.25] in (0.0,) and res[0.75] in (1.0,)
