@@ -1,20 +1,32 @@
 // Batched V_xc build on MI355X (gfx950), mixed form.
 //
-// Collocation values are FP32 (exponential in FP32 on an FP64 range-reduced
-// argument, see sto_value_mixed in sto_eval.hpp).  Both contractions run
-// on v_mfma_f32_16x16x32_bf16 with every FP32 operand split into three bf16
+// Collocation values are FP32 numbers, each rounded once from an FP64
+// evaluation (sto_value_mixed in sto_eval.hpp).  Both contractions run on
+// v_mfma_f32_16x16x32_bf16 with every FP32 operand split into three bf16
 // terms, x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0
-// - x1); Henry, Tang & Heinecke, ARITH 2019), and the six products
-// a0 b0 + a0 b1 + a1 b0 + a0 b2 + a1 b1 + a2 b0 accumulated in FP32.  The
-// dropped terms are O(2^-24) relative.  FP64 is kept where the error budget
-// needs it: every FP32 MFMA accumulator covers at most one 64-function block
-// (density contraction) or one 32-point stage (V contraction) before it is
-// added into FP64 registers, so FP32 accumulation error does not grow with
-// the number of functions or points; the functional and the E_xc / N_el / V
-// reductions are FP64.
+// - x1), round to nearest even; exact for FP32 x; Henry, Tang & Heinecke,
+// ARITH 2019), and the six products a0 b0 + a0 b1 + a1 b0 + a0 b2 + a1 b1 +
+// a2 b0 accumulated in FP32; the dropped terms are O(2^-24) relative.  Every
+// FP32 accumulator covers at most one 64-function block (density) or one
+// 32-point MFMA step (V) before it is added into FP64 registers, so FP32
+// accumulation error does not grow with the system.  The functional and the
+// E_xc / N_el / V reductions are FP64.
 //
-// Same data layout and launch structure as the FP64 form (kernels_vxc.hip);
-// tiles are FP32.
+// Operands are stored pre-split in MFMA-fragment order, so every fragment is
+// one coalesced 16-byte load per lane and the main loops need no LDS staging
+// and no barriers.  For a 64 x 64 tile T[row][k] a "fragment" (rt, ks) holds
+// rows 16 rt .. 16 rt + 15 and k = 32 ks .. 32 ks + 31 as 64 lanes x 8 bf16:
+//   lane l, element e  <->  T[16 rt + l % 16][32 ks + 8 (l / 16) + e],
+// stored at ((rt * 2 + ks) * 64 + l) * 8 + e, per plane (plane-major).
+// This is the A-operand order of v_mfma_f32_16x16x32_bf16 (A[i][k]) and, read
+// as B[k][j] with j the row of T, its B-operand order.
+//
+// Per live slot the tile store holds (bytes per slot, GGA):
+//   F   Phi, rows = functions, k = points        3 planes   24 KB
+//   Pm  Phi, rows = points,    k = functions     3 planes   24 KB
+//   G   dPhi/dx, dPhi/dy, dPhi/dz  FP32 [f][p]              48 KB
+//   Z   Z,   rows = functions, k = points        3 planes   24 KB
+// (LDA: F, Pm, Z.)  P is gathered into fragment order per 64 x 64 block pair.
 #include "nkx_internal.hpp"
 #include "sto_eval.hpp"
 #include "vxc_kernels.hpp"
@@ -32,22 +44,29 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 namespace {
 constexpr int FB = 64;
 constexpr int PT = 64;
-constexpr int TILE = FB * PT;
-constexpr int LDR = 72;  // padded LDS row (bf16 elements)
+constexpr int TILE = FB * PT;          // elements per tile
+constexpr int PLANES = 3 * TILE;       // bf16 elements of a 3-plane tile
 
-// float -> bf16 with round-to-nearest-even, done on the bits so the rounding
-// mode is explicit (finite inputs).  Rounding to nearest makes each residual
-// of the split symmetric about zero; with truncation all three terms would
-// share the sign of x and the dropped products would bias every sum.
-__device__ __forceinline__ __bf16 bf16_rne(float x, float &back) {
+__device__ __forceinline__ int frag_off(int row, int k) {
+  // offset of T[row][k] inside one plane of a fragment-ordered 64 x 64 tile
+  const int rt = row >> 4, i = row & 15, ks = k >> 5, kq = (k >> 3) & 3, e = k & 7;
+  return (((rt * 2 + ks) * 64) + i + 16 * kq) * 8 + e;
+}
+
+// float -> bf16, round to nearest even on the bits (finite inputs).  Rounding
+// to nearest keeps each residual of the split symmetric about zero; with
+// truncation the three terms would share the sign of x and the dropped
+// products would bias every sum.
+__device__ __forceinline__ unsigned short bf16_rne(float x, float &back) {
   uint32_t u = __float_as_uint(x);
   u += 0x7FFFu + ((u >> 16) & 1u);
   u &= 0xFFFF0000u;
   back = __uint_as_float(u);
-  return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+  return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
+__device__ __forceinline__ void split3(float x, unsigned short &a, unsigned short &b,
+                                       unsigned short &c) {
   float xa, xb, xc;
   a = bf16_rne(x, xa);
   const float r1 = x - xa;  // exact
@@ -56,10 +75,21 @@ __device__ __forceinline__ void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c)
   c = bf16_rne(r2, xc);
 }
 
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+// value of T[row][k] from a 3-plane fragment-ordered tile
+__device__ __forceinline__ float tile_value(const unsigned short *t, int row, int k) {
+  const int o = frag_off(row, k);
+  return (bf2f(t[o + 2 * TILE]) + bf2f(t[o + TILE])) + bf2f(t[o]);
+}
+
+__device__ __forceinline__ v8bf ld_frag(const unsigned short *p) {
+  return *reinterpret_cast<const v8bf *>(p);
+}
+
 template <int NPROD>
 __device__ __forceinline__ v4f mfma_split(const v8bf *a, const v8bf *b, v4f acc) {
-  // small terms first
-  if (NPROD == 6) {
+  if (NPROD == 6) {  // small terms first
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
@@ -69,78 +99,121 @@ __device__ __forceinline__ v4f mfma_split(const v8bf *a, const v8bf *b, v4f acc)
   acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
   return acc;
 }
+
+struct MixedLayout {
+  int64_t slot_elems;  // bf16-sized elements per slot
+  int64_t off_F, off_P, off_G, off_Z;
+};
+
+__host__ __device__ inline MixedLayout mixed_layout(int gga) {
+  MixedLayout L;
+  L.off_F = 0;
+  L.off_P = PLANES;
+  L.off_G = 2 * PLANES;                          // FP32 data, 2 bf16 units each
+  L.off_Z = 2 * PLANES + (gga ? 3 * TILE * 2 : 0);
+  L.slot_elems = L.off_Z + PLANES;
+  return L;
+}
 }  // namespace
 
+int64_t mixed_slot_bytes(int gga) { return mixed_layout(gga).slot_elems * 2; }
+
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) colloc_tiles_f32(
+// Collocation: one workgroup per live slot.  Values are computed in FP64,
+// rounded to FP32, split, staged in LDS as [plane][f][p] and written out in
+// both fragment orders.
+__global__ void __launch_bounds__(256) colloc_tiles_mixed(
     int64_t s0, const int *__restrict__ slot_b, const int *__restrict__ b_fbl,
     const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
     const double *__restrict__ px, const double *__restrict__ py,
-    const double *__restrict__ pz, int narr, int gga, float *__restrict__ tiles) {
+    const double *__restrict__ pz, int gga, unsigned short *__restrict__ store) {
   const int64_t s = s0 + blockIdx.x;
   const int b = slot_b[s];
   const int fb = b_fbl[s];
+  const MixedLayout L = mixed_layout(gga);
   __shared__ FuncRec fr[FB];
+  __shared__ __attribute__((aligned(16))) unsigned short sp[3][FB][PT];
   if (threadIdx.x < FB) fr[threadIdx.x] = F[fb * FB + threadIdx.x];
   __syncthreads();
   const int g = threadIdx.x & (PT - 1);
   const int row = threadIdx.x / PT;
   const int64_t gp = (int64_t)b * PT + g;
   const double x = px[gp], y = py[gp], z = pz[gp];
-  float *base = tiles + (int64_t)blockIdx.x * narr * TILE;
+  unsigned short *base = store + (int64_t)blockIdx.x * L.slot_elems;
+  float *G = reinterpret_cast<float *>(base + L.off_G);
 #pragma unroll 1
   for (int f = row; f < FB; f += 256 / PT) {
     const FuncRec rec = fr[f];
-    const int e = f * PT + g;
+    float v;
     if (gga) {
-      float v, a, bb, c;
+      float a, bb, c;
       sto_value_grad_mixed(rec, T, x, y, z, v, a, bb, c);
-      base[e] = v;
-      base[TILE + e] = a;
-      base[2 * TILE + e] = bb;
-      base[3 * TILE + e] = c;
+      G[f * PT + g] = a;
+      G[TILE + f * PT + g] = bb;
+      G[2 * TILE + f * PT + g] = c;
     } else {
-      base[e] = sto_value_mixed(rec, T, x, y, z);
+      v = sto_value_mixed(rec, T, x, y, z);
     }
+    split3(v, sp[0][f][g], sp[1][f][g], sp[2][f][g]);
+  }
+  __syncthreads();
+  // 8 fragments x 64 lanes per plane; one 16-byte chunk per (plane, frag, lane)
+  for (int c = threadIdx.x; c < 3 * 8 * 64; c += 256) {
+    const int pl = c / 512, fr_ = (c / 64) % 8, l = c % 64;
+    const int rt = fr_ >> 1, ks = fr_ & 1;
+    const int rowF = 16 * rt + (l & 15), k0 = 32 * ks + 8 * (l >> 4);
+    // F order: rows = functions, k = points (contiguous in sp)
+    *reinterpret_cast<v8bf *>(base + L.off_F + pl * TILE + c % 512 * 8) =
+        *reinterpret_cast<const v8bf *>(&sp[pl][rowF][k0]);
+    // Pm order: rows = points, k = functions (gather down a column of sp)
+    __attribute__((aligned(16))) unsigned short t[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) t[e] = sp[pl][k0 + e][rowF];
+    *reinterpret_cast<v8bf *>(base + L.off_P + pl * TILE + c % 512 * 8) =
+        *reinterpret_cast<const v8bf *>(t);
   }
 }
 
-// P (FP64, original order) -> three bf16 planes in the permuted, padded order.
-__global__ void __launch_bounds__(256) gather_p_split(int nbf, int nbf_pad,
+// P (FP64, original order) -> three bf16 planes, permuted and padded, in
+// fragment order per 64 x 64 block pair: [plane][fm][fk][fragment tile].
+__global__ void __launch_bounds__(256) gather_p_split(int nbf, int nfb,
                                                       const int *__restrict__ fperm,
                                                       const double *__restrict__ P,
-                                                      __bf16 *__restrict__ Ps) {
+                                                      unsigned short *__restrict__ Ps) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbf_pad = nfb * FB;
   const int64_t n = (int64_t)nbf_pad * nbf_pad;
   if (e >= n) return;
   const int i = (int)(e / nbf_pad), j = (int)(e % nbf_pad);
   const double v = (i < nbf && j < nbf) ? P[(int64_t)fperm[i] * nbf + fperm[j]] : 0.0;
-  __bf16 a, b, c;
+  unsigned short a, b, c;
   split3((float)v, a, b, c);
-  Ps[e] = a;
-  Ps[n + e] = b;
-  Ps[2 * n + e] = c;
+  const int fm = i / FB, fk = j / FB;
+  const int64_t o = ((int64_t)fm * nfb + fk) * TILE + frag_off(i % FB, j % FB);
+  Ps[o] = a;
+  Ps[n + o] = b;
+  Ps[2 * n + o] = c;
 }
 
 // ---------------------------------------------------------------------------
+// Density contraction, functional and Z.  One workgroup per batch; wave w owns
+// rows 16 w .. 16 w + 15 of the current M-block and all 64 points.
 template <int NPROD>
 __global__ void __launch_bounds__(256) density_fused_mixed(
-    int nbf_pad, const __bf16 *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
-    float *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
+    int nfb, const unsigned short *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int gga,
+    unsigned short *__restrict__ store, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN) {
   const int b = (int)(b0 + blockIdx.x);
   const int lo = b_off[b], hi = b_off[b + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
-  const int64_t plane = (int64_t)nbf_pad * nbf_pad;
+  const MixedLayout L = mixed_layout(gga);
+  const int64_t plane = (int64_t)nfb * nfb * TILE;
 
-  __shared__ __attribute__((aligned(16))) __bf16 Pl[3][FB][LDR];  // [plane][mu][nu]
-  __shared__ __attribute__((aligned(16))) __bf16 Fl[3][PT][LDR];  // [plane][point][nu]
   __shared__ double red[4][4][PT];
   __shared__ double colsum[4][PT];
   __shared__ double cf[4][PT];
-
   for (int e = threadIdx.x; e < 4 * PT; e += 256) (&colsum[0][0])[e] = 0.0;
   __syncthreads();
   const int nq = gga ? 4 : 1;
@@ -148,76 +221,59 @@ __global__ void __launch_bounds__(256) density_fused_mixed(
   for (int mi = lo; mi < hi; ++mi) {
     const int fm = b_fbl[mi];
     double acc64[4][4];
+#pragma unroll
     for (int j = 0; j < 4; ++j)
+#pragma unroll
       for (int r = 0; r < 4; ++r) acc64[j][r] = 0.0;
     for (int ki = lo; ki < hi; ++ki) {
       const int fk = b_fbl[ki];
+      const unsigned short *pa = Ps + ((int64_t)fm * nfb + fk) * TILE;
+      const unsigned short *pb = store + (ki - s0) * L.slot_elems + L.off_P;
       v4f acc[4];
+#pragma unroll
       for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-      // P planes: rows mu of block fm, 64 contiguous nu of block fk (16 B chunks)
-      for (int c = threadIdx.x; c < 3 * FB * 8; c += 256) {
-        const int pl = c / (FB * 8), rem = c % (FB * 8);
-        const int r = rem >> 3, k8 = (rem & 7) * 8;
-        *reinterpret_cast<v8bf *>(&Pl[pl][r][k8]) = *reinterpret_cast<const v8bf *>(
-            Ps + pl * plane + (int64_t)(fm * FB + r) * nbf_pad + fk * FB + k8);
-      }
-      // Phi tile of block fk, transposed to [point][nu] and split
-      {
-        const float *tsrc = tiles + (ki - s0) * narr * TILE;
-        const int p = threadIdx.x & 63, f0 = (threadIdx.x >> 6) * 16;
-        __attribute__((aligned(16))) __bf16 h[3][16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) split3(tsrc[(f0 + u) * PT + p], h[0][u], h[1][u], h[2][u]);
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl) {
-          *reinterpret_cast<v8bf *>(&Fl[pl][p][f0]) = *reinterpret_cast<v8bf *>(&h[pl][0]);
-          *reinterpret_cast<v8bf *>(&Fl[pl][p][f0 + 8]) = *reinterpret_cast<v8bf *>(&h[pl][8]);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < FB; ks += 32) {
+      for (int ks = 0; ks < 2; ++ks) {
         v8bf a[3];
 #pragma unroll
         for (int pl = 0; pl < 3; ++pl)
-          a[pl] = *reinterpret_cast<const v8bf *>(&Pl[pl][16 * wv + l16][ks + 8 * q]);
+          a[pl] = ld_frag(pa + pl * plane + ((wv * 2 + ks) * 64 + lane) * 8);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           v8bf bv[3];
 #pragma unroll
           for (int pl = 0; pl < 3; ++pl)
-            bv[pl] = *reinterpret_cast<const v8bf *>(&Fl[pl][16 * j + l16][ks + 8 * q]);
+            bv[pl] = ld_frag(pb + pl * TILE + ((j * 2 + ks) * 64 + lane) * 8);
           acc[j] = mfma_split<NPROD>(a, bv, acc[j]);
         }
       }
-      // FP32 accumulation spans one 64-function block only
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc64[j][r] += (double)acc[j][r];
-      __syncthreads();
     }
     // Epilogue in FP64: column partial sums of Phi X (and grad Phi X).
-    const float *tb = tiles + (mi - s0) * narr * TILE;
+    const unsigned short *tb = store + (mi - s0) * L.slot_elems;
+    const float *G = reinterpret_cast<const float *>(tb + L.off_G);
     double part[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int g = 16 * j + l16;
-      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      double sm[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mu = 16 * wv + 4 * q + r;
         const double x = acc64[j][r];
-        s[0] += (double)tb[mu * PT + g] * x;
+        sm[0] += (double)tile_value(tb + L.off_F, mu, g) * x;
         if (gga) {
-          s[1] += (double)tb[TILE + mu * PT + g] * x;
-          s[2] += (double)tb[2 * TILE + mu * PT + g] * x;
-          s[3] += (double)tb[3 * TILE + mu * PT + g] * x;
+          sm[1] += (double)G[mu * PT + g] * x;
+          sm[2] += (double)G[TILE + mu * PT + g] * x;
+          sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
         }
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        double v = s[c];
+        double v = sm[c];
         v += __shfl_xor(v, 16);
         v += __shfl_xor(v, 32);
         part[j][c] = v;
@@ -266,90 +322,85 @@ __global__ void __launch_bounds__(256) density_fused_mixed(
   }
   __syncthreads();
 
+  // Z = a Phi + b . grad Phi, rounded to FP32, split, fragment order (F).
   for (int mi = lo; mi < hi; ++mi) {
-    float *tb = tiles + (mi - s0) * narr * TILE;
-    float *zt = tb + (narr - 1) * TILE;
-    for (int e = threadIdx.x; e < TILE; e += 256) {
-      const int g = e % PT;
-      double z = cf[0][g] * (double)tb[e];
-      if (gga)
-        z += cf[1][g] * (double)tb[TILE + e] + cf[2][g] * (double)tb[2 * TILE + e] +
-             cf[3][g] * (double)tb[3 * TILE + e];
-      zt[e] = (float)z;
+    unsigned short *tb = store + (mi - s0) * L.slot_elems;
+    const float *G = reinterpret_cast<const float *>(tb + L.off_G);
+    for (int c = threadIdx.x; c < 8 * 64; c += 256) {
+      const int fr_ = c / 64, l = c % 64;
+      const int rt = fr_ >> 1, ks = fr_ & 1;
+      const int f = 16 * rt + (l & 15), k0 = 32 * ks + 8 * (l >> 4);
+      __attribute__((aligned(16))) unsigned short t0[8], t1[8], t2[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int g = k0 + e;
+        double zz = cf[0][g] * (double)tile_value(tb + L.off_F, f, g);
+        if (gga)
+          zz += cf[1][g] * (double)G[f * PT + g] + cf[2][g] * (double)G[TILE + f * PT + g] +
+                cf[3][g] * (double)G[2 * TILE + f * PT + g];
+        split3((float)zz, t0[e], t1[e], t2[e]);
+      }
+      unsigned short *zt = tb + L.off_Z + c * 8;
+      *reinterpret_cast<v8bf *>(zt) = *reinterpret_cast<const v8bf *>(t0);
+      *reinterpret_cast<v8bf *>(zt + TILE) = *reinterpret_cast<const v8bf *>(t1);
+      *reinterpret_cast<v8bf *>(zt + 2 * TILE) = *reinterpret_cast<const v8bf *>(t2);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// acc(I, J) += Phi_I Z_J^T + Z_I Phi_J^T; FP32 within a batch, FP64 across.
-constexpr int HPM = 32;        // points per LDS stage (one MFMA k-step)
-constexpr int LDM = HPM + 8;   // padded row (bf16)
-
+// acc(I, J) += Phi_I Z_J^T + Z_I Phi_J^T over batches where I and J are live.
+// Wave w owns rows 16 w .. 16 w + 15 of block I and all 64 columns of block J.
 template <int NPROD>
 __global__ void __launch_bounds__(256) vgemm_mixed(
     int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
     const int *__restrict__ tile_slot, int64_t nb, int64_t b0, int64_t b1, int64_t s0,
-    int narr, const float *__restrict__ tiles, double *__restrict__ vpart) {
+    int gga, const unsigned short *__restrict__ store, double *__restrict__ vpart) {
   const int p = blockIdx.x % npairs;
   const int split = blockIdx.x / npairs;
   const int I = pair_I[p], J = pair_J[p];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
-  // [operand][plane][function][point]: operands Phi_I, Z_I, Phi_J, Z_J
-  __shared__ __attribute__((aligned(16))) __bf16 S[4][3][FB][LDM];
+  const MixedLayout L = mixed_layout(gga);
   double acc64[4][4];
+#pragma unroll
   for (int j = 0; j < 4; ++j)
+#pragma unroll
     for (int r = 0; r < 4; ++r) acc64[j][r] = 0.0;
   const int *tsI = tile_slot + (int64_t)I * nb;
   const int *tsJ = tile_slot + (int64_t)J * nb;
   for (int64_t b = b0 + split; b < b1; b += nsplit) {
     const int sI = tsI[b], sJ = tsJ[b];
     if (sI < 0 || sJ < 0) continue;
-    const float *src[4] = {tiles + (sI - s0) * narr * TILE,
-                           tiles + (sI - s0) * narr * TILE + (narr - 1) * TILE,
-                           tiles + (sJ - s0) * narr * TILE,
-                           tiles + (sJ - s0) * narr * TILE + (narr - 1) * TILE};
-    for (int h = 0; h < PT; h += HPM) {
-      v4f acc[4];
-      for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-      // stage 4 operands x (64 functions x 32 points), split into 3 planes
+    const unsigned short *tI = store + (sI - s0) * L.slot_elems;
+    const unsigned short *tJ = store + (sJ - s0) * L.slot_elems;
 #pragma unroll
-      for (int op = 0; op < 4; ++op) {
-        const int r = threadIdx.x >> 2, c8 = (threadIdx.x & 3) * 8;
-        const float4 u0 = *reinterpret_cast<const float4 *>(src[op] + r * PT + h + c8);
-        const float4 u1 = *reinterpret_cast<const float4 *>(src[op] + r * PT + h + c8 + 4);
-        const float vals[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-        __attribute__((aligned(16))) __bf16 t[3][8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) split3(vals[u], t[0][u], t[1][u], t[2][u]);
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl)
-          *reinterpret_cast<v8bf *>(&S[op][pl][r][c8]) = *reinterpret_cast<v8bf *>(&t[pl][0]);
-      }
-      __syncthreads();
+    for (int ks = 0; ks < 2; ++ks) {
       v8bf aphi[3], az[3];
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) {
-        aphi[pl] = *reinterpret_cast<const v8bf *>(&S[0][pl][16 * wv + l16][8 * q]);
-        az[pl] = *reinterpret_cast<const v8bf *>(&S[1][pl][16 * wv + l16][8 * q]);
+        const int o = pl * TILE + ((wv * 2 + ks) * 64 + lane) * 8;
+        aphi[pl] = ld_frag(tI + L.off_F + o);
+        az[pl] = ld_frag(tI + L.off_Z + o);
       }
+      v4f acc[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
         v8bf bphi[3], bz[3];
 #pragma unroll
         for (int pl = 0; pl < 3; ++pl) {
-          bphi[pl] = *reinterpret_cast<const v8bf *>(&S[2][pl][16 * j + l16][8 * q]);
-          bz[pl] = *reinterpret_cast<const v8bf *>(&S[3][pl][16 * j + l16][8 * q]);
+          const int o = pl * TILE + ((j * 2 + ks) * 64 + lane) * 8;
+          bphi[pl] = ld_frag(tJ + L.off_F + o);
+          bz[pl] = ld_frag(tJ + L.off_Z + o);
         }
         acc[j] = mfma_split<NPROD>(aphi, bz, acc[j]);
         acc[j] = mfma_split<NPROD>(az, bphi, acc[j]);
       }
-      // FP32 accumulation spans one 32-point stage only
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc64[j][r] += (double)acc[j][r];
-      __syncthreads();
     }
   }
   double *out = vpart + ((int64_t)split * npairs + p) * (FB * FB);
@@ -363,50 +414,49 @@ __global__ void __launch_bounds__(256) vgemm_mixed(
 }
 
 // ---------------------------------------------------------------------------
-void vkm_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const double *P,
+void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
                   uint16_t *Ps) {
-  const int64_t n = (int64_t)nbf_pad * nbf_pad;
+  const int64_t n = (int64_t)nfb * FB * nfb * FB;
   hipLaunchKernelGGL(gather_p_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nbf,
-                     nbf_pad, fperm, P, reinterpret_cast<__bf16 *>(Ps));
+                     nfb, fperm, P, Ps);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
-                float *tiles) {
+void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
+                uint16_t *store) {
   if (s1 <= s0) return;
-  hipLaunchKernelGGL(colloc_tiles_f32, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+  hipLaunchKernelGGL(colloc_tiles_mixed, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
                      pl.slot_b.p, pl.b_fbl.p, pl.funcs.p, pl.terms.p, pl.px.p, pl.py.p, pl.pz.p,
-                     narr, gga, tiles);
+                     gga, store);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
 void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
-                 const uint16_t *Ps16, int narr, int gga, float *tiles, const XcList &xl,
-                 double *coef, double *blockEN) {
+                 const uint16_t *Ps, int gga, uint16_t *store, const XcList &xl, double *coef,
+                 double *blockEN) {
   if (b1 <= b0) return;
-  const __bf16 *Ps = reinterpret_cast<const __bf16 *>(Ps16);
   if (nprod == 3)
     hipLaunchKernelGGL(density_fused_mixed<3>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
-                       pl.nbf_pad, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN);
+                       pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, gga, store, pl.pw.p, xl, coef,
+                       blockEN);
   else
     hipLaunchKernelGGL(density_fused_mixed<6>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
-                       pl.nbf_pad, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN);
+                       pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, gga, store, pl.pw.p, xl, coef,
+                       blockEN);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
 void vkm_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
-               int nsplit, int narr, const float *tiles, double *vpart) {
+               int nsplit, int gga, const uint16_t *store, double *vpart) {
   if (b1 <= b0 || pl.npairs == 0) return;
   if (nprod == 3)
     hipLaunchKernelGGL(vgemm_mixed<3>, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
                        pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0,
-                       b1, s0, narr, tiles, vpart);
+                       b1, s0, gga, store, vpart);
   else
     hipLaunchKernelGGL(vgemm_mixed<6>, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
                        pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0,
-                       b1, s0, narr, tiles, vpart);
+                       b1, s0, gga, store, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -129,9 +129,9 @@ struct Options {
 
 struct Workspace {
   DevBuf<double> Pd;               // permuted padded density matrix (FP64)
-  DevBuf<uint16_t> Ps;             // three bf16 planes of the permuted P (mixed)
+  DevBuf<uint16_t> Ps;             // three bf16 planes of the permuted P, fragment order
   DevBuf<double> tiles64;          // Phi, gx, gy, gz, Z tiles (FP64)
-  DevBuf<float> tiles32;           // same (mixed)
+  DevBuf<uint16_t> mstore;         // mixed-form tile store (bf16 planes + FP32 gradients)
   DevBuf<double> coef;             // per padded point: a, bx, by, bz
   DevBuf<double> blockEN;          // per batch: E, N partial sums
   DevBuf<double> vpart;            // split-K partial V tiles

@@ -275,13 +275,13 @@ void ensure_plan(Ctx &c) {
 
 namespace {
 
-// Number of batches per chunk so that the chunk's tiles fit the budget.
-int64_t chunk_slot_budget(int narr, size_t elem) {
+// Number of slots per chunk so that the chunk's tiles fit the budget.
+int64_t chunk_slot_budget(size_t slot_bytes) {
   size_t freeb = 0, total = 0;
   NKX_HIP_CHECK(hipMemGetInfo(&freeb, &total));
   size_t budget = std::min<size_t>((size_t)48 << 30, freeb / 5 * 2);
   budget = std::max<size_t>(budget, (size_t)64 << 20);
-  return (int64_t)(budget / ((size_t)narr * FB * PT * elem));
+  return (int64_t)(budget / slot_bytes);
 }
 
 int choose_nsplit(const Plan &pl) {
@@ -321,13 +321,15 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   const int nsplit = choose_nsplit(pl);
   const bool mixed = precision == NKX_PREC_MIXED;
   const int nprod = c.opt.mixed_products;
-  int64_t max_slots = chunk_slot_budget(narr, mixed ? sizeof(float) : sizeof(double));
+  const size_t slot_bytes = mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
+                                  : (size_t)narr * FB * PT * sizeof(double);
+  int64_t max_slots = chunk_slot_budget(slot_bytes);
   // at least the largest single batch
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   if (mixed) {
-    ws.tiles32.reserve((size_t)slots_alloc * narr * FB * PT);
+    ws.mstore.reserve((size_t)slots_alloc * slot_bytes / sizeof(uint16_t));
     ws.Ps.reserve((size_t)3 * pl.nbf_pad * pl.nbf_pad);
   } else {
     ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
@@ -343,7 +345,7 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
   if (mixed)
-    vkm_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Ps.p);
+    vkm_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P, ws.Ps.p);
   else
     vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
   NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, vsz * sizeof(double), s));
@@ -353,13 +355,13 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
     const int ig = gga ? 1 : 0;
     int tt = tm.begin(kStageColloc, s);
     if (mixed)
-      vkm_colloc(s, s0, s1, pl, narr, ig, ws.tiles32.p);
+      vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
     else
       vk_colloc(s, s0, s1, pl, narr, ig, ws.tiles64.p);
     tm.end(tt, s);
     tt = tm.begin(kStageDensity, s);
     if (mixed)
-      vkm_density(s, nprod, b0, b1, s0, pl, ws.Ps.p, narr, ig, ws.tiles32.p, xl, ws.coef.p,
+      vkm_density(s, nprod, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
                   ws.blockEN.p);
     else
       vk_density(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
@@ -367,7 +369,7 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
     if (mixed)
-      vkm_vgemm(s, nprod, b0, b1, s0, pl, nsplit, narr, ws.tiles32.p, ws.vpart.p);
+      vkm_vgemm(s, nprod, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p);
     else
       vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
     tm.end(tt, s);
@@ -433,7 +435,7 @@ void overlap_build(Ctx &c, double *S) {
   hipStream_t s = c.stream;
   const int narr = 2;
   const int nsplit = choose_nsplit(pl);
-  int64_t max_slots = chunk_slot_budget(narr, sizeof(double));
+  int64_t max_slots = chunk_slot_budget((size_t)narr * FB * PT * sizeof(double));
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
