@@ -1,5 +1,5 @@
-// nukexc/xc_integrals.hpp -- E_xc and V_xc (xc_integrals.hpp:62-166, :424-541
-// of the reference).  The reference passes a libxc xc_func_type and evaluates
+// nukexc/xc_integrals.hpp -- E_xc and V_xc (xc_integrals.hpp:62-262, :424-718
+// of the reference: compute_lda, compute_lsda, compute_gga, compute_gga_lsda).  The reference passes a libxc xc_func_type and evaluates
 // the functional on the host; here a small Functional handle carries libxc ids
 // and the functional runs on the GPU.
 #pragma once
@@ -14,6 +14,13 @@ struct XC_result {
   double energy = 0.0;
   DeviceView2DLeft potential;  // (N_bf, N_bf); symmetric
   double n_el = 0.0;           // sum_g w rho (batched build only)
+};
+
+struct XC_result_polarized {
+  double energy = 0.0;
+  DeviceView2DLeft potential_alpha;  // (N_bf, N_bf); symmetric
+  DeviceView2DLeft potential_beta;
+  double n_el[2] = {0.0, 0.0};       // per spin (batched build only)
 };
 
 // libxc-numbered functional list (LDA_X = 1, GGA_X_PBE = 101, GGA_C_PBE = 130,
@@ -64,16 +71,53 @@ inline XC_result compute_gga(const DeviceView2DLeft &collocation_values,
   return r;
 }
 
+inline XC_result_polarized compute_lsda(const DeviceView2DLeft &collocation_values,
+                                        const DeviceView1D &weights,
+                                        const DeviceView2DLeft &mo_alpha,
+                                        const DeviceView1D &occ_alpha,
+                                        const DeviceView2DLeft &mo_beta,
+                                        const DeviceView1D &occ_beta, const Functional &func) {
+  const size_t nbf = collocation_values.extent(0), npts = collocation_values.extent(1);
+  XC_result_polarized r;
+  r.potential_alpha.allocate(nbf, nbf);
+  r.potential_beta.allocate(nbf, nbf);
+  check(nkx_xc_from_collocation_polarized(
+            context().get(), (int)nbf, (int64_t)npts, collocation_values.data(), nullptr, nullptr,
+            nullptr, weights.data(), (int)mo_alpha.extent(1), mo_alpha.data(), occ_alpha.data(),
+            (int)mo_beta.extent(1), mo_beta.data(), occ_beta.data(), (int)func.ids.size(),
+            func.ids.data(), NKX_FAMILY_LDA, &r.energy, r.potential_alpha.data(),
+            r.potential_beta.data()),
+        context().get());
+  return r;
+}
+
+inline XC_result_polarized compute_gga_lsda(
+    const DeviceView2DLeft &collocation_values, const DeviceView2DLeft &collocation_gx,
+    const DeviceView2DLeft &collocation_gy, const DeviceView2DLeft &collocation_gz,
+    const DeviceView1D &weights, const DeviceView2DLeft &mo_alpha, const DeviceView1D &occ_alpha,
+    const DeviceView2DLeft &mo_beta, const DeviceView1D &occ_beta, const Functional &func) {
+  const size_t nbf = collocation_values.extent(0), npts = collocation_values.extent(1);
+  XC_result_polarized r;
+  r.potential_alpha.allocate(nbf, nbf);
+  r.potential_beta.allocate(nbf, nbf);
+  check(nkx_xc_from_collocation_polarized(
+            context().get(), (int)nbf, (int64_t)npts, collocation_values.data(),
+            collocation_gx.data(), collocation_gy.data(), collocation_gz.data(), weights.data(),
+            (int)mo_alpha.extent(1), mo_alpha.data(), occ_alpha.data(), (int)mo_beta.extent(1),
+            mo_beta.data(), occ_beta.data(), (int)func.ids.size(), func.ids.data(),
+            NKX_FAMILY_GGA, &r.energy, r.potential_alpha.data(), r.potential_beta.data()),
+        context().get());
+  return r;
+}
+
 // Batched build from a density matrix (row-major = LayoutLeft for symmetric
 // P): the per-SCF-iteration XC call, without materialising N_bf x N_pts
 // collocation matrices.
 enum class Precision { FP64 = NKX_PREC_FP64, Mixed = NKX_PREC_MIXED };
 
-inline XC_result compute_vxc(const FlatGrid &grid, const STOBasisSet &basis,
-                             const DeviceView2DLeft &P, const Functional &func,
-                             Precision prec = Precision::FP64) {
-  nkx_ctx *ctx = context().get();
-  // the grid is handed over once; repeated builds on it reuse the launch plan
+namespace detail {
+// The grid is handed over once; repeated builds on it reuse the launch plan.
+inline void use_grid(nkx_ctx *ctx, const FlatGrid &grid) {
   thread_local const void *last_grid = nullptr;
   thread_local size_t last_npts = 0;
   if (last_grid != grid.weights.data() || last_npts != grid.weights.size()) {
@@ -86,11 +130,37 @@ inline XC_result compute_vxc(const FlatGrid &grid, const STOBasisSet &basis,
     last_grid = grid.weights.data();
     last_npts = grid.weights.size();
   }
+}
+}  // namespace detail
+
+inline XC_result compute_vxc(const FlatGrid &grid, const STOBasisSet &basis,
+                             const DeviceView2DLeft &P, const Functional &func,
+                             Precision prec = Precision::FP64) {
+  nkx_ctx *ctx = context().get();
+  detail::use_grid(ctx, grid);
   set_basis(basis);
   XC_result r;
   r.potential.allocate(basis.nbf(), basis.nbf());
   check(nkx_vxc(ctx, P.data(), (int)func.ids.size(), func.ids.data(), (int)prec, &r.energy,
                 &r.n_el, r.potential.data()),
+        ctx);
+  return r;
+}
+
+// Open-shell batched build from the spin density matrices (FP64 form).
+inline XC_result_polarized compute_vxc_polarized(const FlatGrid &grid, const STOBasisSet &basis,
+                                                 const DeviceView2DLeft &P_alpha,
+                                                 const DeviceView2DLeft &P_beta,
+                                                 const Functional &func) {
+  nkx_ctx *ctx = context().get();
+  detail::use_grid(ctx, grid);
+  set_basis(basis);
+  XC_result_polarized r;
+  r.potential_alpha.allocate(basis.nbf(), basis.nbf());
+  r.potential_beta.allocate(basis.nbf(), basis.nbf());
+  check(nkx_vxc_polarized(ctx, P_alpha.data(), P_beta.data(), (int)func.ids.size(),
+                          func.ids.data(), NKX_PREC_FP64, &r.energy, r.n_el,
+                          r.potential_alpha.data(), r.potential_beta.data()),
         ctx);
   return r;
 }

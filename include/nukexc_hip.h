@@ -166,6 +166,16 @@ int nkx_functional_eval(nkx_ctx *ctx, int nfunc, const int *func_ids, int64_t np
                         const double *rho_dev, const double *sigma_dev,
                         double *exc_dev, double *vrho_dev, double *vsigma_dev);
 
+/* Spin-polarised evaluation (libxc XC_POLARIZED, the host calls at
+   xc_integrals.hpp:203 and :617, done on the GPU).  Interleaved layouts:
+   rho[2g+s], sigma[3g+k] (k: aa, ab, bb), vrho[2g+s], vsigma[3g+k]; exc per
+   particle.  Inputs are clamped as libxc does (rho_s >= thr,
+   sigma_ss >= thr^(8/3), |sigma_ab| <= (sigma_aa + sigma_bb)/2). */
+int nkx_functional_eval_polarized(nkx_ctx *ctx, int nfunc, const int *func_ids,
+                                  int64_t npts, const double *rho_dev,
+                                  const double *sigma_dev, double *exc_dev,
+                                  double *vrho_dev, double *vsigma_dev);
+
 /* ----------------------------------------------------- reference-shaped XC */
 /* compute_lda (gx == NULL, xc_integrals.hpp:73-166) and compute_gga
    (:424-541): E_xc and V_xc (row-major N_bf x N_bf, device) from caller
@@ -178,6 +188,18 @@ int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *p
                             const int *func_ids, int family_check, double *E_xc,
                             double *V_dev);
 
+/* compute_lsda (gx == NULL, xc_integrals.hpp:169-262) and compute_gga_lsda
+   (:543-718): E_xc and the two spin potentials V_alpha, V_beta from caller
+   collocation arrays, weights and per-spin orbitals / occupations. */
+int nkx_xc_from_collocation_polarized(nkx_ctx *ctx, int nbf, int64_t npts,
+                                      const double *phi_dev, const double *gx_dev,
+                                      const double *gy_dev, const double *gz_dev,
+                                      const double *w_dev, int nocc_a, const double *Ca_dev,
+                                      const double *fa_dev, int nocc_b, const double *Cb_dev,
+                                      const double *fb_dev, int nfunc, const int *func_ids,
+                                      int family_check, double *E_xc, double *Va_dev,
+                                      double *Vb_dev);
+
 /* ---------------------------------------------------------------- hot path */
 /* One V_xc build on the ctx grid and basis (the SCF-iteration XC call,
    tests/standalone.cxx:724-741 -> compute_gga): density from the density
@@ -188,6 +210,14 @@ int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *p
    function lists; precision selects the FP64 or the mixed form. */
 int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
             int precision, double *E_xc, double *N_el, double *V_dev);
+/* Spin-polarised build (the open-shell SCF iteration, tests/standalone.cxx:
+   321-360 -> standalone_helpers.hpp:229-256): spin density matrices Pa, Pb,
+   E_xc = sum_g w (rho_a + rho_b) eps, N_el[0..1] = per-spin electron counts
+   (may be NULL), V_alpha and V_beta (row-major N_bf x N_bf, device).
+   FP64 form only: precision must be NKX_PREC_FP64. */
+int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, int nfunc,
+                      const int *func_ids, int precision, double *E_xc, double *N_el,
+                      double *Va_dev, double *Vb_dev);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);

@@ -6,8 +6,10 @@ and a mirror of the reference's nukexc/ interface (see api.py).
 """
 from ._capi import LIB_PATH, NukexcError, lib  # noqa: F401
 from .api import (PBE, TA_M3, TA_M4, Context, FlatGrid, STOBasisSet, XC_result,  # noqa: F401
-                  compute_density, compute_density_and_sigma, compute_gga, compute_lda,
-                  fill_collocation, fill_grad_collocation, functional_eval, functional_info,
+                  XC_result_polarized, compute_density, compute_density_and_sigma, compute_gga,
+                  compute_gga_lsda, compute_lda, compute_lsda, fill_collocation,
+                  fill_grad_collocation, functional_eval, functional_eval_polarized,
+                  functional_info,
                   lebedev_rule, make_manual_basis, partition_becke, partition_becke_team,
                   radial_rule, radial_scale, solid_harmonic, sto_norm_cutoff)
 

@@ -63,6 +63,11 @@ SIGNATURES = {
     "nkx_xc_from_collocation": [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp,
                                 _c_int, _pi, _c_int, _pd, _vp],
     "nkx_vxc": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp],
+    "nkx_functional_eval_polarized": [_vp, _c_int, _pi, _c_i64, _vp, _vp, _vp, _vp, _vp],
+    "nkx_xc_from_collocation_polarized": [_vp, _c_int, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                          _vp, _vp, _c_int, _vp, _vp, _c_int, _pi, _c_int, _pd,
+                                          _vp, _vp],
+    "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_set_screening": [_vp, _c_double, _c_int],
     "nkx_set_mixed_products": [_vp, _c_int],
     "nkx_plan_stats": [_vp, _pi64, _pi64, _pi64, _pi64, _pi64],

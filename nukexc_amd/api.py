@@ -82,6 +82,15 @@ class XC_result:
     n_el: float = float("nan")
 
 
+@dataclass
+class XC_result_polarized:
+    """xc_integrals.hpp:67-71; n_el holds (N_alpha, N_beta) from the batched build."""
+    energy: float
+    potential_alpha: "torch.Tensor"  # (N_bf, N_bf)
+    potential_beta: "torch.Tensor"
+    n_el: tuple = (float("nan"), float("nan"))
+
+
 def sto_norm_cutoff(n, zeta, tol=1e-10):
     a, b = ctypes.c_double(), ctypes.c_double()
     check(lib().nkx_sto_norm_cutoff(int(n), float(zeta), float(tol), ctypes.byref(a),
@@ -260,6 +269,22 @@ class Context:
                                 _ptr(V)))
         return XC_result(E.value, V, N.value)
 
+    def vxc_polarized(self, Pa, Pb, funcs=PBE, precision="fp64"):
+        """Spin-polarised build from the spin density matrices (FP64 form).
+        Returns XC_result_polarized(energy, V_alpha, V_beta, (N_alpha, N_beta))."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        Pa = _dev_f64(Pa, self.device).contiguous()
+        Pb = _dev_f64(Pb, self.device).contiguous()
+        nbf = Pa.shape[0]
+        Va, Vb = (torch.empty((nbf, nbf), dtype=torch.float64, device=self.device)
+                  for _ in range(2))
+        ids, nf = _ids(funcs)
+        E = ctypes.c_double()
+        N = (ctypes.c_double * 2)()
+        self._chk(lib().nkx_vxc_polarized(self.h, _ptr(Pa), _ptr(Pb), nf, ids, prec,
+                                          ctypes.byref(E), N, _ptr(Va), _ptr(Vb)))
+        return XC_result_polarized(E.value, Va, Vb, (N[0], N[1]))
+
     STAGES = ("collocation", "density", "vgemm", "final", "build")
 
     def set_timing(self, on=True):
@@ -374,6 +399,53 @@ def functional_eval(ctx, funcs, rho, sigma=None):
                                        _ptr(None if sigma is None else sigma.contiguous()),
                                        _ptr(exc), _ptr(vr), _ptr(vs)))
     return exc, vr, vs
+
+
+def functional_eval_polarized(ctx, funcs, rho, sigma=None):
+    """Spin-polarised exc, vrho, vsigma on device; rho (N, 2), sigma (N, 3)
+    in libxc's interleaved XC_POLARIZED layout."""
+    ids, nf = _ids(funcs)
+    n = rho.shape[0]
+    exc = torch.empty(n, dtype=torch.float64, device=ctx.device)
+    vr = torch.empty((n, 2), dtype=torch.float64, device=ctx.device)
+    vs = torch.empty((n, 3), dtype=torch.float64, device=ctx.device) if sigma is not None else None
+    ctx._done(lib().nkx_functional_eval_polarized(
+        ctx.h, nf, ids, n, _ptr(rho.contiguous()),
+        _ptr(None if sigma is None else sigma.contiguous()), _ptr(exc), _ptr(vr), _ptr(vs)))
+    return exc, vr, vs
+
+
+def _xc_pol(ctx, phi, grads, weights, mo_alpha, occ_alpha, mo_beta, occ_beta, funcs, family):
+    phi = _left(phi)
+    nbf, n = phi.shape
+    g = [None, None, None] if grads is None else [_left(x) for x in grads]
+    Ca = _left(_dev_f64(mo_alpha, ctx.device).reshape(nbf, -1))
+    Cb = _left(_dev_f64(mo_beta, ctx.device).reshape(nbf, -1))
+    fa = _dev_f64(occ_alpha, ctx.device).contiguous()
+    fb = _dev_f64(occ_beta, ctx.device).contiguous()
+    w = weights.contiguous()
+    Va, Vb = (torch.empty((nbf, nbf), dtype=torch.float64, device=ctx.device) for _ in range(2))
+    E = ctypes.c_double()
+    ids, nf = _ids(funcs)
+    ctx._chk(lib().nkx_xc_from_collocation_polarized(
+        ctx.h, nbf, n, _ptr(phi), _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), _ptr(w), Ca.shape[1],
+        _ptr(Ca), _ptr(fa), Cb.shape[1], _ptr(Cb), _ptr(fb), nf, ids, family, ctypes.byref(E),
+        _ptr(Va), _ptr(Vb)))
+    return XC_result_polarized(E.value, Va, Vb)
+
+
+def compute_lsda(ctx, collocation_values, weights, mo_alpha, occ_alpha, mo_beta, occ_beta,
+                 funcs):
+    """compute_lsda (xc_integrals.hpp:169-262)."""
+    return _xc_pol(ctx, collocation_values, None, weights, mo_alpha, occ_alpha, mo_beta,
+                   occ_beta, funcs, _capi.FAMILY_LDA)
+
+
+def compute_gga_lsda(ctx, collocation_values, gx, gy, gz, weights, mo_alpha, occ_alpha,
+                     mo_beta, occ_beta, funcs):
+    """compute_gga_lsda (xc_integrals.hpp:543-718)."""
+    return _xc_pol(ctx, collocation_values, (gx, gy, gz), weights, mo_alpha, occ_alpha,
+                   mo_beta, occ_beta, funcs, _capi.FAMILY_GGA)
 
 
 def _xc(ctx, phi, grads, weights, mo_orbitals, mo_coeff, funcs, family):

@@ -485,6 +485,66 @@ int nkx_functional_eval(nkx_ctx *ctx, int nfunc, const int *ids, int64_t npts,
   });
 }
 
+namespace {
+// compute_lda / compute_gga family checks (xc_integrals.hpp:85-91, :437-447)
+void family_require(int family_check, int nfunc, const int *ids, bool gga) {
+  if (family_check == NKX_FAMILY_LDA)
+    NKX_REQUIRE(!gga, NKX_ERR_FUNCTIONAL_FAMILY,
+                "Provided functional is not part of the LDA family");
+  if (family_check == NKX_FAMILY_GGA) {
+    bool all_gga = true;
+    for (int k = 0; k < nfunc; ++k) {
+      int fam, kind;
+      functional_info(ids[k], &fam, &kind);
+      all_gga = all_gga && fam == NKX_FAMILY_GGA;
+    }
+    NKX_REQUIRE(all_gga, NKX_ERR_FUNCTIONAL_FAMILY,
+                "Provided funtional is not a part of the GGA or Hybrid GGA family");
+  }
+}
+}  // namespace
+
+int nkx_functional_eval_polarized(nkx_ctx *ctx, int nfunc, const int *ids, int64_t npts,
+                                  const double *rho, const double *sigma, double *exc,
+                                  double *vrho, double *vsigma) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(!gga || (sigma && vsigma), NKX_ERR_FUNCTIONAL_FAMILY,
+                "GGA functionals need sigma and vsigma");
+    NKX_REQUIRE(npts >= 0 && (npts == 0 || (rho && exc && vrho)), NKX_ERR_INVALID_ARGUMENT,
+                "functional_eval_polarized: bad arguments");
+    nkx::launch_functional_eval_pol(ctx->c.stream, xl, npts, rho, gga ? sigma : nullptr, exc,
+                                    vrho, gga ? vsigma : nullptr);
+  });
+}
+
+int nkx_xc_from_collocation_polarized(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi,
+                                      const double *gx, const double *gy, const double *gz,
+                                      const double *w, int nocca, const double *Ca,
+                                      const double *fa, int noccb, const double *Cb,
+                                      const double *fb, int nfunc, const int *ids,
+                                      int family_check, double *E, double *Va, double *Vb) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    family_require(family_check, nfunc, ids, gga);
+    NKX_REQUIRE(!gga || (gx && gy && gz), NKX_ERR_INVALID_ARGUMENT,
+                "GGA functionals need the collocation gradients");
+    NKX_REQUIRE(nbf > 0 && npts >= 0 && nocca >= 0 && noccb >= 0 && phi && w && E && Va && Vb &&
+                    (nocca == 0 || (Ca && fa)) && (noccb == 0 || (Cb && fb)),
+                NKX_ERR_INVALID_ARGUMENT, "xc_from_collocation_polarized: bad arguments");
+    nkx::xc_from_collocation_pol(ctx->c, nbf, npts, phi, gx, gy, gz, w, nocca, Ca, fa, noccb, Cb,
+                                 fb, xl, gga, E, Va, Vb);
+  });
+}
+
 int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *phi,
                             const double *gx, const double *gy, const double *gz,
                             const double *w, int nocc, const double *C, const double *f,
@@ -495,19 +555,7 @@ int nkx_xc_from_collocation(nkx_ctx *ctx, int nbf, int64_t npts, const double *p
     bool gga = false;
     NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
                 NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
-    if (family_check == NKX_FAMILY_LDA)
-      NKX_REQUIRE(!gga, NKX_ERR_FUNCTIONAL_FAMILY,
-                  "Provided funtional is not a part of the LDA family");
-    if (family_check == NKX_FAMILY_GGA) {
-      bool all_gga = true;
-      for (int k = 0; k < nfunc; ++k) {
-        int fam, kind;
-        functional_info(ids[k], &fam, &kind);
-        all_gga = all_gga && fam == NKX_FAMILY_GGA;
-      }
-      NKX_REQUIRE(all_gga, NKX_ERR_FUNCTIONAL_FAMILY,
-                  "Provided funtional is not a part of the GGA or Hybrid GGA family");
-    }
+    family_require(family_check, nfunc, ids, gga);
     NKX_REQUIRE(!gga || (gx && gy && gz), NKX_ERR_INVALID_ARGUMENT,
                 "GGA functionals need the collocation gradients");
     NKX_REQUIRE(nbf > 0 && npts >= 0 && phi && w && C && f && E && V,
@@ -528,6 +576,32 @@ int nkx_vxc(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precis
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
     NKX_REQUIRE(P && E && N && V, NKX_ERR_INVALID_ARGUMENT, "vxc: bad arguments");
     nkx::vxc_build(ctx->c, P, xl, gga, precision, E, N, V);
+  });
+}
+
+int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfunc,
+                      const int *ids, int precision, double *E, double *N, double *Va,
+                      double *Vb) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_INVALID_ARGUMENT, "unknown precision");
+    NKX_REQUIRE(precision == NKX_PREC_FP64, NKX_ERR_UNSUPPORTED,
+                "the spin-polarised build is available in the FP64 form only");
+    NKX_REQUIRE(Pa && Pb && E && Va && Vb && Va != Vb, NKX_ERR_INVALID_ARGUMENT,
+                "vxc_polarized: bad arguments");
+    const double *P[2] = {Pa, Pb};
+    double *V[2] = {Va, Vb};
+    double n2[2];
+    nkx::vxc_build_spin(ctx->c, 2, P, xl, gga, precision, E, n2, V);
+    if (N) {
+      N[0] = n2[0];
+      N[1] = n2[1];
+    }
   });
 }
 

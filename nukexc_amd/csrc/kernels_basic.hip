@@ -5,6 +5,7 @@
 #include "nkx_internal.hpp"
 #include "sto_eval.hpp"
 #include "xc_functionals.hpp"
+#include "xc_polarized.hpp"
 
 namespace nkx {
 
@@ -203,6 +204,39 @@ void launch_functional_eval(hipStream_t s, XcList xl, int64_t npts, const double
   NKX_HIP_CHECK(hipGetLastError());
 }
 
+// Spin-polarised evaluation, libxc XC_POLARIZED layout (rho[2g+s],
+// sigma[3g+k], vrho[2g+s], vsigma[3g+k]); exc per particle.
+__global__ void __launch_bounds__(256) functional_pol_kernel(XcList xl, int64_t npts,
+                                                             const double *__restrict__ rho,
+                                                             const double *__restrict__ sigma,
+                                                             double *exc, double *vrho,
+                                                             double *vsigma) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts) return;
+  const double r[2] = {rho[2 * g], rho[2 * g + 1]};
+  const double sg[3] = {sigma ? sigma[3 * g] : 0.0, sigma ? sigma[3 * g + 1] : 0.0,
+                        sigma ? sigma[3 * g + 2] : 0.0};
+  double e, vr[2], vs[3];
+  xc_eval_polarized(xl.n, xl.ids, xl.thr, r, sg, e, vr, vs);
+  exc[g] = e;
+  vrho[2 * g] = vr[0];
+  vrho[2 * g + 1] = vr[1];
+  if (vsigma) {
+    vsigma[3 * g] = vs[0];
+    vsigma[3 * g + 1] = vs[1];
+    vsigma[3 * g + 2] = vs[2];
+  }
+}
+
+void launch_functional_eval_pol(hipStream_t s, XcList xl, int64_t npts, const double *rho,
+                                const double *sigma, double *exc, double *vrho,
+                                double *vsigma) {
+  if (npts == 0) return;
+  hipLaunchKernelGGL(functional_pol_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
+                     s, xl, npts, rho, sigma, exc, vrho, vsigma);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // Reference-shaped density (density.hpp:150-255): psi_i = sum_mu C(mu,i)
 // Phi(mu,g), rho = sum_i f_i psi_i^2, grad rho = 2 sum_i f_i psi_i grad psi_i.
@@ -340,6 +374,132 @@ __global__ void symmetrize_kernel(int nbf, const double *__restrict__ Vb, double
   if (e >= (int64_t)nbf * nbf) return;
   const int i = (int)(e / nbf), j = (int)(e % nbf);
   V[e] = Vb[(size_t)i * nbf + j] + Vb[(size_t)j * nbf + i];
+}
+
+// Spin-polarised Z for spin s (compute_lsda xc_integrals.hpp:218-240,
+// compute_gga_lsda :653-712): Z = w (1/2 Phi vrho_s + vec_s . grad Phi) with
+// vec_a = 2 vsigma_aa grad rho_a + vsigma_ab grad rho_b (and a <-> b).
+// gr holds grad rho_a (x, y, z) then grad rho_b (x, y, z), npts each.
+__global__ void __launch_bounds__(256) zmat_pol_kernel(int nbf, int64_t npts, int spin,
+                                                       const double *__restrict__ phi,
+                                                       const double *__restrict__ gx,
+                                                       const double *__restrict__ gy,
+                                                       const double *__restrict__ gz,
+                                                       const double *__restrict__ w,
+                                                       const double *__restrict__ gr,
+                                                       const double *__restrict__ vrho,
+                                                       const double *__restrict__ vsig,
+                                                       double *Z) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nbf * npts) return;
+  const int64_t g = e / nbf;
+  double z = 0.5 * phi[e] * vrho[2 * g + spin];
+  if (gx) {
+    const double *gs = gr + (int64_t)(3 * spin) * npts, *go = gr + (int64_t)(3 - 3 * spin) * npts;
+    const double vss = 2.0 * vsig[3 * g + 2 * spin], vab = vsig[3 * g + 1];
+    const double vx = vss * gs[g] + vab * go[g];
+    const double vy = vss * gs[npts + g] + vab * go[npts + g];
+    const double vz = vss * gs[2 * npts + g] + vab * go[2 * npts + g];
+    z += vx * gx[e] + vy * gy[e] + vz * gz[e];
+  }
+  Z[e] = w[g] * z;
+}
+
+// rho (2g + s) and sigma (3g + k) interleaved from per-spin densities and
+// gradients; energy sum_g w (rho_a + rho_b) exc accumulated by a later pass.
+__global__ void __launch_bounds__(256) pack_pol_kernel(int64_t npts, const double *__restrict__ ra,
+                                                       const double *__restrict__ rb,
+                                                       const double *__restrict__ gr,
+                                                       double *rho2, double *sig3) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npts) return;
+  rho2[2 * g] = ra[g];
+  rho2[2 * g + 1] = rb[g];
+  if (sig3) {
+    const double ax = gr[g], ay = gr[npts + g], az = gr[2 * npts + g];
+    const double bx = gr[3 * npts + g], by = gr[4 * npts + g], bz = gr[5 * npts + g];
+    sig3[3 * g] = ax * ax + ay * ay + az * az;
+    sig3[3 * g + 1] = ax * bx + ay * by + az * bz;
+    sig3[3 * g + 2] = bx * bx + by * by + bz * bz;
+  }
+}
+
+__global__ void __launch_bounds__(256) energy_pol_kernel(int64_t npts, const double *__restrict__ w,
+                                                         const double *__restrict__ rho2,
+                                                         const double *__restrict__ exc,
+                                                         double *EN) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (g < npts) e = w[g] * (rho2[2 * g] + rho2[2 * g + 1]) * exc[g];
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(EN, part[0] + part[1] + part[2] + part[3]);
+}
+
+// V = Phi Z^T + (Phi Z^T)^T for one spin channel.
+static void assemble_v(hipStream_t s, int nbf, int64_t npts, const double *phi, const double *Z,
+                       double *Vb, double *V) {
+  NKX_HIP_CHECK(hipMemsetAsync(Vb, 0, (size_t)nbf * nbf * sizeof(double), s));
+  if (npts > 0) {
+    const int tiles = (nbf + 63) / 64;
+    int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(512 / (tiles * tiles) + 1,
+                                                            (npts + 1023) / 1024));
+    const int64_t kchunk = ((npts + nsplit - 1) / nsplit + 15) / 16 * 16;
+    nsplit = (npts + kchunk - 1) / kchunk;
+    hipLaunchKernelGGL(gemm_nt_ref_kernel, dim3(tiles, tiles, (unsigned)nsplit), dim3(256), 0, s,
+                       nbf, npts, phi, Z, Vb, kchunk);
+    NKX_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t nn = (int64_t)nbf * nbf;
+  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nbf,
+                     Vb, V);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void xc_from_collocation_pol(Ctx &c, int nbf, int64_t npts, const double *phi, const double *gx,
+                             const double *gy, const double *gz, const double *w, int nocca,
+                             const double *Ca, const double *fa, int noccb, const double *Cb,
+                             const double *fb, const XcList &xl, bool gga, double *E,
+                             double *Va, double *Vb) {
+  hipStream_t s = c.stream;
+  const bool grad = gga && gx;
+  // per-point scratch: ra, rb, grad (6), rho2 (2), sig3 (3), exc, vrho (2),
+  // vsig (3) = 19 npts; Z; Vbar; E
+  const size_t per = 19 * (size_t)npts;
+  c.ws.scratch.reserve(per + (size_t)nbf * npts + (size_t)nbf * nbf + 1);
+  double *ra = c.ws.scratch.p, *rb = ra + npts, *gr = rb + npts, *rho2 = gr + 6 * npts,
+         *sig3 = rho2 + 2 * npts, *exc = sig3 + 3 * npts, *vr = exc + npts, *vs = vr + 2 * npts;
+  double *Z = c.ws.scratch.p + per;
+  double *Vbar = Z + (size_t)nbf * npts;
+  double *EN = Vbar + (size_t)nbf * nbf;
+  launch_density_ref(s, nbf, npts, phi, grad ? gx : nullptr, gy, gz, nocca, Ca, fa, ra,
+                     grad ? gr : nullptr, gr + npts, gr + 2 * npts, nullptr);
+  launch_density_ref(s, nbf, npts, phi, grad ? gx : nullptr, gy, gz, noccb, Cb, fb, rb,
+                     grad ? gr + 3 * npts : nullptr, gr + 4 * npts, gr + 5 * npts, nullptr);
+  const unsigned nblk = (unsigned)((npts + 255) / 256);
+  NKX_HIP_CHECK(hipMemsetAsync(EN, 0, sizeof(double), s));
+  if (npts > 0) {
+    hipLaunchKernelGGL(pack_pol_kernel, dim3(nblk), dim3(256), 0, s, npts, ra, rb, gr, rho2,
+                       grad ? sig3 : nullptr);
+    NKX_HIP_CHECK(hipGetLastError());
+    launch_functional_eval_pol(s, xl, npts, rho2, grad ? sig3 : nullptr, exc, vr,
+                               grad ? vs : nullptr);
+    hipLaunchKernelGGL(energy_pol_kernel, dim3(nblk), dim3(256), 0, s, npts, w, rho2, exc, EN);
+    NKX_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t ne = (int64_t)nbf * npts;
+  for (int spin = 0; spin < 2; ++spin) {
+    if (ne > 0) {
+      hipLaunchKernelGGL(zmat_pol_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                         nbf, npts, spin, phi, grad ? gx : nullptr, gy, gz, w, gr, vr, vs, Z);
+      NKX_HIP_CHECK(hipGetLastError());
+    }
+    assemble_v(s, nbf, npts, phi, Z, Vbar, spin ? Vb : Va);
+  }
+  NKX_HIP_CHECK(hipMemcpyAsync(E, EN, sizeof(double), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void xc_from_collocation(Ctx &c, int nbf, int64_t npts, const double *phi, const double *gx,

@@ -2,7 +2,8 @@
 //
 // Data layout in HBM (one chunk of point batches at a time):
 //   tile store   [slot][array][64 functions][PT points], array = Phi, dPhi/dx,
-//                dPhi/dy, dPhi/dz, Z (GGA) or Phi, Z (LDA).  A slot is one
+//                dPhi/dy, dPhi/dz, Z (GGA) or Phi, Z (LDA); spin-polarised
+//                builds carry Z_alpha, Z_beta in place of Z.  A slot is one
 //                (function block, point batch) pair that survived screening.
 //   Pd           permuted, zero-padded density matrix (nbf_pad x nbf_pad).
 //   vpart        split-K partial V tiles [split][pair][64][64].
@@ -22,6 +23,7 @@
 #include "sto_eval.hpp"
 #include "vxc_kernels.hpp"
 #include "xc_functionals.hpp"
+#include "xc_polarized.hpp"
 
 namespace nkx {
 
@@ -70,6 +72,11 @@ __global__ void __launch_bounds__(256) colloc_tiles_f64(
 }
 
 // ---------------------------------------------------------------------------
+// NSPIN = 1: P is the total density matrix.  NSPIN = 2: Pd holds P_alpha then
+// P_beta (each nbf_pad^2); the functional is evaluated spin-polarised and the
+// tile store carries Z_alpha, Z_beta as its last two arrays.  blockEN holds
+// (E, N_alpha, N_beta) per batch (N_beta = 0 for NSPIN = 1, N_alpha = N).
+template <int NSPIN>
 __global__ void __launch_bounds__(256) density_fused_f64(
     int nbf_pad, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
@@ -80,86 +87,90 @@ __global__ void __launch_bounds__(256) density_fused_f64(
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
 
-  __shared__ double Pk[FB][FB];       // Pk[k][i] = P(mu = i, nu = k) of the block pair
-  __shared__ double Ph[FB][PT];       // Phi tile of the K block
-  __shared__ double red[4][4][PT];    // per-wave column partials
-  __shared__ double colsum[4][PT];    // rho, d/dx, d/dy, d/dz column sums
-  __shared__ double cf[4][PT];        // Z coefficients
+  __shared__ double Pk[FB][FB];          // Pk[k][i] = P(mu = i, nu = k) of the block pair
+  __shared__ double Ph[FB][PT];          // Phi tile of the K block
+  __shared__ double red[4][4][PT];       // per-wave column partials
+  __shared__ double colsum[NSPIN][4][PT];  // rho, d/dx, d/dy, d/dz column sums
+  __shared__ double cf[NSPIN][4][PT];      // Z coefficients
 
-  for (int e = threadIdx.x; e < 4 * PT; e += 256) (&colsum[0][0])[e] = 0.0;
+  for (int e = threadIdx.x; e < NSPIN * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
   __syncthreads();
 
   const int nq = gga ? 4 : 1;
-  for (int mi = lo; mi < hi; ++mi) {
-    const int fm = b_fbl[mi];
-    v4d acc[4];
-    for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int ki = lo; ki < hi; ++ki) {
-      const int fk = b_fbl[ki];
-      {
-        // P block: rows nu in block fk, cols mu in block fm (P symmetric), and
-        // the Phi tile of block fk; consecutive lanes move consecutive double2.
-        const double *psrc = Pd + (int64_t)(fk * FB) * nbf_pad + fm * FB;
-        const double *tsrc = tiles + (ki - s0) * narr * TILE;
+  for (int sp = 0; sp < NSPIN; ++sp) {
+    const double *Psp = Pd + (int64_t)sp * nbf_pad * nbf_pad;
+    for (int mi = lo; mi < hi; ++mi) {
+      const int fm = b_fbl[mi];
+      v4d acc[4];
+      for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int ki = lo; ki < hi; ++ki) {
+        const int fk = b_fbl[ki];
+        {
+          // P block: rows nu in block fk, cols mu in block fm (P symmetric), and
+          // the Phi tile of block fk; consecutive lanes move consecutive double2.
+          const double *psrc = Psp + (int64_t)(fk * FB) * nbf_pad + fm * FB;
+          const double *tsrc = tiles + (ki - s0) * narr * TILE;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = 2 * (u * 256 + (int)threadIdx.x);
-          const int r = e >> 6, c = e & 63;
-          *reinterpret_cast<double2 *>(&Pk[r][c]) =
-              *reinterpret_cast<const double2 *>(psrc + (int64_t)r * nbf_pad + c);
-          *reinterpret_cast<double2 *>(&Ph[r][c]) =
-              *reinterpret_cast<const double2 *>(tsrc + e);
+          for (int u = 0; u < 8; ++u) {
+            const int e = 2 * (u * 256 + (int)threadIdx.x);
+            const int r = e >> 6, c = e & 63;
+            *reinterpret_cast<double2 *>(&Pk[r][c]) =
+                *reinterpret_cast<const double2 *>(psrc + (int64_t)r * nbf_pad + c);
+            *reinterpret_cast<double2 *>(&Ph[r][c]) =
+                *reinterpret_cast<const double2 *>(tsrc + e);
+          }
         }
-      }
-      __syncthreads();
+        __syncthreads();
 #pragma unroll 4
-      for (int k0 = 0; k0 < FB; k0 += 4) {
-        const double a = Pk[k0 + q][16 * wv + l16];
+        for (int k0 = 0; k0 < FB; k0 += 4) {
+          const double a = Pk[k0 + q][16 * wv + l16];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double bv = Ph[k0 + q][16 * j + l16];
-          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) {
+            const double bv = Ph[k0 + q][16 * j + l16];
+            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[j], 0, 0, 0);
+          }
         }
+        __syncthreads();
+      }
+      // Epilogue: column partial sums sum_mu Phi(mu,g) X(mu,g) (and gradients).
+      const double *tb = tiles + (mi - s0) * narr * TILE;
+      double part[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int g = 16 * j + l16;
+        double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = 16 * wv + 4 * r + q;
+          const double x = acc[j][r];
+          sacc[0] += tb[mu * PT + g] * x;
+          if (gga) {
+            sacc[1] += tb[TILE + mu * PT + g] * x;
+            sacc[2] += tb[2 * TILE + mu * PT + g] * x;
+            sacc[3] += tb[3 * TILE + mu * PT + g] * x;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = sacc[c];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          part[j][c] = v;
+        }
+      }
+      if (q == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
+      }
+      __syncthreads();
+      {
+        const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+        if (c < nq)
+          colsum[sp][c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
       }
       __syncthreads();
     }
-    // Epilogue: column partial sums sum_mu Phi(mu,g) X(mu,g) (and gradients).
-    const double *tb = tiles + (mi - s0) * narr * TILE;
-    double part[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int g = 16 * j + l16;
-      double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int mu = 16 * wv + 4 * r + q;
-        const double x = acc[j][r];
-        s[0] += tb[mu * PT + g] * x;
-        if (gga) {
-          s[1] += tb[TILE + mu * PT + g] * x;
-          s[2] += tb[2 * TILE + mu * PT + g] * x;
-          s[3] += tb[3 * TILE + mu * PT + g] * x;
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        double v = s[c];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        part[j][c] = v;
-      }
-    }
-    if (q == 0) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
-    }
-    __syncthreads();
-    {
-      const int c = threadIdx.x / PT, g = threadIdx.x % PT;
-      if (c < nq) colsum[c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
-    }
-    __syncthreads();
   }
 
   // Functional and Z coefficients, one thread per point (wave 0).
@@ -167,41 +178,72 @@ __global__ void __launch_bounds__(256) density_fused_f64(
     const int g = threadIdx.x;
     const int64_t gp = (int64_t)b * PT + g;
     const double w = pw[gp];
-    const double rho = colsum[0][g];
-    const double gxr = 2.0 * colsum[1][g], gyr = 2.0 * colsum[2][g], gzr = 2.0 * colsum[3][g];
-    const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
-    const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
-    const double a = 0.5 * w * o.vrho;
-    const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
-    cf[0][g] = a;
-    cf[1][g] = bs * gxr;
-    cf[2][g] = bs * gyr;
-    cf[3][g] = bs * gzr;
-    coef[4 * gp + 0] = a;
-    coef[4 * gp + 1] = bs * gxr;
-    coef[4 * gp + 2] = bs * gyr;
-    coef[4 * gp + 3] = bs * gzr;
-    double e = w * o.e, n = w * rho;
+    double e, na, nb = 0.0;
+    if (NSPIN == 1) {
+      const double rho = colsum[0][0][g];
+      const double gxr = 2.0 * colsum[0][1][g], gyr = 2.0 * colsum[0][2][g],
+                   gzr = 2.0 * colsum[0][3][g];
+      const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
+      const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
+      const double a = 0.5 * w * o.vrho;
+      const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
+      cf[0][0][g] = a;
+      cf[0][1][g] = bs * gxr;
+      cf[0][2][g] = bs * gyr;
+      cf[0][3][g] = bs * gzr;
+      e = w * o.e;
+      na = w * rho;
+    } else {
+      const double r2[2] = {colsum[0][0][g], colsum[NSPIN - 1][0][g]};
+      double gr[2][3];
+      for (int s2 = 0; s2 < 2; ++s2)
+        for (int c = 0; c < 3; ++c) gr[s2][c] = 2.0 * colsum[s2 & (NSPIN - 1)][c + 1][g];
+      double sg[3] = {0.0, 0.0, 0.0};
+      if (gga) {
+        sg[0] = gr[0][0] * gr[0][0] + gr[0][1] * gr[0][1] + gr[0][2] * gr[0][2];
+        sg[1] = gr[0][0] * gr[1][0] + gr[0][1] * gr[1][1] + gr[0][2] * gr[1][2];
+        sg[2] = gr[1][0] * gr[1][0] + gr[1][1] * gr[1][1] + gr[1][2] * gr[1][2];
+      }
+      double exc, vr[2], vs[3];
+      xc_eval_polarized(xl.n, xl.ids, xl.thr, r2, sg, exc, vr, vs);
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const double vss = gga ? 2.0 * w * vs[2 * s2] : 0.0, vab = gga ? w * vs[1] : 0.0;
+        cf[s2 & (NSPIN - 1)][0][g] = 0.5 * w * vr[s2];
+        for (int c = 0; c < 3; ++c)
+          cf[s2 & (NSPIN - 1)][c + 1][g] = vss * gr[s2][c] + vab * gr[1 - s2][c];
+      }
+      e = w * (r2[0] + r2[1]) * exc;
+      na = w * r2[0];
+      nb = w * r2[1];
+    }
+    for (int s2 = 0; s2 < NSPIN; ++s2)
+      for (int c = 0; c < 4; ++c) coef[4 * NSPIN * gp + 4 * s2 + c] = cf[s2][c][g];
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       e += __shfl_down(e, o2);
-      n += __shfl_down(n, o2);
+      na += __shfl_down(na, o2);
+      nb += __shfl_down(nb, o2);
     }
     if (g == 0) {
-      blockEN[2 * b] = e;
-      blockEN[2 * b + 1] = n;
+      blockEN[3 * b] = e;
+      blockEN[3 * b + 1] = na;
+      blockEN[3 * b + 2] = nb;
     }
   }
   __syncthreads();
 
-  // Z tiles.
+  // Z tiles, Z_s in array narr - NSPIN + s.
   for (int mi = lo; mi < hi; ++mi) {
     double *tb = tiles + (mi - s0) * narr * TILE;
-    double *zt = tb + (narr - 1) * TILE;
-    for (int e = threadIdx.x; e < TILE; e += 256) {
-      const int g = e % PT;
-      double z = cf[0][g] * tb[e];
-      if (gga) z += cf[1][g] * tb[TILE + e] + cf[2][g] * tb[2 * TILE + e] + cf[3][g] * tb[3 * TILE + e];
-      zt[e] = z;
+    for (int sp = 0; sp < NSPIN; ++sp) {
+      double *zt = tb + (narr - NSPIN + sp) * TILE;
+      for (int e = threadIdx.x; e < TILE; e += 256) {
+        const int g = e % PT;
+        double z = cf[sp][0][g] * tb[e];
+        if (gga)
+          z += cf[sp][1][g] * tb[TILE + e] + cf[sp][2][g] * tb[2 * TILE + e] +
+               cf[sp][3][g] * tb[3 * TILE + e];
+        zt[e] = z;
+      }
     }
   }
 }
@@ -211,12 +253,17 @@ __global__ void __launch_bounds__(256) density_fused_f64(
 constexpr int HP = 32;      // points per LDS stage
 constexpr int LDP = HP + 1; // padded row
 
+// blockIdx.x = (zs * nsplit + split) * npairs + pair; Z is tile array
+// zbase + zs, and the partials go to vpart + zs * nsplit * npairs * FB * FB.
 __global__ void __launch_bounds__(256) vgemm_f64(
     int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
     const int *__restrict__ tile_slot, int64_t nb, int64_t b0, int64_t b1, int64_t s0,
-    int narr, const double *__restrict__ tiles, double *__restrict__ vpart) {
+    int narr, int zbase, const double *__restrict__ tiles, double *__restrict__ vpart) {
   const int p = blockIdx.x % npairs;
-  const int split = blockIdx.x / npairs;
+  const int split = (blockIdx.x / npairs) % nsplit;
+  const int zs = blockIdx.x / (npairs * nsplit);
+  const int zarr = zbase + zs;
+  vpart += (int64_t)zs * nsplit * npairs * (FB * FB);
   const int I = pair_I[p], J = pair_J[p];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
@@ -232,7 +279,7 @@ __global__ void __launch_bounds__(256) vgemm_f64(
     const double *tJ = tiles + (sJ - s0) * narr * TILE;
     for (int h = 0; h < PT; h += HP) {
       {
-        const double *zI = tI + (narr - 1) * TILE, *zJ = tJ + (narr - 1) * TILE;
+        const double *zI = tI + zarr * TILE, *zJ = tJ + zarr * TILE;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int e = 2 * (u * 256 + (int)threadIdx.x);
@@ -296,28 +343,22 @@ __global__ void __launch_bounds__(256) vreduce_kernel(int npairs, int nsplit, in
   }
 }
 
+// Fixed-order sum of the per-batch (E, N_alpha, N_beta) triples.
 __global__ void __launch_bounds__(256) en_reduce_kernel(int64_t nb, const double *__restrict__ en,
                                                         double *__restrict__ out) {
-  __shared__ double se[256], sn[256];
-  double e = 0.0, n = 0.0;
-  for (int64_t b = threadIdx.x; b < nb; b += 256) {
-    e += en[2 * b];
-    n += en[2 * b + 1];
-  }
-  se[threadIdx.x] = e;
-  sn[threadIdx.x] = n;
+  __shared__ double sv[3][256];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < nb; b += 256)
+    for (int c = 0; c < 3; ++c) v[c] += en[3 * b + c];
+  for (int c = 0; c < 3; ++c) sv[c][threadIdx.x] = v[c];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      se[threadIdx.x] += se[threadIdx.x + s];
-      sn[threadIdx.x] += sn[threadIdx.x + s];
-    }
+    if (threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) sv[c][threadIdx.x] += sv[c][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[0] = se[0];
-    out[1] = sn[0];
-  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) out[c] = sv[c][0];
 }
 
 __global__ void __launch_bounds__(256) gather_p_f64(int nbf, int nbf_pad,
@@ -359,13 +400,18 @@ void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, 
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vk_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
                 double *coef, double *blockEN) {
   if (b1 <= b0) return;
-  hipLaunchKernelGGL(density_fused_f64, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, pl.nbf_pad,
-                     Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p, xl, coef,
-                     blockEN);
+  if (nspin == 2)
+    hipLaunchKernelGGL(density_fused_f64<2>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
+                       pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
+                       xl, coef, blockEN);
+  else
+    hipLaunchKernelGGL(density_fused_f64<1>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
+                       pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
+                       xl, coef, blockEN);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
@@ -378,11 +424,11 @@ void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr
 }
 
 void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
-              int narr, const double *tiles, double *vpart) {
+              int narr, int nz, const double *tiles, double *vpart) {
   if (b1 <= b0 || pl.npairs == 0) return;
-  hipLaunchKernelGGL(vgemm_f64, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
+  hipLaunchKernelGGL(vgemm_f64, dim3((unsigned)(pl.npairs * nsplit * nz)), dim3(256), 0, s,
                      pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0, b1,
-                     s0, narr, tiles, vpart);
+                     s0, narr, narr - nz, tiles, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

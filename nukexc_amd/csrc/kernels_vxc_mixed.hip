@@ -1,16 +1,16 @@
 // Batched V_xc build on MI355X (gfx950), mixed form.
 //
-// Collocation values are FP32 numbers, each rounded once from an FP64
-// evaluation (sto_value_mixed in sto_eval.hpp).  Both contractions run on
-// v_mfma_f32_16x16x32_bf16 with every FP32 operand split into three bf16
-// terms, x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0
-// - x1), round to nearest even; exact for FP32 x; Henry, Tang & Heinecke,
-// ARITH 2019), and the six products a0 b0 + a0 b1 + a1 b0 + a0 b2 + a1 b1 +
-// a2 b0 accumulated in FP32; the dropped terms are O(2^-24) relative.  Every
-// FP32 accumulator covers at most one 64-function block (density) or one
-// 32-point MFMA step (V) before it is added into FP64 registers, so FP32
-// accumulation error does not grow with the system.  The functional and the
-// E_xc / N_el / V reductions are FP64.
+// Collocation values, Z and P are FP64 numbers split into three bf16 terms,
+// x ~ x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1),
+// round to nearest even, residuals formed in FP64; |x - x0 - x1 - x2| is
+// about 2^-27 |x|; Henry, Tang & Heinecke, ARITH 2019, split an FP32 number
+// the same way).  Both contractions run on v_mfma_f32_16x16x32_bf16 with the
+// six products a0 b0 + a0 b1 + a1 b0 + a0 b2 + a1 b1 + a2 b0 accumulated in
+// FP32; the dropped terms are O(2^-27) relative.  Collocation gradients are
+// FP32 (rounded once from FP64).  Every FP32 accumulator covers at most one
+// 64-function block (density) or one 64-point batch (V) before it is added
+// into FP64 registers, so FP32 accumulation error does not grow with the
+// system.  The functional and the E_xc / N_el / V reductions are FP64.
 //
 // Operands are stored pre-split in MFMA-fragment order, so every fragment is
 // one coalesced 16-byte load per lane and the main loops need no LDS staging
@@ -65,22 +65,24 @@ __device__ __forceinline__ unsigned short bf16_rne(float x, float &back) {
   return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ void split3(float x, unsigned short &a, unsigned short &b,
+// Three-term split of an FP64 value; the residuals are exact in FP64, each
+// term is the bf16 rounding of the FP32 rounding of the current residual.
+__device__ __forceinline__ void split3(double x, unsigned short &a, unsigned short &b,
                                        unsigned short &c) {
   float xa, xb, xc;
-  a = bf16_rne(x, xa);
-  const float r1 = x - xa;  // exact
-  b = bf16_rne(r1, xb);
-  const float r2 = r1 - xb;  // exact
-  c = bf16_rne(r2, xc);
+  a = bf16_rne((float)x, xa);
+  const double r1 = x - (double)xa;
+  b = bf16_rne((float)r1, xb);
+  const double r2 = r1 - (double)xb;
+  c = bf16_rne((float)r2, xc);
 }
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
-// value of T[row][k] from a 3-plane fragment-ordered tile
-__device__ __forceinline__ float tile_value(const unsigned short *t, int row, int k) {
+// value of T[row][k] from a 3-plane fragment-ordered tile (exact in FP64)
+__device__ __forceinline__ double tile_value(const unsigned short *t, int row, int k) {
   const int o = frag_off(row, k);
-  return (bf2f(t[o + 2 * TILE]) + bf2f(t[o + TILE])) + bf2f(t[o]);
+  return ((double)bf2f(t[o + 2 * TILE]) + (double)bf2f(t[o + TILE])) + (double)bf2f(t[o]);
 }
 
 __device__ __forceinline__ v8bf ld_frag(const unsigned short *p) {
@@ -120,8 +122,8 @@ int64_t mixed_slot_bytes(int gga) { return mixed_layout(gga).slot_elems * 2; }
 
 // ---------------------------------------------------------------------------
 // Collocation: one workgroup per live slot.  Values are computed in FP64,
-// rounded to FP32, split, staged in LDS as [plane][f][p] and written out in
-// both fragment orders.
+// split, staged in LDS as [plane][f][p] and written out in both fragment
+// orders.
 __global__ void __launch_bounds__(256) colloc_tiles_mixed(
     int64_t s0, const int *__restrict__ slot_b, const int *__restrict__ b_fbl,
     const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(256) colloc_tiles_mixed(
 #pragma unroll 1
   for (int f = row; f < FB; f += 256 / PT) {
     const FuncRec rec = fr[f];
-    float v;
+    double v;
     if (gga) {
       float a, bb, c;
       sto_value_grad_mixed(rec, T, x, y, z, v, a, bb, c);
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(256) gather_p_split(int nbf, int nfb,
   const int i = (int)(e / nbf_pad), j = (int)(e % nbf_pad);
   const double v = (i < nbf && j < nbf) ? P[(int64_t)fperm[i] * nbf + fperm[j]] : 0.0;
   unsigned short a, b, c;
-  split3((float)v, a, b, c);
+  split3(v, a, b, c);
   const int fm = i / FB, fk = j / FB;
   const int64_t o = ((int64_t)fm * nfb + fk) * TILE + frag_off(i % FB, j % FB);
   Ps[o] = a;
@@ -264,7 +266,7 @@ __global__ void __launch_bounds__(256) density_fused_mixed(
       for (int r = 0; r < 4; ++r) {
         const int mu = 16 * wv + 4 * q + r;
         const double x = acc64[j][r];
-        sm[0] += (double)tile_value(tb + L.off_F, mu, g) * x;
+        sm[0] += tile_value(tb + L.off_F, mu, g) * x;
         if (gga) {
           sm[1] += (double)G[mu * PT + g] * x;
           sm[2] += (double)G[TILE + mu * PT + g] * x;
@@ -315,14 +317,15 @@ __global__ void __launch_bounds__(256) density_fused_mixed(
       e += __shfl_down(e, o2);
       n += __shfl_down(n, o2);
     }
-    if (g == 0) {
-      blockEN[2 * b] = e;
-      blockEN[2 * b + 1] = n;
+    if (g == 0) {  // (E, N_alpha, N_beta) per batch, N_beta = 0 here
+      blockEN[3 * b] = e;
+      blockEN[3 * b + 1] = n;
+      blockEN[3 * b + 2] = 0.0;
     }
   }
   __syncthreads();
 
-  // Z = a Phi + b . grad Phi, rounded to FP32, split, fragment order (F).
+  // Z = a Phi + b . grad Phi in FP64, split, fragment order (F).
   for (int mi = lo; mi < hi; ++mi) {
     unsigned short *tb = store + (mi - s0) * L.slot_elems;
     const float *G = reinterpret_cast<const float *>(tb + L.off_G);
@@ -334,11 +337,11 @@ __global__ void __launch_bounds__(256) density_fused_mixed(
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int g = k0 + e;
-        double zz = cf[0][g] * (double)tile_value(tb + L.off_F, f, g);
+        double zz = cf[0][g] * tile_value(tb + L.off_F, f, g);
         if (gga)
           zz += cf[1][g] * (double)G[f * PT + g] + cf[2][g] * (double)G[TILE + f * PT + g] +
                 cf[3][g] * (double)G[2 * TILE + f * PT + g];
-        split3((float)zz, t0[e], t1[e], t2[e]);
+        split3(zz, t0[e], t1[e], t2[e]);
       }
       unsigned short *zt = tb + L.off_Z + c * 8;
       *reinterpret_cast<v8bf *>(zt) = *reinterpret_cast<const v8bf *>(t0);

@@ -132,8 +132,8 @@ struct Workspace {
   DevBuf<uint16_t> Ps;             // three bf16 planes of the permuted P, fragment order
   DevBuf<double> tiles64;          // Phi, gx, gy, gz, Z tiles (FP64)
   DevBuf<uint16_t> mstore;         // mixed-form tile store (bf16 planes + FP32 gradients)
-  DevBuf<double> coef;             // per padded point: a, bx, by, bz
-  DevBuf<double> blockEN;          // per batch: E, N partial sums
+  DevBuf<double> coef;             // per padded point and spin: a, bx, by, bz
+  DevBuf<double> blockEN;          // per batch: E, N_alpha, N_beta partial sums
   DevBuf<double> vpart;            // split-K partial V tiles
   DevBuf<double> red;              // small reduction outputs
   DevBuf<double> scratch;
@@ -214,6 +214,10 @@ void launch_functional_eval(hipStream_t s, XcList xl, int64_t npts, const double
                             const double *sigma, double *exc, double *vrho,
                             double *vsigma);
 
+void launch_functional_eval_pol(hipStream_t s, XcList xl, int64_t npts, const double *rho,
+                                const double *sigma, double *exc, double *vrho,
+                                double *vsigma);
+
 void launch_density_ref(hipStream_t s, int nbf, int64_t npts, const double *phi,
                         const double *gx, const double *gy, const double *gz, int nocc,
                         const double *C, const double *f, double *rho, double *grx,
@@ -223,10 +227,18 @@ void xc_from_collocation(Ctx &c, int nbf, int64_t npts, const double *phi,
                          const double *w, int nocc, const double *C, const double *f,
                          const XcList &xl, bool gga, double *E, double *V);
 
+void xc_from_collocation_pol(Ctx &c, int nbf, int64_t npts, const double *phi,
+                             const double *gx, const double *gy, const double *gz,
+                             const double *w, int nocca, const double *Ca, const double *fa,
+                             int noccb, const double *Cb, const double *fb, const XcList &xl,
+                             bool gga, double *E, double *Va, double *Vb);
+
 // Batched build (vxc.cpp)
 void ensure_plan(Ctx &c);
 void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
                double *E, double *N, double *V);
+void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+                    int precision, double *E, double *N, double *const *V);
 void overlap_build(Ctx &c, double *S);
 
 }  // namespace nkx

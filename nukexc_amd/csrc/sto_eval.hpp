@@ -75,19 +75,22 @@ __device__ __forceinline__ void sto_value_grad(const FuncRec &f, const HarmTerm 
   gz = R * Sz + S * (dz * common);
 }
 
-// Mixed-form collocation: FP32 output values, each rounded once from an FP64
-// evaluation.  Rounding is then the only FP32 error, unbiased and independent
-// from point to point.  Evaluating in FP32 instead (zeta r, normalisation or
-// harmonic coefficients rounded to FP32, or an FP32 exponential) shifts every
-// value of a function by a common relative amount of 1e-8 .. 1e-7; such a
-// systematic error survives the sum over millions of grid points and alone
-// exceeds the mixed-form budget (|dE_xc| <= 1e-6 Eh) on large systems.
-// The exponential uses an argument reduction zeta r = k ln2 + f, |f| <= ln2/2,
-// and a degree-11 Taylor polynomial for e^{-f} (truncation < 3e-13).
+// Mixed-form collocation: values evaluated in FP64 and handed to the caller
+// in FP64 (they are split into three bf16 terms there, ~2^-27 relative);
+// gradients are rounded once to FP32.  Evaluating in FP32 instead (zeta r,
+// normalisation or harmonic coefficients rounded to FP32, or an FP32
+// exponential) shifts every value of a function by a common relative amount
+// of 1e-8 .. 1e-7; such a systematic error survives the sum over millions of
+// grid points and alone exceeds the mixed-form budget (|dE_xc| <= 1e-6 Eh) on
+// large systems.
+// The exponential uses Cody-Waite argument reduction, zeta r = k ln2 + f,
+// |f| <= ln2/2, with ln2 = hi + lo, hi = ln2 truncated to its leading 40
+// significand bits (k hi is exact for |k| < 2^13) and lo = ln2 - hi rounded
+// to FP64, then a degree-11 Taylor polynomial for e^{-f} (truncation < 3e-13).
 __host__ __device__ inline double exp_neg_mixed(double y) {
   const double k = rint(y * 1.4426950408889634);  // y / ln2
-  double f = fma(-k, 6.93147180369123816490e-01, y);  // ln2 split hi / lo
-  f = fma(-k, 1.90821492927058770002e-10, f);
+  double f = fma(-k, 0x1.62e42fefa2000p-1, y);  // hi
+  f = fma(-k, 0x1.9ef35793c7673p-41, f);         // lo
   const double x = -f;
   double p = 1.0 / 39916800.0;
   p = fma(p, x, 1.0 / 3628800.0);
@@ -104,8 +107,8 @@ __host__ __device__ inline double exp_neg_mixed(double y) {
   return ldexp(p, -(int)k);
 }
 
-__device__ __forceinline__ float sto_value_mixed(const FuncRec &f, const HarmTerm *terms,
-                                                 double x, double y, double z) {
+__device__ __forceinline__ double sto_value_mixed(const FuncRec &f, const HarmTerm *terms,
+                                                  double x, double y, double z) {
   const double dx = x - f.cx, dy = y - f.cy, dz = z - f.cz;
   const double r = sqrt(dx * dx + dy * dy + dz * dz);
   const double R = f.norm * ipow(r, f.k) * exp_neg_mixed(f.zeta * r);
@@ -114,11 +117,11 @@ __device__ __forceinline__ float sto_value_mixed(const FuncRec &f, const HarmTer
     const HarmTerm h = terms[f.term_off + t];
     S += h.coef * ipow(dx, h.a) * ipow(dy, h.b) * ipow(dz, h.c);
   }
-  return (float)(R * S);
+  return R * S;
 }
 
 __device__ __forceinline__ void sto_value_grad_mixed(const FuncRec &f, const HarmTerm *terms,
-                                                     double x, double y, double z, float &v,
+                                                     double x, double y, double z, double &v,
                                                      float &gx, float &gy, float &gz) {
   const double dx = x - f.cx, dy = y - f.cy, dz = z - f.cz;
   const double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -138,7 +141,7 @@ __device__ __forceinline__ void sto_value_grad_mixed(const FuncRec &f, const Har
     if (h.c > 0) Sz += h.coef * h.c * xa * yb * zc1;
   }
   const double common = ((double)f.k / r - f.zeta) * R / r;
-  v = (float)(R * S);
+  v = R * S;
   gx = (float)(R * Sx + S * (dx * common));
   gy = (float)(R * Sy + S * (dy * common));
   gz = (float)(R * Sz + S * (dz * common));

@@ -311,15 +311,21 @@ void for_each_chunk(const Plan &pl, int64_t max_slots, F &&fn) {
 
 }  // namespace
 
-void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *E,
-               double *N, double *V) {
+// One V_xc build.  nspin = 1: P is the total density matrix, V the potential.
+// nspin = 2 (FP64 only): P[0], P[1] are the spin density matrices and V[0],
+// V[1] the spin potentials.  N[0..nspin-1] receives the electron counts.
+void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+                    int precision, double *E, double *N, double *const *V) {
   ensure_plan(c);
   Plan &pl = c.plan;
   Workspace &ws = c.ws;
   hipStream_t s = c.stream;
-  const int narr = gga ? 5 : 2;
+  const int nval = gga ? 4 : 1;
+  const int narr = nval + nspin;
   const int nsplit = choose_nsplit(pl);
   const bool mixed = precision == NKX_PREC_MIXED;
+  NKX_REQUIRE(!mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
+              "the spin-polarised build is available in the FP64 form only");
   const int nprod = c.opt.mixed_products;
   const size_t slot_bytes = mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
                                   : (size_t)narr * FB * PT * sizeof(double);
@@ -328,28 +334,31 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
+  const size_t psz = (size_t)pl.nbf_pad * pl.nbf_pad;
   if (mixed) {
     ws.mstore.reserve((size_t)slots_alloc * slot_bytes / sizeof(uint16_t));
     ws.Ps.reserve((size_t)3 * pl.nbf_pad * pl.nbf_pad);
   } else {
     ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
-    ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+    ws.Pd.reserve(nspin * psz);
   }
-  ws.coef.reserve((size_t)4 * pl.npts_pad);
-  ws.blockEN.reserve((size_t)2 * pl.nb);
+  ws.coef.reserve((size_t)4 * nspin * pl.npts_pad);
+  ws.blockEN.reserve((size_t)3 * pl.nb);
   const size_t vsz = (size_t)nsplit * std::max(pl.npairs, 1) * FB * FB;
-  ws.vpart.reserve(vsz);
-  ws.red.reserve(2);
+  ws.vpart.reserve(nspin * vsz);
+  ws.red.reserve(3);
 
   Timing &tm = c.timing;
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
-  if (mixed)
-    vkm_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P, ws.Ps.p);
-  else
-    vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
-  NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, vsz * sizeof(double), s));
-  NKX_HIP_CHECK(hipMemsetAsync(V, 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
+  for (int sp = 0; sp < nspin; ++sp) {
+    if (mixed)
+      vkm_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P[sp], ws.Ps.p);
+    else
+      vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P[sp], ws.Pd.p + sp * psz);
+    NKX_HIP_CHECK(hipMemsetAsync(V[sp], 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
+  }
+  NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, nspin * vsz * sizeof(double), s));
   tm.end(t, s);
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     const int ig = gga ? 1 : 0;
@@ -364,30 +373,35 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
       vkm_density(s, nprod, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
                   ws.blockEN.p);
     else
-      vk_density(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+      vk_density(s, nspin, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
                  ws.blockEN.p);
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
     if (mixed)
       vkm_vgemm(s, nprod, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p);
     else
-      vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
+      vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, nspin, ws.tiles64.p, ws.vpart.p);
     tm.end(tt, s);
   });
   t = tm.begin(kStageFinal, s);
-  vk_vreduce(s, pl, nsplit, ws.vpart.p, V);
+  for (int sp = 0; sp < nspin; ++sp) vk_vreduce(s, pl, nsplit, ws.vpart.p + sp * vsz, V[sp]);
   vk_en_reduce(s, pl.nb, ws.blockEN.p, ws.red.p);
   tm.end(t, s);
   tm.end(t_build, s);
-  double en[2];
+  double en[3];
   NKX_HIP_CHECK(hipMemcpyAsync(en, ws.red.p, sizeof(en), hipMemcpyDeviceToHost, s));
   NKX_HIP_CHECK(hipStreamSynchronize(s));
   tm.collect();
-  c.work.flops_density = 2.0 * pl.work_nb2g;
-  c.work.flops_v = 2.0 * pl.work_nb2g;
+  c.work.flops_density = 2.0 * nspin * pl.work_nb2g;
+  c.work.flops_v = 2.0 * nspin * pl.work_nb2g;
   c.work.colloc_elems = pl.work_nbg;
   *E = en[0];
-  *N = en[1];
+  for (int sp = 0; sp < nspin; ++sp) N[sp] = en[1 + sp];
+}
+
+void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *E,
+               double *N, double *V) {
+  vxc_build_spin(c, 1, &P, xl, gga, precision, E, N, &V);
 }
 
 int Timing::begin(int stage, hipStream_t s) {
@@ -447,7 +461,7 @@ void overlap_build(Ctx &c, double *S) {
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     vk_colloc(s, s0, s1, pl, narr, 0, ws.tiles64.p);
     vk_zoverlap(s, s0, s1, pl, narr, ws.tiles64.p);
-    vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, ws.tiles64.p, ws.vpart.p);
+    vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, 1, ws.tiles64.p, ws.vpart.p);
   });
   vk_vreduce(s, pl, nsplit, ws.vpart.p, S);
   NKX_HIP_CHECK(hipStreamSynchronize(s));

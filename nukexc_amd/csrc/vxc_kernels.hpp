@@ -9,13 +9,15 @@ void vk_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const do
                  double *Pd);
 void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
                double *tiles);
-void vk_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
                 double *coef, double *blockEN);
 void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr,
                  double *tiles);
+// nz Z arrays (the last nz of the narr per slot), partials for Z array k at
+// vpart + k * nsplit * npairs * 64 * 64
 void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
-              int narr, const double *tiles, double *vpart);
+              int narr, int nz, const double *tiles, double *vpart);
 void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V);
 void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out);
 

@@ -30,6 +30,7 @@
 //                         gives PW92, derivatives by finite differences).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -352,6 +353,112 @@ bool xc_eval_point(int id, double rho, double sigma, double thr, XcOut *o) {
 
 bool is_gga(int id) { return id == 101 || id == 130; }
 
+// ---------------------------------------------------------------------------
+// Spin-polarised functionals (libxc XC_POLARIZED, called by compute_lsda /
+// compute_gga_lsda, xc_integrals.hpp:203, :617).  Energy densities are written
+// for a generic number type; the potentials come from complex-step
+// differentiation, d e / d x = Im e(x + i h) / h with h = 1e-30, which is
+// exact to rounding (no subtractive cancellation).
+// Exchange: spin scaling E_x[a, b] = (E_x[2a] + E_x[2b]) / 2.
+// PW92 (Perdew & Wang 1992, Eq. 8 with the libxc parameter sets):
+//   eps = G0 - G2 f(z) (1 - z^4) / f''(0) + (G1 - G0) f(z) z^4.
+// PBE correlation (Perdew, Burke & Ernzerhof 1996, Eqs. 7-8) with
+//   phi = ((1+z)^(2/3) + (1-z)^(2/3)) / 2, t^2 = |grad rho|^2 / (2 phi ks rho)^2.
+// 1 +- z are formed as 2 rho_a / rho, 2 rho_b / rho and floored at DBL_EPSILON
+// (libxc's zeta threshold).
+// ---------------------------------------------------------------------------
+using cplx = std::complex<double>;
+
+template <typename T>
+T pw92_g(T rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  T srs = std::sqrt(rs);
+  T Q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * std::log(1.0 + 1.0 / Q1);
+}
+
+template <typename T>
+T floor_eps(T x) { return std::real(x) < 2.220446049250313e-16 ? T(2.220446049250313e-16) : x; }
+
+template <typename T>
+T pw92_pol(T rs, T z, T opz, T omz, bool mod) {
+  T G0 = pw92_g(rs, mod ? 0.0310907 : 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  T G1 = pw92_g(rs, mod ? 0.01554535 : 0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  T G2 = pw92_g(rs, mod ? 0.0168869 : 0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const double fpp0 = mod ? 1.709920934161365617563962776245 : 1.709921;
+  T f = (std::pow(floor_eps(opz), 4.0 / 3.0) + std::pow(floor_eps(omz), 4.0 / 3.0) - 2.0) /
+        (std::pow(2.0, 4.0 / 3.0) - 2.0);
+  T z4 = z * z * z * z;
+  return G0 - G2 * f * (1.0 - z4) / fpp0 + (G1 - G0) * f * z4;
+}
+
+template <typename T>
+T pbe_c_pol(T ra, T rb, T stot) {
+  const double beta = 0.06672455060314922, gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
+  T rho = ra + rb;
+  T z = (ra - rb) / rho;
+  T opz = 2.0 * ra / rho, omz = 2.0 * rb / rho;
+  T rs = std::pow(3.0 / (4.0 * kPi * rho), 1.0 / 3.0);
+  T ec = pw92_pol(rs, z, opz, omz, true);
+  T phi = 0.5 * (std::pow(floor_eps(opz), 2.0 / 3.0) + std::pow(floor_eps(omz), 2.0 / 3.0));
+  T kF = std::pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
+  T ks2 = 4.0 * kF / kPi;
+  T t2 = stot / (4.0 * phi * phi * ks2 * rho * rho);
+  T g3 = gamma * phi * phi * phi;
+  T A = beta / gamma / (std::exp(-ec / g3) - 1.0);
+  T u = A * t2;
+  T H = g3 * std::log(1.0 + beta / gamma * t2 * (1.0 + u) / (1.0 + u + u * u));
+  return rho * (ec + H);
+}
+
+template <typename T>
+T x_unpol(int id, T n, T s) {
+  const double Ax = -0.75 * std::cbrt(3.0 / kPi);
+  T n43 = std::pow(n, 4.0 / 3.0);
+  if (id == 1) return Ax * n43;
+  const double kappa = 0.8040, mu = 0.06672455060314922 * kPi * kPi / 3.0;
+  T s2 = s / (4.0 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0) * n43 * n43);
+  return Ax * n43 * (1.0 + kappa - kappa / (1.0 + mu * s2 / kappa));
+}
+
+// Energy density of one functional at an input point already clamped by
+// clamp_polarized: (ra, rb, saa, sab, sbb).  Spin-scaled exchange drops a
+// channel whose density is at or below the threshold (libxc screen_dens_zeta).
+template <typename T>
+T e_polarized(int id, const T *x, double thr) {
+  T ra = x[0], rb = x[1], saa = x[2], sab = x[3], sbb = x[4];
+  switch (id) {
+    case 1:
+    case 101: {
+      T e = 0.0;
+      if (std::real(ra) > thr) e += 0.5 * x_unpol(id, T(2.0) * ra, T(4.0) * saa);
+      if (std::real(rb) > thr) e += 0.5 * x_unpol(id, T(2.0) * rb, T(4.0) * sbb);
+      return e;
+    }
+    case 12:
+    case 13: {
+      T rho = ra + rb;
+      T rs = std::pow(3.0 / (4.0 * kPi * rho), 1.0 / 3.0);
+      return rho * pw92_pol(rs, (ra - rb) / rho, T(2.0) * ra / rho, T(2.0) * rb / rho, id == 13);
+    }
+    case 130:
+      return pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+  }
+  return T(0.0);
+}
+
+// libxc's polarised input clamps: rho_s >= thr, sigma_ss >= thr^(8/3),
+// |sigma_ab| <= (sigma_aa + sigma_bb) / 2.  Potentials are the derivatives at
+// the clamped point.
+void clamp_polarized(double *x, double thr) {
+  const double st = std::pow(thr, 8.0 / 3.0);
+  x[0] = std::max(x[0], thr);
+  x[1] = std::max(x[1], thr);
+  x[2] = std::max(x[2], st);
+  x[4] = std::max(x[4], st);
+  const double ave = 0.5 * (x[2] + x[4]);
+  x[3] = std::min(std::max(x[3], -ave), ave);
+}
+
 }  // namespace
 
 extern "C" {
@@ -658,6 +765,49 @@ int orc_xc_eval(int nfunc, const int *ids, double dens_thr, long npts,
 
 int orc_is_gga(int id) { return is_gga(id) ? 1 : 0; }
 
+// Polarised functional evaluation, libxc layout: rho[2 g + s],
+// sigma[3 g + {aa, ab, bb}], vrho[2 g + s], vsigma[3 g + k]; exc per particle
+// (the caller integrates w (rho_a + rho_b) exc, xc_integrals.hpp:251-258).
+int orc_xc_eval_polarized(int nfunc, const int *ids, double dens_thr, long npts,
+                          const double *rho, const double *sigma, double *exc, double *vrho,
+                          double *vsigma) {
+  for (int k = 0; k < nfunc; ++k) {
+    XcOut o;
+    if (!xc_eval_point(ids[k], 1.0, 1.0, 0.5, &o)) return 1;
+  }
+  const double h = 1e-30;
+#pragma omp parallel for schedule(static)
+  for (long g = 0; g < npts; ++g) {
+    double x[5] = {rho[2 * g], rho[2 * g + 1], sigma ? sigma[3 * g] : 0.0,
+                   sigma ? sigma[3 * g + 1] : 0.0, sigma ? sigma[3 * g + 2] : 0.0};
+    double e = 0, d[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < nfunc; ++k) {
+      const double thr = dens_thr > 0 ? dens_thr : default_dens_threshold(ids[k]);
+      if (!(x[0] + x[1] >= thr)) continue;
+      double xc[5] = {x[0], x[1], x[2], x[3], x[4]};
+      clamp_polarized(xc, thr);
+      cplx cx[5];
+      for (int j = 0; j < 5; ++j) cx[j] = cplx(xc[j], 0.0);
+      // libxc returns the energy per particle of the clamped point
+      e += std::real(e_polarized(ids[k], cx, thr)) / (xc[0] + xc[1]);
+      for (int j = 0; j < 5; ++j) {
+        cx[j] = cplx(xc[j], h);
+        d[j] += std::imag(e_polarized(ids[k], cx, thr)) / h;
+        cx[j] = cplx(xc[j], 0.0);
+      }
+    }
+    exc[g] = e;
+    vrho[2 * g] = d[0];
+    vrho[2 * g + 1] = d[1];
+    if (vsigma) {
+      vsigma[3 * g] = d[2];
+      vsigma[3 * g + 1] = d[3];
+      vsigma[3 * g + 2] = d[4];
+    }
+  }
+  return 0;
+}
+
 // E_xc, N_el and V_xc from collocation data (compute_lda xc_integrals.hpp:
 // 73-166, compute_gga :424-541): Z_ug = w_g (1/2 Phi_ug vrho_g
 // + 2 (grad rho . grad Phi_ug) vsigma_g), Vbar = Phi Z^T, V = Vbar + Vbar^T,
@@ -713,6 +863,82 @@ int orc_vxc_from_collocation(int nbf, long npts, const double *phi,
       V[(size_t)u * nbf + v] += s;
       if (u != v) V[(size_t)v * nbf + u] += s;
     }
+  return 0;
+}
+
+// Spin-polarised E_xc and V_alpha, V_beta (compute_lsda xc_integrals.hpp:
+// 169-262, compute_gga_lsda :543-718) from spin density matrices:
+// Z_s = w (1/2 Phi vrho_s + vec_s . grad Phi), vec_a = 2 vsigma_aa grad rho_a
+// + vsigma_ab grad rho_b, V_s = Phi Z_s^T + Z_s Phi^T, E = sum w (rho_a +
+// rho_b) exc.  Accumulates into E, N[2], Va, Vb.
+int orc_vxc_polarized(int nbf, long npts, const double *phi, const double *gx,
+                      const double *gy, const double *gz, const double *w,
+                      const double *Pa, const double *Pb, int nfunc, const int *ids,
+                      double dens_thr, double *E, double *N, double *Va, double *Vb) {
+  const bool gga = gx != nullptr;
+  for (int k = 0; k < nfunc; ++k)
+    if (is_gga(ids[k]) && !gga) return 2;
+  std::vector<double> rs[2], gs[2], tmp(gga ? npts : 0);
+  const double *Ps[2] = {Pa, Pb};
+  for (int s = 0; s < 2; ++s) {
+    rs[s].resize(npts);
+    gs[s].resize(gga ? 3 * npts : 0);
+    orc_density_P(nbf, npts, phi, gx, gy, gz, Ps[s], rs[s].data(),
+                  gga ? gs[s].data() : nullptr, gga ? tmp.data() : nullptr);
+  }
+  std::vector<double> rho2(2 * npts), sig3(gga ? 3 * npts : 0);
+  for (long g = 0; g < npts; ++g) {
+    rho2[2 * g] = rs[0][g];
+    rho2[2 * g + 1] = rs[1][g];
+    if (gga) {
+      const double *a = &gs[0][3 * g], *b = &gs[1][3 * g];
+      sig3[3 * g] = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+      sig3[3 * g + 1] = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+      sig3[3 * g + 2] = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    }
+  }
+  std::vector<double> exc(npts), vr(2 * npts), vs(3 * npts);
+  if (orc_xc_eval_polarized(nfunc, ids, dens_thr, npts, rho2.data(),
+                            gga ? sig3.data() : nullptr, exc.data(), vr.data(),
+                            gga ? vs.data() : nullptr))
+    return 1;
+  double e = 0, na = 0, nb = 0;
+  for (long g = 0; g < npts; ++g) {
+    e += w[g] * (rs[0][g] + rs[1][g]) * exc[g];
+    na += w[g] * rs[0][g];
+    nb += w[g] * rs[1][g];
+  }
+  *E += e;
+  N[0] += na;
+  N[1] += nb;
+  std::vector<double> Z((size_t)nbf * npts);
+  for (int s = 0; s < 2; ++s) {
+    const int o = 1 - s;
+#pragma omp parallel for schedule(static)
+    for (long g = 0; g < npts; ++g) {
+      double vec[3] = {0, 0, 0};
+      if (gga)
+        for (int c = 0; c < 3; ++c)
+          vec[c] = 2.0 * vs[3 * g + 2 * s] * gs[s][3 * g + c] + vs[3 * g + 1] * gs[o][3 * g + c];
+      for (int u = 0; u < nbf; ++u) {
+        const size_t i = (size_t)u * npts + g;
+        double z = 0.5 * phi[i] * vr[2 * g + s];
+        if (gga) z += vec[0] * gx[i] + vec[1] * gy[i] + vec[2] * gz[i];
+        Z[i] = w[g] * z;
+      }
+    }
+    double *V = s ? Vb : Va;
+#pragma omp parallel for schedule(dynamic)
+    for (int u = 0; u < nbf; ++u)
+      for (int v = 0; v <= u; ++v) {
+        double acc = 0;
+        const double *a = phi + (size_t)u * npts, *b = Z.data() + (size_t)v * npts;
+        const double *c = phi + (size_t)v * npts, *d = Z.data() + (size_t)u * npts;
+        for (long g = 0; g < npts; ++g) acc += a[g] * b[g] + c[g] * d[g];
+        V[(size_t)u * nbf + v] += acc;
+        if (u != v) V[(size_t)v * nbf + u] += acc;
+      }
+  }
   return 0;
 }
 

@@ -45,6 +45,11 @@ def lib():
         L.orc_density_C.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d,
                                     _d, _d, _d, _d]
         L.orc_xc_eval.argtypes = [ctypes.c_int, _i, ctypes.c_double, ctypes.c_long, _d, _d, _d, _d, _d]
+        L.orc_xc_eval_polarized.argtypes = [ctypes.c_int, _i, ctypes.c_double, ctypes.c_long, _d,
+                                             _d, _d, _d, _d]
+        L.orc_vxc_polarized.argtypes = [ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d, _d, _d,
+                                        ctypes.c_int, _i, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_double), _d, _d, _d]
         L.orc_vxc_from_collocation.argtypes = [ctypes.c_int, ctypes.c_long, _d, _d, _d, _d, _d, _d,
                                                ctypes.c_int, _d, _d, ctypes.c_int, _i,
                                                ctypes.c_double, _d, _d, _d]
@@ -174,6 +179,19 @@ def xc_eval(ids, rho, sigma=None, dens_thr=0.0):
     return exc, vr, vs
 
 
+def xc_eval_polarized(ids, rho2, sigma3=None, dens_thr=0.0):
+    """rho2 (n, 2), sigma3 (n, 3) -> exc (n,), vrho (n, 2), vsigma (n, 3)."""
+    ids = _i32(ids)
+    rho2 = _f64(rho2)
+    sigma3 = None if sigma3 is None else _f64(sigma3)
+    n = rho2.shape[0]
+    exc, vr, vs = np.empty(n), np.empty((n, 2)), np.empty((n, 3))
+    if lib().orc_xc_eval_polarized(len(ids), _p(ids, _i), dens_thr, n, _p(rho2), _p(sigma3),
+                                   _p(exc), _p(vr), _p(vs)):
+        raise ValueError(f"unknown functional id in {list(ids)}")
+    return exc, vr, vs
+
+
 def vxc_from_collocation(phi, w, ids, P=None, C=None, f=None, gx=None, gy=None, gz=None,
                          dens_thr=0.0, chunk=None):
     """E_xc, N_el, V_xc from collocation data (compute_lda / compute_gga).
@@ -200,6 +218,28 @@ def vxc_from_collocation(phi, w, ids, P=None, C=None, f=None, gx=None, gy=None, 
         if rc:
             raise ValueError(f"oracle vxc failed (code {rc})")
     return E.value, N.value, V
+
+
+def vxc_polarized(phi, w, ids, Pa, Pb, gx=None, gy=None, gz=None, dens_thr=0.0, chunk=None):
+    """E_xc, (N_alpha, N_beta), V_alpha, V_beta from collocation data and spin
+    density matrices (compute_lsda / compute_gga_lsda)."""
+    nbf, npts = phi.shape
+    ids = _i32(ids)
+    E = ctypes.c_double(0.0)
+    N = np.zeros(2)
+    Va, Vb = np.zeros((nbf, nbf)), np.zeros((nbf, nbf))
+    Pa, Pb = _f64(Pa), _f64(Pb)
+    step = chunk or npts
+    for s in range(0, npts, step):
+        e = slice(s, min(npts, s + step))
+        a = [_f64(x[:, e]) if x is not None else None for x in (phi, gx, gy, gz)]
+        ww = _f64(w[e])
+        rc = lib().orc_vxc_polarized(nbf, ww.shape[0], _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]),
+                                     _p(ww), _p(Pa), _p(Pb), len(ids), _p(ids, _i), dens_thr,
+                                     ctypes.byref(E), _p(N), _p(Va), _p(Vb))
+        if rc:
+            raise ValueError(f"oracle polarised vxc failed (code {rc})")
+    return E.value, N, Va, Vb
 
 
 def overlap(phi, w):

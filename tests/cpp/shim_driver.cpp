@@ -1,7 +1,8 @@
 // Exercises the C++ shim (include/nukexc/*.hpp) the way the reference's unit
 // tests exercise its headers (test_hydrogen.cxx:815-912, test_helium.cxx:
-// 67-121): grid, basis, collocation, compute_lda / compute_gga, plus the
-// batched compute_vxc and the error path.  Prints one line per check and
+// 67-121): grid, basis, collocation, compute_lda / compute_gga, the
+// polarised compute_lsda / compute_gga_lsda, the batched compute_vxc(_polarized)
+// and the error path.  Prints one line per check and
 // exits nonzero on the first failure.
 #include <cmath>
 #include <cstdio>
@@ -52,6 +53,23 @@ int main() {
   expect_rel("H batched N_el", vb.n_el, 1.0, 1e-10);
   auto vm = compute_vxc(grid, basis, P, Functional({101}), Precision::Mixed);
   expect_rel("H batched mixed energy", vm.energy, -0.253995708307881, 1e-6);
+
+  // Spin-polarised H (compute_lsda, xc_integrals.hpp:169-262): one alpha
+  // electron, the beta orbital unoccupied.  Spin scaling of exchange gives
+  // E_x[rho, 0] = 2^(1/3) E_x^unpolarised[rho].
+  DeviceView1D f0(1);
+  f0.from_host(std::vector<double>{0.0});
+  auto lsda = compute_lsda(phi, grid.weights, C, f, C, f0, Functional({1}));
+  expect_rel("H LSDA_X energy", lsda.energy, std::cbrt(2.0) * -0.2127415030860106, 1e-10);
+  DeviceView2DLeft P0(1, 1);
+  P0.from_host(std::vector<double>{0.0});
+  auto vp = compute_vxc_polarized(grid, basis, P, P0, Functional({1}));
+  expect_rel("H batched LSDA_X energy", vp.energy, lsda.energy, 1e-12);
+  expect_rel("H batched LSDA V_alpha", vp.potential_alpha.to_host()[0],
+             lsda.potential_alpha.to_host()[0], 1e-12);
+  auto glsda = compute_gga_lsda(phi, gx, gy, gz, grid.weights, C, f, C, f0, Functional({101, 130}));
+  auto vpg = compute_vxc_polarized(grid, basis, P, P0, Functional({101, 130}));
+  expect_rel("H batched PBE polarised", vpg.energy, glsda.energy, 1e-12);
 
   // xc_integrals.hpp:434-438: an LDA functional handed to compute_gga throws
   bool threw = false;
