@@ -218,6 +218,16 @@ int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
 int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, int nfunc,
                       const int *func_ids, int precision, double *E_xc, double *N_el,
                       double *Va_dev, double *Vb_dev);
+/* Nuclear gradient dE_xc/dR_A at fixed P (FP64; no counterpart in the
+   reference, which has no gradients): basis-centre, grid-point-motion and
+   partition-weight terms for a grid whose points move with their owner atom.
+   partition: 0 = weights held fixed, 1 = capped Becke (the partition of
+   nkx_make_flat_grid, partitioning.hpp:126-213), 2 = Becke
+   (partitioning.hpp:63-120).  Every basis function must be centred on an atom
+   of the ctx grid.  E_xc (may be NULL) and grad (host, 3 * natoms, atom-major
+   x, y, z) are written on return. */
+int nkx_xc_grad(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
+                int partition, double *E_xc, double *grad_host);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);

@@ -68,6 +68,7 @@ SIGNATURES = {
                                           _vp, _vp, _c_int, _vp, _vp, _c_int, _pi, _c_int, _pd,
                                           _vp, _vp],
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
+    "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_set_screening": [_vp, _c_double, _c_int],
     "nkx_set_mixed_products": [_vp, _c_int],
     "nkx_plan_stats": [_vp, _pi64, _pi64, _pi64, _pi64, _pi64],

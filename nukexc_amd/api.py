@@ -285,6 +285,21 @@ class Context:
                                           ctypes.byref(E), N, _ptr(Va), _ptr(Vb)))
         return XC_result_polarized(E.value, Va, Vb, (N[0], N[1]))
 
+    PARTITION = {"fixed": 0, "becke_capped": 1, "becke": 2}
+
+    def xc_grad(self, P, funcs=PBE, partition="becke_capped"):
+        """Nuclear gradient of E_xc at fixed P on the ctx grid (points move
+        with their owner atom, weights from the named partition).
+        Returns (E_xc, grad ndarray (natoms, 3))."""
+        P = _dev_f64(P, self.device).contiguous()
+        natoms = self.grid.atom_centers.shape[0]
+        g = np.zeros(3 * natoms)
+        ids, nf = _ids(funcs)
+        E = ctypes.c_double()
+        self._chk(lib().nkx_xc_grad(self.h, _ptr(P), nf, ids, self.PARTITION[partition],
+                                    ctypes.byref(E), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return E.value, g.reshape(natoms, 3)
+
     STAGES = ("collocation", "density", "vgemm", "final", "build")
 
     def set_timing(self, on=True):

@@ -605,6 +605,21 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfun
   });
 }
 
+int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
+                double *E, double *grad) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(P && grad, NKX_ERR_INVALID_ARGUMENT, "xc_grad: bad arguments");
+    double e = 0.0;
+    nkx::xc_grad(ctx->c, P, xl, gga, partition, &e, grad);
+    if (E) *E = e;
+  });
+}
+
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

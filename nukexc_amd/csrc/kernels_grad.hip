@@ -1,0 +1,363 @@
+// Nuclear gradient of E_xc at fixed density matrix P (FP64).  The reference
+// has no gradient; the terms follow from E = sum_g w_g f(rho_g, sigma_g) with
+// basis functions centred on atoms, grid points moving with their owner atom
+// and Becke weights w_g = w0_g P_B(r_g) / sum_i P_i(r_g) (B = owner):
+//
+//  basis-centre term, atom A, function mu on A, per point g:
+//    c_mu(g) = -2 [ d phi_mu(g) (P Y)_mu(g) + (H_mu(g) b_g) (P Phi)_mu(g) ]
+//    Y_nu(g) = w v_rho phi_nu + b_g . grad phi_nu,   b_g = 2 w v_sigma grad rho
+//    (H_mu = Hessian of phi_mu; GGA only)
+//  point-motion term: rigid translation of every atom together with the
+//    point leaves rho(r_g) unchanged, so the owner receives -sum_mu c_mu(g);
+//  weight term: f_g dw_g/dR_A = (w f)_g sum_i ([i == B]/P_B - 1/Z) dP_i/dR_A
+//    for A != B, and the owner receives minus their sum (w is invariant under
+//    rigid translation of atoms and point).
+// Becke derivatives: P_i = prod_j s(mu_ij), mu_ij = (r_i - r_j)/R~_ij,
+//   R~ = min(R_ij, 5) (capped) or R_ij, s = (1 - p(p(p(mu))))/2,
+//   dmu_ij/dR_i = -u_i/R~ - [R~ = R_ij] mu (R_i - R_j)/R_ij^2,
+//   dmu_ij/dR_j = +u_j/R~ + [R~ = R_ij] mu (R_i - R_j)/R_ij^2,
+//   u_i = (r - R_i)/r_i; s' = 0 where mu is clamped at +-1.
+#include "nkx_internal.hpp"
+#include "sto_eval.hpp"
+#include "vxc_kernels.hpp"
+
+namespace nkx {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+constexpr int FB = 64;
+constexpr int PT = 64;
+constexpr int TILE = FB * PT;
+constexpr int GC = FB * 3;  // per-slot partial: 64 functions x 3 components
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// One workgroup per batch.  Tile store arrays: Phi, dPhi/dx, dPhi/dy, dPhi/dz
+// (narr >= 4).  coef per point: (w v_rho / 2, b_x, b_y, b_z).
+__global__ void __launch_bounds__(256) grad_basis_f64(
+    int nbf_pad, const double *__restrict__ Pd, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
+    const double *__restrict__ tiles, const double *__restrict__ coef,
+    const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
+    const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, double *__restrict__ cpart, double *__restrict__ tpt) {
+  const int b = (int)(b0 + blockIdx.x);
+  const int lo = b_off[b], hi = b_off[b + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l16 = lane & 15, q = lane >> 4;
+
+  __shared__ double Pk[FB][FB];   // Pk[k][i] = P(mu = i, nu = k)
+  __shared__ double Ph[FB][PT];   // Phi of the K block
+  __shared__ double Yh[FB][PT];   // Y of the K block
+  __shared__ double cfs[4][PT];
+  __shared__ double red[4][3][PT];
+  __shared__ double tsum[3][PT];
+
+  for (int e = threadIdx.x; e < 4 * PT; e += 256) {
+    const int c = e / PT, g = e % PT;
+    cfs[c][g] = coef[4 * ((int64_t)b * PT + g) + c];
+  }
+  for (int e = threadIdx.x; e < 3 * PT; e += 256) (&tsum[0][0])[e] = 0.0;
+  __syncthreads();
+
+  for (int mi = lo; mi < hi; ++mi) {
+    const int fm = b_fbl[mi];
+    v4d accX[4], accW[4];
+    for (int j = 0; j < 4; ++j) accX[j] = accW[j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int ki = lo; ki < hi; ++ki) {
+      const int fk = b_fbl[ki];
+      const double *psrc = Pd + (int64_t)(fk * FB) * nbf_pad + fm * FB;
+      const double *tsrc = tiles + (ki - s0) * narr * TILE;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = 2 * (u * 256 + (int)threadIdx.x);
+        const int r = e >> 6, c = e & 63;
+        *reinterpret_cast<double2 *>(&Pk[r][c]) =
+            *reinterpret_cast<const double2 *>(psrc + (int64_t)r * nbf_pad + c);
+        const double2 ph = *reinterpret_cast<const double2 *>(tsrc + e);
+        *reinterpret_cast<double2 *>(&Ph[r][c]) = ph;
+        double2 y;
+        y.x = 2.0 * cfs[0][c] * ph.x;
+        y.y = 2.0 * cfs[0][c + 1] * ph.y;
+        if (gga) {
+#pragma unroll
+          for (int d = 1; d <= 3; ++d) {
+            const double2 gd = *reinterpret_cast<const double2 *>(tsrc + d * TILE + e);
+            y.x += cfs[d][c] * gd.x;
+            y.y += cfs[d][c + 1] * gd.y;
+          }
+        }
+        *reinterpret_cast<double2 *>(&Yh[r][c]) = y;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int k0 = 0; k0 < FB; k0 += 4) {
+        const double a = Pk[k0 + q][16 * wv + l16];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          accX[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ph[k0 + q][16 * j + l16], accX[j], 0,
+                                                          0, 0);
+          accW[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Yh[k0 + q][16 * j + l16], accW[j], 0,
+                                                          0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    // Epilogue: c_mu(g) for the 16 (mu, g) pairs of this lane.
+    const double *tb = tiles + (mi - s0) * narr * TILE;
+    double cm[4][3], tp[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      for (int k = 0; k < 3; ++k) cm[r][k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) tp[j][k] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mu = 16 * wv + 4 * r + q;
+      const FuncRec rec = F[fm * FB + mu];
+      if (rec.norm == 0.0) continue;  // padding function
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int g = 16 * j + l16;
+        const double X = accX[j][r], W = accW[j][r];
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = -2.0 * tb[(1 + k) * TILE + mu * PT + g] * W;
+        if (gga) {
+          const int64_t gp = (int64_t)b * PT + g;
+          double v, gr[3], h[6];
+          sto_value_grad_hess(rec, T, px[gp], py[gp], pz[gp], v, gr, h);
+          const double bx = cfs[1][g], by = cfs[2][g], bz = cfs[3][g];
+          c[0] -= 2.0 * (h[0] * bx + h[1] * by + h[2] * bz) * X;
+          c[1] -= 2.0 * (h[1] * bx + h[3] * by + h[4] * bz) * X;
+          c[2] -= 2.0 * (h[2] * bx + h[4] * by + h[5] * bz) * X;
+        }
+        for (int k = 0; k < 3; ++k) {
+          cm[r][k] += c[k];
+          tp[j][k] += c[k];
+        }
+      }
+    }
+    // per function: sum over the batch's points (j in registers, l16 lanes)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      for (int k = 0; k < 3; ++k) {
+        double v = cm[r][k];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        if (l16 == 0) cpart[(mi - s0) * GC + (16 * wv + 4 * r + q) * 3 + k] = v;
+      }
+    // per point: sum over this block's functions (q lanes, then waves)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) {
+        double v = tp[j][k];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (q == 0) red[wv][k][16 * j + l16] = v;
+      }
+    __syncthreads();
+    if (threadIdx.x < 3 * PT) {
+      const int k = threadIdx.x / PT, g = threadIdx.x % PT;
+      tsum[k][g] += red[0][k][g] + red[1][k][g] + red[2][k][g] + red[3][k][g];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3 * PT) {
+    const int k = threadIdx.x / PT, g = threadIdx.x % PT;
+    tpt[3 * ((int64_t)b * PT + g) + k] = -tsum[k][g];
+  }
+}
+
+// gacc[seg][I * 192 + t] += sum over batches of segment seg of cpart of block I
+__global__ void __launch_bounds__(256) grad_fold_kernel(int64_t nb, int64_t b0, int64_t b1,
+                                                        int64_t s0, int nseg, int nbf_pad,
+                                                        const int *__restrict__ tile_slot,
+                                                        const double *__restrict__ cpart,
+                                                        double *__restrict__ gacc) {
+  const int I = blockIdx.x, seg = blockIdx.y;
+  const int t = threadIdx.x;
+  if (t >= GC) return;
+  const int64_t n = b1 - b0;
+  const int64_t lo = b0 + n * seg / nseg, hi = b0 + n * (seg + 1) / nseg;
+  const int *ts = tile_slot + (int64_t)I * nb;
+  double acc = 0.0;
+  for (int64_t b = lo; b < hi; ++b) {
+    const int sl = ts[b];
+    if (sl >= 0) acc += cpart[(sl - s0) * GC + t];
+  }
+  gacc[(int64_t)seg * nbf_pad * 3 + I * GC + t] += acc;
+}
+
+__global__ void __launch_bounds__(256) owners_kernel(int64_t n, const int *__restrict__ pidx,
+                                                     const int *__restrict__ owner,
+                                                     int *__restrict__ pown) {
+  const int64_t sp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp >= n) return;
+  const int src = pidx[sp];
+  pown[sp] = src >= 0 ? owner[src] : -1;
+}
+
+__device__ __forceinline__ double bp(double x) { return 1.5 * x - 0.5 * x * x * x; }
+__device__ __forceinline__ double bdp(double x) { return 1.5 * (1.0 - x * x); }
+
+__device__ __forceinline__ double gdist(double x, double y, double z, const double *c) {
+  const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// Becke cell function P_i at (x, y, z); same screening and clamping as the
+// partition kernel.
+__device__ double becke_cell(int i, int natoms, const double *C, double x, double y, double z,
+                             double ri, bool cap) {
+  double Pi = 1.0;
+  for (int j = 0; j < natoms; ++j) {
+    if (j == i) continue;
+    const double rj = gdist(x, y, z, C + 3 * j);
+    if (cap && ri - rj <= -5.0) continue;  // mu <= -1: factor 1
+    double Rij = gdist(C[3 * i], C[3 * i + 1], C[3 * i + 2], C + 3 * j);
+    if (cap) Rij = fmin(Rij, 5.0);
+    double mu = (ri - rj) / Rij;
+    mu = mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
+    Pi *= 0.5 * (1.0 - bp(bp(bp(mu))));
+    if (Pi == 0.0) break;
+  }
+  return Pi;
+}
+
+// One thread per sorted point; per-atom sums in LDS (3 natoms doubles).
+__global__ void __launch_bounds__(256) grad_weights_kernel(
+    int64_t npts_pad, int natoms, const double *__restrict__ C, const double *__restrict__ px,
+    const double *__restrict__ py, const double *__restrict__ pz, const int *__restrict__ pown,
+    const double *__restrict__ epoint, const double *__restrict__ tpt, int partition,
+    double *__restrict__ wpart) {
+  extern __shared__ double acc[];
+  for (int e = threadIdx.x; e < 3 * natoms; e += blockDim.x) acc[e] = 0.0;
+  __syncthreads();
+  const bool cap = partition == 1;
+  for (int64_t sp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; sp < npts_pad;
+       sp += (int64_t)gridDim.x * blockDim.x) {
+    const int B = pown[sp];
+    if (B < 0 || B >= natoms) continue;  // padding (or an owner outside the atom list)
+    double S[3] = {-tpt[3 * sp], -tpt[3 * sp + 1], -tpt[3 * sp + 2]};  // owner gets -S
+    const double wf = epoint[sp];
+    if (partition != 0 && wf != 0.0) {
+      const double x = px[sp], y = py[sp], z = pz[sp];
+      double rmin = 1e300;
+      if (cap)
+        for (int i = 0; i < natoms; ++i) rmin = fmin(rmin, gdist(x, y, z, C + 3 * i));
+      double Z = 0.0, PB = 0.0;
+      for (int i = 0; i < natoms; ++i) {
+        const double ri = gdist(x, y, z, C + 3 * i);
+        if (cap && ri - rmin >= 5.0) continue;
+        const double Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
+        Z += Pi;
+        if (i == B) PB = Pi;
+      }
+      if (PB > 0.0 && Z > 0.0) {
+        for (int i = 0; i < natoms; ++i) {
+          const double ri = gdist(x, y, z, C + 3 * i);
+          if (cap && ri - rmin >= 5.0) continue;
+          const double Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
+          if (Pi == 0.0) continue;
+          const double coefi = wf * ((i == B ? 1.0 / PB : 0.0) - 1.0 / Z) * Pi;
+          const double *Ci = C + 3 * i;
+          const double ui[3] = {(x - Ci[0]) / ri, (y - Ci[1]) / ri, (z - Ci[2]) / ri};
+          double gi[3] = {0.0, 0.0, 0.0};
+          for (int j = 0; j < natoms; ++j) {
+            if (j == i) continue;
+            const double *Cj = C + 3 * j;
+            const double rj = gdist(x, y, z, Cj);
+            if (cap && ri - rj <= -5.0) continue;
+            const double d[3] = {Ci[0] - Cj[0], Ci[1] - Cj[1], Ci[2] - Cj[2]};
+            const double Rij = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const bool capped_here = cap && Rij >= 5.0;
+            const double Rt = capped_here ? 5.0 : Rij;
+            const double mu = (ri - rj) / Rt;
+            if (mu <= -1.0 || mu >= 1.0) continue;  // clamped: s' = 0
+            const double p1 = bp(mu), p2 = bp(p1), p3 = bp(p2);
+            const double sv = 0.5 * (1.0 - p3);
+            const double ds = -0.5 * bdp(p2) * bdp(p1) * bdp(mu);
+            const double f = coefi * ds / sv;  // s > 0 because P_i > 0
+            const double ct = capped_here ? 0.0 : mu / (Rij * Rij);
+            const double uj[3] = {(x - Cj[0]) / rj, (y - Cj[1]) / rj, (z - Cj[2]) / rj};
+            for (int k = 0; k < 3; ++k) {
+              gi[k] += f * (-ui[k] / Rt - ct * d[k]);
+              if (j != B) {
+                const double gj = f * (uj[k] / Rt + ct * d[k]);
+                atomicAdd(&acc[3 * j + k], gj);
+                S[k] += gj;
+              }
+            }
+          }
+          if (i != B)
+            for (int k = 0; k < 3; ++k) {
+              atomicAdd(&acc[3 * i + k], gi[k]);
+              S[k] += gi[k];
+            }
+        }
+      }
+    }
+    for (int k = 0; k < 3; ++k) atomicAdd(&acc[3 * B + k], -S[k]);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 3 * natoms; e += blockDim.x)
+    wpart[(int64_t)blockIdx.x * 3 * natoms + e] = acc[e];
+}
+
+__global__ void __launch_bounds__(256) colsum_kernel(int nrow, int ncol,
+                                                     const double *__restrict__ in,
+                                                     double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncol) return;
+  double v = 0.0;
+  for (int r = 0; r < nrow; ++r) v += in[(int64_t)r * ncol + k];
+  out[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+void vkg_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+               const double *Pd, int narr, int gga, const double *tiles, const double *coef,
+               double *cpart, double *tpt) {
+  if (b1 <= b0) return;
+  hipLaunchKernelGGL(grad_basis_f64, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, pl.nbf_pad, Pd,
+                     pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, coef, pl.funcs.p,
+                     pl.terms.p, pl.px.p, pl.py.p, pl.pz.p, cpart, tpt);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkg_fold(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nseg,
+              const double *cpart, double *gacc) {
+  if (b1 <= b0) return;
+  hipLaunchKernelGGL(grad_fold_kernel, dim3((unsigned)pl.nfb, (unsigned)nseg), dim3(256), 0, s,
+                     pl.nb, b0, b1, s0, nseg, pl.nbf_pad, pl.tile_slot.p, cpart, gacc);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkg_owners(hipStream_t s, const Plan &pl, const int *owner, int *pown) {
+  const int64_t n = pl.npts_pad;
+  hipLaunchKernelGGL(owners_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     pl.pidx.p, owner, pown);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *centers,
+                 const int *pown, const double *epoint, const double *tpt, int partition,
+                 int nblk, double *wpart) {
+  hipLaunchKernelGGL(grad_weights_kernel, dim3((unsigned)nblk), dim3(256),
+                     (size_t)3 * natoms * sizeof(double), s, pl.npts_pad, natoms, centers,
+                     pl.px.p, pl.py.p, pl.pz.p, pown, epoint, tpt, partition, wpart);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkg_colsum(hipStream_t s, int nrow, int ncol, const double *in, double *out) {
+  if (ncol <= 0) return;
+  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, nrow,
+                     ncol, in, out);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace nkx

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
     int nbf_pad, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
     double *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
-    double *__restrict__ coef, double *__restrict__ blockEN) {
+    double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
   const int b = (int)(b0 + blockIdx.x);
   const int lo = b_off[b], hi = b_off[b + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -218,6 +218,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
     }
     for (int s2 = 0; s2 < NSPIN; ++s2)
       for (int c = 0; c < 4; ++c) coef[4 * NSPIN * gp + 4 * s2 + c] = cf[s2][c][g];
+    if (epoint) epoint[gp] = e;  // w e_xc of this point
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       e += __shfl_down(e, o2);
       na += __shfl_down(na, o2);
@@ -402,16 +403,16 @@ void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, 
 
 void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
-                double *coef, double *blockEN) {
+                double *coef, double *blockEN, double *epoint) {
   if (b1 <= b0) return;
   if (nspin == 2)
     hipLaunchKernelGGL(density_fused_f64<2>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
                        pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN);
+                       xl, coef, blockEN, epoint);
   else
     hipLaunchKernelGGL(density_fused_f64<1>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
                        pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN);
+                       xl, coef, blockEN, epoint);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

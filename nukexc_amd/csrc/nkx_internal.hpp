@@ -137,6 +137,8 @@ struct Workspace {
   DevBuf<double> vpart;            // split-K partial V tiles
   DevBuf<double> red;              // small reduction outputs
   DevBuf<double> scratch;
+  DevBuf<double> gbuf;             // nuclear-gradient buffers
+  DevBuf<int> gint;
 };
 
 // Per-stage kernel timing with HIP events on the ctx stream
@@ -240,5 +242,7 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
 void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
                     int precision, double *E, double *N, double *const *V);
 void overlap_build(Ctx &c, double *S);
+void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, double *E,
+             double *grad);
 
 }  // namespace nkx

@@ -75,6 +75,52 @@ __device__ __forceinline__ void sto_value_grad(const FuncRec &f, const HarmTerm 
   gz = R * Sz + S * (dz * common);
 }
 
+// Value, gradient and Hessian (FP64), for the nuclear gradient of GGA
+// functionals.  With R(r) = N r^k e^{-zeta r}, R' = (k/r - zeta) R,
+// R'' = ((k/r - zeta)^2 - k/r^2) R and u = rvec / r:
+//   d_i d_j phi = R S_ij + (R'/r)(x_i S_j + x_j S_i)
+//                 + S [(R'' - R'/r) x_i x_j / r^2 + (R'/r) delta_ij].
+// h = (xx, xy, xz, yy, yz, zz).
+__device__ __forceinline__ void sto_value_grad_hess(const FuncRec &f, const HarmTerm *terms,
+                                                    double x, double y, double z, double &v,
+                                                    double g[3], double h[6]) {
+  const double d[3] = {x - f.cx, y - f.cy, z - f.cz};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double R = f.norm * ipow(r, f.k) * exp(-f.zeta * r);
+  double S = 0.0, Sg[3] = {0.0, 0.0, 0.0}, Sh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < f.term_cnt; ++t) {
+    const HarmTerm ht = terms[f.term_off + t];
+    const int e[3] = {ht.a, ht.b, ht.c};
+    double p0[3], p1[3], p2[3];  // d^0, d^1, d^2 of d_i^{e_i}
+    for (int i = 0; i < 3; ++i) {
+      p0[i] = ipow(d[i], e[i]);
+      p1[i] = e[i] > 0 ? e[i] * ipow(d[i], e[i] - 1) : 0.0;
+      p2[i] = e[i] > 1 ? e[i] * (e[i] - 1) * ipow(d[i], e[i] - 2) : 0.0;
+    }
+    const double c = ht.coef;
+    S += c * p0[0] * p0[1] * p0[2];
+    Sg[0] += c * p1[0] * p0[1] * p0[2];
+    Sg[1] += c * p0[0] * p1[1] * p0[2];
+    Sg[2] += c * p0[0] * p0[1] * p1[2];
+    Sh[0] += c * p2[0] * p0[1] * p0[2];
+    Sh[1] += c * p1[0] * p1[1] * p0[2];
+    Sh[2] += c * p1[0] * p0[1] * p1[2];
+    Sh[3] += c * p0[0] * p2[1] * p0[2];
+    Sh[4] += c * p0[0] * p1[1] * p1[2];
+    Sh[5] += c * p0[0] * p0[1] * p2[2];
+  }
+  const double q = f.k / r - f.zeta;
+  const double R1r = q * R / r;                                 // R' / r
+  const double A = ((q * q - f.k / (r * r)) * R - R1r) / (r * r);  // (R'' - R'/r) / r^2
+  v = R * S;
+  for (int i = 0; i < 3; ++i) g[i] = R * Sg[i] + S * d[i] * R1r;
+  const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+  for (int m = 0; m < 6; ++m) {
+    const int i = ii[m], j = jj[m];
+    h[m] = R * Sh[m] + R1r * (d[i] * Sg[j] + d[j] * Sg[i]) + S * (A * d[i] * d[j] + (i == j ? R1r : 0.0));
+  }
+}
+
 // Mixed-form collocation: values evaluated in FP64 and handed to the caller
 // in FP64 (they are split into three bf16 terms there, ~2^-27 relative);
 // gradients are rounded once to FP32.  Evaluating in FP32 instead (zeta r,

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
 
 #include "nkx_internal.hpp"
 #include "vxc_kernels.hpp"
@@ -374,7 +375,7 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
                   ws.blockEN.p);
     else
       vk_density(s, nspin, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
-                 ws.blockEN.p);
+                 ws.blockEN.p, nullptr);
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
     if (mixed)
@@ -402,6 +403,82 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
 void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *E,
                double *N, double *V) {
   vxc_build_spin(c, 1, &P, xl, gga, precision, E, N, &V);
+}
+
+// Nuclear gradient of E_xc at fixed P (kernels_grad.hip).  Every basis
+// function must sit on an atom of the grid; grad is host memory, 3 natoms.
+void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, double *E,
+             double *grad) {
+  ensure_plan(c);
+  Plan &pl = c.plan;
+  Workspace &ws = c.ws;
+  hipStream_t s = c.stream;
+  const int natoms = c.grid.natoms;
+  NKX_REQUIRE(natoms > 0 && natoms <= 2048, NKX_ERR_UNSUPPORTED,
+              "the gradient supports 1 .. 2048 atoms");
+  NKX_REQUIRE(partition >= 0 && partition <= 2, NKX_ERR_INVALID_ARGUMENT,
+              "partition: 0 fixed weights, 1 capped Becke, 2 Becke");
+  // function (sorted order) -> atom
+  std::vector<int> fatom(pl.nbf, -1);
+  for (int sp = 0; sp < pl.nbf; ++sp) {
+    const int i = pl.fperm[sp];
+    const double ctr[3] = {c.basis.cx[i], c.basis.cy[i], c.basis.cz[i]};
+    for (int a = 0; a < natoms && fatom[sp] < 0; ++a) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) d2 += std::pow(ctr[k] - c.grid.centers_host[3 * a + k], 2);
+      if (d2 <= 1e-20) fatom[sp] = a;
+    }
+    NKX_REQUIRE(fatom[sp] >= 0, NKX_ERR_INVALID_ARGUMENT,
+                "basis function " + std::to_string(i) + " is not centred on an atom of the grid");
+  }
+  const int narr = 5;  // Phi, grad Phi (3), Z
+  const size_t slot_bytes = (size_t)narr * FB * PT * sizeof(double);
+  // tiles plus the per-slot gradient partials
+  int64_t max_slots = chunk_slot_budget(slot_bytes + 3 * FB * sizeof(double));
+  for (int64_t b = 0; b < pl.nb; ++b)
+    max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
+  const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
+  const int nseg = 32;
+  const int nblk = 1024;
+  ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
+  ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+  ws.coef.reserve((size_t)4 * pl.npts_pad);
+  ws.blockEN.reserve((size_t)3 * pl.nb);
+  ws.red.reserve(3);
+  // gradient buffers: epoint, tpt, cpart, gacc, gfun, wpart, wsum
+  const size_t n_ep = pl.npts_pad, n_tp = 3 * pl.npts_pad, n_cp = (size_t)slots_alloc * 3 * FB;
+  const size_t n_ga = (size_t)nseg * pl.nbf_pad * 3, n_gf = (size_t)pl.nbf_pad * 3;
+  const size_t n_wp = (size_t)nblk * 3 * natoms, n_ws = (size_t)3 * natoms;
+  ws.gbuf.reserve(n_ep + n_tp + n_cp + n_ga + n_gf + n_wp + n_ws);
+  double *epoint = ws.gbuf.p, *tpt = epoint + n_ep, *cpart = tpt + n_tp, *gacc = cpart + n_cp,
+         *gfun = gacc + n_ga, *wpart = gfun + n_gf, *wsum = wpart + n_wp;
+  ws.gint.reserve(pl.npts_pad);
+
+  vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
+  NKX_HIP_CHECK(hipMemsetAsync(gacc, 0, n_ga * sizeof(double), s));
+  const int ig = gga ? 1 : 0;
+  for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
+    vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
+    vk_density(s, 1, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+               ws.blockEN.p, epoint);
+    vkg_basis(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, ws.coef.p, cpart, tpt);
+    vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
+  });
+  vkg_colsum(s, nseg, pl.nbf_pad * 3, gacc, gfun);
+  vk_en_reduce(s, pl.nb, ws.blockEN.p, ws.red.p);
+  vkg_owners(s, pl, c.grid.owner.p, ws.gint.p);
+  vkg_weights(s, pl, natoms, c.grid.centers.p, ws.gint.p, epoint, tpt, partition, nblk, wpart);
+  vkg_colsum(s, nblk, 3 * natoms, wpart, wsum);
+  std::vector<double> gf(n_gf), wsh(n_ws);
+  double en[3];
+  NKX_HIP_CHECK(hipMemcpyAsync(gf.data(), gfun, n_gf * sizeof(double), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipMemcpyAsync(wsh.data(), wsum, n_ws * sizeof(double), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipMemcpyAsync(en, ws.red.p, sizeof(en), hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
+  for (int k = 0; k < 3 * natoms; ++k) grad[k] = wsh[k];
+  for (int sp = 0; sp < pl.nbf; ++sp)
+    for (int k = 0; k < 3; ++k) grad[3 * fatom[sp] + k] += gf[3 * sp + k];
+  *E = en[0];
 }
 
 int Timing::begin(int stage, hipStream_t s) {

@@ -9,9 +9,10 @@ void vk_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const do
                  double *Pd);
 void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
                double *tiles);
+// epoint (may be NULL): w e_xc per sorted point
 void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
-                double *coef, double *blockEN);
+                double *coef, double *blockEN, double *epoint);
 void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr,
                  double *tiles);
 // nz Z arrays (the last nz of the narr per slot), partials for Z array k at
@@ -32,5 +33,25 @@ void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, c
                  double *blockEN);
 void vkm_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                int nsplit, int gga, const uint16_t *store, double *vpart);
+
+// nuclear gradient (kernels_grad.hip)
+// grad_basis: per batch, per-function partial sums of the basis-centre
+// derivative over the batch (cpart[(slot - s0) * 64 * 3 + mu * 3 + k]) and the
+// per-point sum tpt[3 * gp + k] with the sign of the point-motion term.
+void vkg_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+               const double *Pd, int narr, int gga, const double *tiles, const double *coef,
+               double *cpart, double *tpt);
+// fold the per-slot partials of batches [b0, b1) into gacc[seg][nbf_pad * 3]
+void vkg_fold(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nseg,
+              const double *cpart, double *gacc);
+// pown[sp] = owner of sorted point sp (-1 for padding)
+void vkg_owners(hipStream_t s, const Plan &pl, const int *owner, int *pown);
+// point-motion and Becke weight-derivative terms per atom; partition 0 = fixed
+// weights, 1 = capped Becke, 2 = Becke.  wpart[block][3 natoms], nblk blocks.
+void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *centers,
+                 const int *pown, const double *epoint, const double *tpt, int partition,
+                 int nblk, double *wpart);
+// out[k] = sum over rows r < nrow of in[r * ncol + k], fixed order
+void vkg_colsum(hipStream_t s, int nrow, int ncol, const double *in, double *out);
 
 }  // namespace nkx

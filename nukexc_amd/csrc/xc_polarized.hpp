@@ -148,8 +148,19 @@ __host__ __device__ inline D5 pbe_c_eps(D5 rho, D5 zeta, D5 opz, D5 omz, D5 sig_
   D5 kF = xcd::kKf * dpow(rho, 1.0 / 3.0);
   D5 t2 = sig_tot * kPi / (16.0 * phi * phi * kF * rho * rho);
   D5 A = bg / (dexp(-1.0 * ec / (gam * phi3)) - 1.0);
-  D5 u = A * t2;
-  D5 y = bg * t2 * (1.0 + u) / (1.0 + u + u * u);
+  // y = bg t2 (1 + u) / (1 + u + u^2), u = A t2, with its partial
+  // derivatives in closed form: dy/dt2 = bg (1 + 2u) / D^2 and
+  // dy/dA = -bg t2^2 u (2 + u) / D^2 (D = 1 + u + u^2).  Differentiating the
+  // quotient term by term cancels to a relative 1/u^2 where the enhancement
+  // saturates (low density, large gradients).
+  D5 y;
+  {
+    const double u = A.v * t2.v, D = 1.0 + u + u * u;
+    const double dt = bg * (1.0 + 2.0 * u) / (D * D);
+    const double dA = -bg * t2.v * t2.v * u * (2.0 + u) / (D * D);
+    y.v = bg * t2.v * (1.0 + u) / D;
+    for (int k = 0; k < 5; ++k) y.d[k] = dt * t2.d[k] + dA * A.d[k];
+  }
   D5 H = gam * phi3 * dlog(1.0 + y);
   return ec + H;
 }

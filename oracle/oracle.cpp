@@ -391,6 +391,22 @@ T pw92_pol(T rs, T z, T opz, T omz, bool mod) {
   return G0 - G2 * f * (1.0 - z4) / fpp0 + (G1 - G0) * f * z4;
 }
 
+// y = bg t2 (1 + u) / (1 + u + u^2), u = A t2.  For the complex step the
+// imaginary part is formed from the closed-form partials dy/dt2 = bg (1 + 2u)
+// / D^2 and dy/dA = -bg t2^2 u (2 + u) / D^2 (D = 1 + u + u^2): complex
+// arithmetic on the quotient would cancel to a relative 1/u^2 where the
+// enhancement saturates.
+inline double pbe_y(double t2, double A, double bg) {
+  const double u = A * t2;
+  return bg * t2 * (1.0 + u) / (1.0 + u + u * u);
+}
+inline cplx pbe_y(cplx t2, cplx A, double bg) {
+  const double t = t2.real(), a = A.real(), u = a * t, D = 1.0 + u + u * u;
+  const double dt = bg * (1.0 + 2.0 * u) / (D * D);
+  const double da = -bg * t * t * u * (2.0 + u) / (D * D);
+  return cplx(pbe_y(t, a, bg), dt * t2.imag() + da * A.imag());
+}
+
 template <typename T>
 T pbe_c_pol(T ra, T rb, T stot) {
   const double beta = 0.06672455060314922, gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
@@ -405,8 +421,7 @@ T pbe_c_pol(T ra, T rb, T stot) {
   T t2 = stot / (4.0 * phi * phi * ks2 * rho * rho);
   T g3 = gamma * phi * phi * phi;
   T A = beta / gamma / (std::exp(-ec / g3) - 1.0);
-  T u = A * t2;
-  T H = g3 * std::log(1.0 + beta / gamma * t2 * (1.0 + u) / (1.0 + u + u * u));
+  T H = g3 * std::log(1.0 + pbe_y(t2, A, beta / gamma));
   return rho * (ec + H);
 }
 

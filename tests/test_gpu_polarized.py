@@ -55,8 +55,13 @@ def test_functional_eval_polarized_matches_oracle(ctx, ids):
     assert de.max() < 1e-10 and dv.max() < 1e-9
     if gga:
         s1 = s1.cpu().numpy()
+        # where the PBE enhancement saturates (rho ~ 1e-8 with large
+        # gradients) vsigma is ~1e-19 and rounding noise in both codes; the
+        # last term measures a vsigma error by what it contributes to the
+        # energy density, |dv| sigma against 1e-9 |e_x|
+        st = (sig[:, 0] + sig[:, 2] + np.abs(sig[:, 1]))[:, None]
         ds = np.abs(s1 - s0) / (np.abs(s0) + 1e-3 * np.abs(sx[:, [0, 2]]).max(1, keepdims=True)
-                                + 1e-18)
+                                + 1e-9 * (np.abs(ex) * rt)[:, None] / (st + 1e-300) + 1e-300)
         print(f"{ids}: vsigma {ds.max():.1e}")
         assert ds.max() < 1e-9
 
