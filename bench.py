@@ -12,6 +12,7 @@ algorithmic flop count; cpu_baseline times the FP64 CPU restatement of the
 reference's dense algorithm on a bounded sample of the same workload.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -20,7 +21,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0}
+PEAK_TFLOPS = {"f64": 78.6, "i8": 5000.0, "bf16": 2500.0}
+# The mixed form runs each tile product as ten int8 slice products on MFMA.
+MIXED_OPS_PER_FLOP = 10
 PEAK_HBM_GBS = 8000.0
 
 
@@ -94,12 +97,16 @@ def cpu_baseline(w, grid, C, budget_s=15.0, chunk=16384):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="water100")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "mixed_bf16"])
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="skip the FP64 reference build of a mixed run (profiling)")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="timed FP64 builds for the mixed-vs-FP64 speedup (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (a known-size kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -114,22 +121,39 @@ def main():
     import torch
 
     if args.copy_calibration:
+        # one plain 16-byte vector copy kernel (nkx_copy_d2d) over a buffer
+        # twice the size of the MI355X last-level cache, after a fill of the
+        # source, so the counters see MIB MiB read from and written to HBM
+        import nukexc_amd as nk
+        from nukexc_amd import _capi
+        c = nk.Context(0)
         src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {args.copy_calibration} MiB read + written")
+        _capi.check(_capi.lib().nkx_copy_d2d(c.h, ctypes.c_void_p(dst.data_ptr()),
+                                             ctypes.c_void_p(src.data_ptr()),
+                                             args.copy_calibration << 20), c.h)
+        c.sync()
+        log(f"copy calibration: nkx_copy_d2d of {args.copy_calibration} MiB")
         del src, dst
+        c.close()
 
     w, ctx, grid, C, P = setup(args.workload)
     funcs = w["funcs"]
     npts = grid.weights.shape[0]
 
-    ref = None
-    if args.precision != "fp64":
+    ref, fp64_ms = None, None
+    if args.precision != "fp64" and not args.no_reference:
         ref = ctx.vxc(P, funcs, precision="fp64")
         log(f"FP64 reference build: E_xc = {ref.energy:.10f}")
+        if args.fp64_steps > 0:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.fp64_steps):
+                ctx.vxc(P, funcs, precision="fp64")
+            torch.cuda.synchronize()
+            fp64_ms = 1e3 * (time.perf_counter() - t0) / args.fp64_steps
+            log(f"FP64 form: {fp64_ms:.2f} ms/build over {args.fp64_steps} builds")
 
     for i in range(args.warmup):
         r = ctx.vxc(P, funcs, precision=args.precision)
@@ -139,8 +163,11 @@ def main():
     ctx.reset_timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    step_ms = []
     for i in range(args.steps):
-        r = ctx.vxc(P, funcs, precision=args.precision)
+        ts = time.perf_counter()
+        r = ctx.vxc(P, funcs, precision=args.precision)  # returns after E, N reach the host
+        step_ms.append(1e3 * (time.perf_counter() - ts))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
@@ -160,10 +187,15 @@ def main():
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None}
     else:
-        peak = PEAK_TFLOPS["f64" if args.precision == "fp64" else "bf16"]
+        peak = PEAK_TFLOPS[{"fp64": "f64", "mixed": "i8", "mixed_bf16": "bf16"}[args.precision]]
         achieved = stage_flops[dom] * args.steps / (dom_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
+        if args.precision != "fp64":
+            # algorithmic flops above; the MFMA executes ten int8 slice
+            # products (six bf16 products) per tile product
+            ops = MIXED_OPS_PER_FLOP if args.precision == "mixed" else 6
+            roof["mfma_issue_frac"] = achieved * ops / peak
     roof.update({"kernel": {"density": "density_fused", "vgemm": "vgemm",
                             "collocation": "colloc_tiles"}[dom],
                  "avg_launch_ms": dom_ms / max(dom_launches, 1),
@@ -190,13 +222,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "fp32+bf16x3 (f64 functional/reductions)",
+        "dtype": {"fp64": "f64", "mixed": "i8 slices x4 (exact i32 MFMA) + f64",
+                  "mixed_bf16": "bf16x3 split (f32 MFMA) + f64"}[args.precision],
         "data": "synthetic (seeded geometry, STO exponents and orbitals)",
         "config": {"workload": f"{w['name']}: {len(w['Z'])} atoms, {w['basis'].nbf()} STO "
                                f"functions, TA {w['grid']['nrad']} x Lebedev order "
                                f"{w['grid']['nang_order']}, {npts} points, "
                                f"{'PBE' if max(funcs) > 100 else 'LDA_X'}",
                    "precision": args.precision, "builds_per_s": 1e3 / ms_step,
+                   "median_ms_per_build": float(np.median(step_ms)),
                    "E_xc": r.energy, "N_el": r.n_el},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -204,6 +238,9 @@ def main():
     if ref is not None:
         out["config"]["dE_xc_vs_fp64"] = r.energy - ref.energy
         out["config"]["max_dV_vs_fp64"] = float((r.potential - ref.potential).abs().max())
+        if fp64_ms is not None:
+            out["config"]["fp64_ms_per_build"] = fp64_ms
+            out["config"]["speedup_vs_fp64"] = fp64_ms / ms_step
     print(json.dumps(out), flush=True)
     return 0
 

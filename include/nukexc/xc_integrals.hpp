@@ -113,7 +113,7 @@ inline XC_result_polarized compute_gga_lsda(
 // Batched build from a density matrix (row-major = LayoutLeft for symmetric
 // P): the per-SCF-iteration XC call, without materialising N_bf x N_pts
 // collocation matrices.
-enum class Precision { FP64 = NKX_PREC_FP64, Mixed = NKX_PREC_MIXED };
+enum class Precision { FP64 = NKX_PREC_FP64, Mixed = NKX_PREC_MIXED, MixedBF16 = NKX_PREC_MIXED_BF16 };
 
 namespace detail {
 // The grid is handed over once; repeated builds on it reuse the launch plan.

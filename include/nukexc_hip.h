@@ -49,9 +49,12 @@ typedef enum nkx_status {
 } nkx_status;
 
 typedef enum nkx_precision {
-  NKX_PREC_FP64 = 0,  /* FP64 collocation and FP64 MFMA contractions */
-  NKX_PREC_MIXED = 1  /* FP32 collocation, split-BF16 MFMA contractions with
-                         FP32 accumulation; FP64 functional and reductions */
+  NKX_PREC_FP64 = 0,       /* FP64 collocation and FP64 MFMA contractions */
+  NKX_PREC_MIXED = 1,      /* operands as four int8 slices with per-row power-of-two
+                              scales, exact int32 MFMA accumulation; FP64
+                              collocation arithmetic, functional and reductions */
+  NKX_PREC_MIXED_BF16 = 2  /* operands as three bf16 terms (split-BF16), six
+                              products per contraction on FP32-accumulating MFMA */
 } nkx_precision;
 
 typedef enum nkx_radial_scheme {
@@ -82,6 +85,10 @@ int nkx_free(nkx_ctx *ctx, void *ptr_dev);
 int nkx_memcpy_h2d(nkx_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int nkx_memcpy_d2h(nkx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 int nkx_memset(nkx_ctx *ctx, void *dst_dev, int value, size_t bytes);
+/* Stream-ordered device-to-device copy by a plain 16-byte vector load/store
+   kernel (no copy engine); pointers and size 16-byte aligned.  Also the
+   known-size kernel against which HBM traffic counters are calibrated. */
+int nkx_copy_d2d(nkx_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
 /* ------------------------------------------------------- angular / radial */
 /* Lebedev rule for the smallest tabulated algebraic order >= order
@@ -231,9 +238,6 @@ int nkx_xc_grad(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_id
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);
-/* Number of bf16 products per split contraction in the mixed form: 6 (default;
-   all terms down to O(2^-24)) or 3 (a0 b0 + a0 b1 + a1 b0). */
-int nkx_set_mixed_products(nkx_ctx *ctx, int nprod);
 /* Statistics of the current plan: number of points, padded points, batches,
    screened (function-block, batch) pairs kept, total pairs. */
 int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npts_padded, int64_t *nbatches,

@@ -17,6 +17,7 @@ STATUS_NAMES = {
 }
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_BF16 = 2
 RADIAL_TA = 0
 RADIAL_BECKE = 1
 FAMILY_LDA = 1
@@ -40,6 +41,7 @@ SIGNATURES = {
     "nkx_memcpy_h2d": [_vp, _vp, _vp, _c_size],
     "nkx_memcpy_d2h": [_vp, _vp, _vp, _c_size],
     "nkx_memset": [_vp, _vp, _c_int, _c_size],
+    "nkx_copy_d2d": [_vp, _vp, _vp, _c_size],
     "nkx_lebedev_npts": [_c_int, _pi, _pi],
     "nkx_lebedev_rule": [_c_int, _pd],
     "nkx_radial_rule": [_c_int, _c_int, _c_double, _c_double, _pd, _pd],
@@ -70,7 +72,6 @@ SIGNATURES = {
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_set_screening": [_vp, _c_double, _c_int],
-    "nkx_set_mixed_products": [_vp, _c_int],
     "nkx_plan_stats": [_vp, _pi64, _pi64, _pi64, _pi64, _pi64],
     "nkx_overlap": [_vp, _vp],
     "nkx_set_timing": [_vp, _c_int],

@@ -244,10 +244,6 @@ class Context:
     def set_screening(self, tol, batch_points=0):
         self._chk(lib().nkx_set_screening(self.h, float(tol), int(batch_points)))
 
-    def set_mixed_products(self, nprod):
-        """bf16 products per split contraction in the mixed form (3 or 6)."""
-        self._chk(lib().nkx_set_mixed_products(self.h, int(nprod)))
-
     def plan_stats(self):
         v = [ctypes.c_int64() for _ in range(5)]
         self._chk(lib().nkx_plan_stats(self.h, *[ctypes.byref(x) for x in v]))
@@ -258,7 +254,8 @@ class Context:
     def vxc(self, P, funcs=PBE, precision="fp64", out=None):
         """One V_xc build on the ctx grid and basis from a density matrix P.
         Returns XC_result(energy, potential, n_el)."""
-        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
+                "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
         P = _dev_f64(P, self.device).contiguous()
         nbf = P.shape[0]
         V = out if out is not None else torch.empty((nbf, nbf), dtype=torch.float64,

@@ -2,6 +2,7 @@
 // Every call converts internal errors to an nkx_status and records the
 // message on the ctx; no exception leaves this file.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -178,6 +179,16 @@ int nkx_memset(nkx_ctx *ctx, void *dst, int value, size_t bytes) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
     if (bytes) NKX_HIP_CHECK(hipMemsetAsync(dst, value, bytes, ctx->c.stream));
+  });
+}
+
+int nkx_copy_d2d(nkx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+                NKX_ERR_INVALID_ARGUMENT, "copy_d2d: 16-byte aligned pointers and size");
+    NKX_REQUIRE(bytes == 0 || (dst && src), NKX_ERR_INVALID_ARGUMENT, "copy_d2d: null pointer");
+    nkx::launch_copy16(ctx->c.stream, dst, src, bytes);
   });
 }
 
@@ -572,7 +583,8 @@ int nkx_vxc(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precis
     bool gga = false;
     NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
                 NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
-    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED ||
+                    precision == NKX_PREC_MIXED_BF16,
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
     NKX_REQUIRE(P && E && N && V, NKX_ERR_INVALID_ARGUMENT, "vxc: bad arguments");
     nkx::vxc_build(ctx->c, P, xl, gga, precision, E, N, V);
@@ -588,7 +600,8 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfun
     bool gga = false;
     NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
                 NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
-    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED ||
+                    precision == NKX_PREC_MIXED_BF16,
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
     NKX_REQUIRE(precision == NKX_PREC_FP64, NKX_ERR_UNSUPPORTED,
                 "the spin-polarised build is available in the FP64 form only");
@@ -627,14 +640,6 @@ int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
     NKX_REQUIRE(batch_points == 0 || batch_points == 64, NKX_ERR_UNSUPPORTED,
                 "batch size 64 is the only one built");
     ctx->c.opt.screen_tol = tol;
-  });
-}
-
-int nkx_set_mixed_products(nkx_ctx *ctx, int nprod) {
-  NKX_CTX_CHECK(ctx);
-  return guarded(ctx, [&] {
-    NKX_REQUIRE(nprod == 3 || nprod == 6, NKX_ERR_INVALID_ARGUMENT, "products must be 3 or 6");
-    ctx->c.opt.mixed_products = nprod;
   });
 }
 

@@ -1,4 +1,5 @@
 // Grid partition, compaction, reference-shaped collocation / density / XC.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -176,6 +177,25 @@ void launch_collocation_ref(hipStream_t s, int nbf, const FuncRec *f, const Harm
   if (n == 0) return;
   hipLaunchKernelGGL(colloc_ref_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nbf,
                      f, t, npts, pts, phi, gx, gy, gz);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Plain device copy with 16-byte vector loads and stores (grid-stride).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) copy16_kernel(int64_t n, const int4 *__restrict__ src,
+                                                     int4 *__restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy16(hipStream_t s, void *dst, const void *src, size_t bytes) {
+  const int64_t n = (int64_t)(bytes / 16);
+  if (n == 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 32);
+  hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n,
+                     reinterpret_cast<const int4 *>(src), reinterpret_cast<int4 *>(dst));
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -10,7 +10,7 @@
 //
 // Kernels per chunk:
 //   colloc_tiles     Phi and grad Phi for every live slot (HBM-write bound).
-//   density_fused    one workgroup per batch: X = P Phi on MFMA over the
+//   density_fused    one workgroup per 4 batches: X = P Phi on MFMA over the
 //                    batch's live function blocks, rho / grad rho epilogue in
 //                    FP64, the functional, then Z = w (v_rho/2 Phi + 2 v_sigma
 //                    grad rho . grad Phi) written back to the tile store.
@@ -72,132 +72,186 @@ __global__ void __launch_bounds__(256) colloc_tiles_f64(
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup per group of GB consecutive (Morton-adjacent) batches: every
+// 64 x 64 block of P is loaded into LDS once and applied to all batches of
+// the group in which both of its blocks are live, so P is streamed from the
+// cache hierarchy once per GB batches.  For each batch the K blocks are
+// visited in ascending order, as with one batch per workgroup, so every sum
+// is formed in the same order.
+//
 // NSPIN = 1: P is the total density matrix.  NSPIN = 2: Pd holds P_alpha then
 // P_beta (each nbf_pad^2); the functional is evaluated spin-polarised and the
 // tile store carries Z_alpha, Z_beta as its last two arrays.  blockEN holds
 // (E, N_alpha, N_beta) per batch (N_beta = 0 for NSPIN = 1, N_alpha = N).
+constexpr int GB = 4;          // batches per workgroup
+constexpr int MAXNFB = 1024;   // function blocks the union table holds
+
 template <int NSPIN>
 __global__ void __launch_bounds__(256) density_fused_f64(
-    int nbf_pad, const double *__restrict__ Pd, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
+    int nbf_pad, int nfb, const double *__restrict__ Pd, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int narr, int gga,
     double *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
-  const int b = (int)(b0 + blockIdx.x);
-  const int lo = b_off[b], hi = b_off[b + 1];
+  const int64_t bg = b0 + (int64_t)blockIdx.x * GB;
+  const int nsub = (int)(b1 - bg < GB ? b1 - bg : GB);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
 
-  __shared__ double Pk[FB][FB];          // Pk[k][i] = P(mu = i, nu = k) of the block pair
-  __shared__ double Ph[FB][PT];          // Phi tile of the K block
-  __shared__ double red[4][4][PT];       // per-wave column partials
-  __shared__ double colsum[NSPIN][4][PT];  // rho, d/dx, d/dy, d/dz column sums
-  __shared__ double cf[NSPIN][4][PT];      // Z coefficients
+  __shared__ double Pk[FB][FB];   // Pk[k][i] = P(mu = i, nu = k) of the block pair
+  __shared__ double Ph[FB][PT];   // Phi tile of the K block
+  __shared__ double red[4][4][PT];
+  __shared__ double colsum[NSPIN][GB][4][PT];  // rho, d/dx, d/dy, d/dz column sums
+  __shared__ double cf[NSPIN][GB][4][PT];      // Z coefficients
+  __shared__ int tab[GB][MAXNFB];              // slot of (sub-batch, block) or -1
+  __shared__ int ublk[MAXNFB];                 // union of live blocks, ascending
+  __shared__ int nu_s;
 
-  for (int e = threadIdx.x; e < NSPIN * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
+  for (int e = threadIdx.x; e < GB * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+  for (int e = threadIdx.x; e < NSPIN * GB * 4 * PT; e += 256) (&colsum[0][0][0][0])[e] = 0.0;
   __syncthreads();
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
+    for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
+  }
+  __syncthreads();
+  if (wv == 0) {  // ordered compaction of the union
+    int cnt = 0;
+    for (int base = 0; base < nfb; base += 64) {
+      const int f = base + lane;
+      bool live = false;
+      if (f < nfb)
+        for (int sb = 0; sb < GB; ++sb) live = live || tab[sb][f] >= 0;
+      const unsigned long long m = __ballot(live);
+      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nu_s = cnt;
+  }
+  __syncthreads();
+  const int nu = nu_s;
 
   const int nq = gga ? 4 : 1;
   for (int sp = 0; sp < NSPIN; ++sp) {
     const double *Psp = Pd + (int64_t)sp * nbf_pad * nbf_pad;
-    for (int mi = lo; mi < hi; ++mi) {
-      const int fm = b_fbl[mi];
-      v4d acc[4];
-      for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-      for (int ki = lo; ki < hi; ++ki) {
-        const int fk = b_fbl[ki];
+    for (int mi = 0; mi < nu; ++mi) {
+      const int fm = ublk[mi];
+      v4d acc[GB][4];
+#pragma unroll
+      for (int sb = 0; sb < GB; ++sb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[sb][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int ki = 0; ki < nu; ++ki) {
+        const int fk = ublk[ki];
+        bool any = false;
+        for (int sb = 0; sb < nsub; ++sb) any = any || (tab[sb][fk] >= 0 && tab[sb][fm] >= 0);
+        if (!any) continue;
         {
-          // P block: rows nu in block fk, cols mu in block fm (P symmetric), and
-          // the Phi tile of block fk; consecutive lanes move consecutive double2.
           const double *psrc = Psp + (int64_t)(fk * FB) * nbf_pad + fm * FB;
-          const double *tsrc = tiles + (ki - s0) * narr * TILE;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int e = 2 * (u * 256 + (int)threadIdx.x);
             const int r = e >> 6, c = e & 63;
             *reinterpret_cast<double2 *>(&Pk[r][c]) =
                 *reinterpret_cast<const double2 *>(psrc + (int64_t)r * nbf_pad + c);
-            *reinterpret_cast<double2 *>(&Ph[r][c]) =
-                *reinterpret_cast<const double2 *>(tsrc + e);
           }
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int k0 = 0; k0 < FB; k0 += 4) {
-          const double a = Pk[k0 + q][16 * wv + l16];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const double bv = Ph[k0 + q][16 * j + l16];
-            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[j], 0, 0, 0);
+        for (int sb = 0; sb < GB; ++sb) {
+          if (sb >= nsub) break;
+          const int slk = tab[sb][fk];
+          if (slk < 0 || tab[sb][fm] < 0) continue;
+          const double *tsrc = tiles + (slk - s0) * narr * TILE;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int e = 2 * (u * 256 + (int)threadIdx.x);
+            const int r = e >> 6, c = e & 63;
+            *reinterpret_cast<double2 *>(&Ph[r][c]) = *reinterpret_cast<const double2 *>(tsrc + e);
           }
+          __syncthreads();
+#pragma unroll 4
+          for (int k0 = 0; k0 < FB; k0 += 4) {
+            const double a = Pk[k0 + q][16 * wv + l16];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              acc[sb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ph[k0 + q][16 * j + l16],
+                                                               acc[sb][j], 0, 0, 0);
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
       // Epilogue: column partial sums sum_mu Phi(mu,g) X(mu,g) (and gradients).
-      const double *tb = tiles + (mi - s0) * narr * TILE;
-      double part[4][4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int g = 16 * j + l16;
-        double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int sb = 0; sb < GB; ++sb) {
+        if (sb >= nsub) break;
+        const int slm = tab[sb][fm];
+        if (slm < 0) continue;
+        const double *tb = tiles + (slm - s0) * narr * TILE;
+        double part[4][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 16 * wv + 4 * r + q;
-          const double x = acc[j][r];
-          sacc[0] += tb[mu * PT + g] * x;
-          if (gga) {
-            sacc[1] += tb[TILE + mu * PT + g] * x;
-            sacc[2] += tb[2 * TILE + mu * PT + g] * x;
-            sacc[3] += tb[3 * TILE + mu * PT + g] * x;
+        for (int j = 0; j < 4; ++j) {
+          const int g = 16 * j + l16;
+          double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mu = 16 * wv + 4 * r + q;
+            const double x = acc[sb][j][r];
+            sacc[0] += tb[mu * PT + g] * x;
+            if (gga) {
+              sacc[1] += tb[TILE + mu * PT + g] * x;
+              sacc[2] += tb[2 * TILE + mu * PT + g] * x;
+              sacc[3] += tb[3 * TILE + mu * PT + g] * x;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            double v = sacc[c];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            part[j][c] = v;
           }
         }
+        if (q == 0) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = sacc[c];
-          v += __shfl_xor(v, 16);
-          v += __shfl_xor(v, 32);
-          part[j][c] = v;
+          for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
         }
+        __syncthreads();
+        {
+          const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+          if (c < nq)
+            colsum[sp][sb][c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
+        }
+        __syncthreads();
       }
-      if (q == 0) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
-      }
-      __syncthreads();
-      {
-        const int c = threadIdx.x / PT, g = threadIdx.x % PT;
-        if (c < nq)
-          colsum[sp][c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
-      }
-      __syncthreads();
     }
   }
 
-  // Functional and Z coefficients, one thread per point (wave 0).
-  if (threadIdx.x < PT) {
-    const int g = threadIdx.x;
-    const int64_t gp = (int64_t)b * PT + g;
+  // Functional and Z coefficients: wave sb handles the 64 points of batch sb.
+  if (wv < nsub) {
+    const int sb = wv, g = lane;
+    const int64_t b = bg + sb;
+    const int64_t gp = b * PT + g;
     const double w = pw[gp];
     double e, na, nb = 0.0;
     if (NSPIN == 1) {
-      const double rho = colsum[0][0][g];
-      const double gxr = 2.0 * colsum[0][1][g], gyr = 2.0 * colsum[0][2][g],
-                   gzr = 2.0 * colsum[0][3][g];
+      const double rho = colsum[0][sb][0][g];
+      const double gxr = 2.0 * colsum[0][sb][1][g], gyr = 2.0 * colsum[0][sb][2][g],
+                   gzr = 2.0 * colsum[0][sb][3][g];
       const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
       const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
       const double a = 0.5 * w * o.vrho;
       const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
-      cf[0][0][g] = a;
-      cf[0][1][g] = bs * gxr;
-      cf[0][2][g] = bs * gyr;
-      cf[0][3][g] = bs * gzr;
+      cf[0][sb][0][g] = a;
+      cf[0][sb][1][g] = bs * gxr;
+      cf[0][sb][2][g] = bs * gyr;
+      cf[0][sb][3][g] = bs * gzr;
       e = w * o.e;
       na = w * rho;
     } else {
-      const double r2[2] = {colsum[0][0][g], colsum[NSPIN - 1][0][g]};
+      const double r2[2] = {colsum[0][sb][0][g], colsum[NSPIN - 1][sb][0][g]};
       double gr[2][3];
       for (int s2 = 0; s2 < 2; ++s2)
-        for (int c = 0; c < 3; ++c) gr[s2][c] = 2.0 * colsum[s2 & (NSPIN - 1)][c + 1][g];
+        for (int c = 0; c < 3; ++c) gr[s2][c] = 2.0 * colsum[s2 & (NSPIN - 1)][sb][c + 1][g];
       double sg[3] = {0.0, 0.0, 0.0};
       if (gga) {
         sg[0] = gr[0][0] * gr[0][0] + gr[0][1] * gr[0][1] + gr[0][2] * gr[0][2];
@@ -208,16 +262,16 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       xc_eval_polarized(xl.n, xl.ids, xl.thr, r2, sg, exc, vr, vs);
       for (int s2 = 0; s2 < 2; ++s2) {
         const double vss = gga ? 2.0 * w * vs[2 * s2] : 0.0, vab = gga ? w * vs[1] : 0.0;
-        cf[s2 & (NSPIN - 1)][0][g] = 0.5 * w * vr[s2];
+        cf[s2 & (NSPIN - 1)][sb][0][g] = 0.5 * w * vr[s2];
         for (int c = 0; c < 3; ++c)
-          cf[s2 & (NSPIN - 1)][c + 1][g] = vss * gr[s2][c] + vab * gr[1 - s2][c];
+          cf[s2 & (NSPIN - 1)][sb][c + 1][g] = vss * gr[s2][c] + vab * gr[1 - s2][c];
       }
       e = w * (r2[0] + r2[1]) * exc;
       na = w * r2[0];
       nb = w * r2[1];
     }
     for (int s2 = 0; s2 < NSPIN; ++s2)
-      for (int c = 0; c < 4; ++c) coef[4 * NSPIN * gp + 4 * s2 + c] = cf[s2][c][g];
+      for (int c = 0; c < 4; ++c) coef[4 * NSPIN * gp + 4 * s2 + c] = cf[s2][sb][c][g];
     if (epoint) epoint[gp] = e;  // w e_xc of this point
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       e += __shfl_down(e, o2);
@@ -233,17 +287,22 @@ __global__ void __launch_bounds__(256) density_fused_f64(
   __syncthreads();
 
   // Z tiles, Z_s in array narr - NSPIN + s.
-  for (int mi = lo; mi < hi; ++mi) {
-    double *tb = tiles + (mi - s0) * narr * TILE;
-    for (int sp = 0; sp < NSPIN; ++sp) {
-      double *zt = tb + (narr - NSPIN + sp) * TILE;
-      for (int e = threadIdx.x; e < TILE; e += 256) {
-        const int g = e % PT;
-        double z = cf[sp][0][g] * tb[e];
-        if (gga)
-          z += cf[sp][1][g] * tb[TILE + e] + cf[sp][2][g] * tb[2 * TILE + e] +
-               cf[sp][3][g] * tb[3 * TILE + e];
-        zt[e] = z;
+  for (int mi = 0; mi < nu; ++mi) {
+    const int fm = ublk[mi];
+    for (int sb = 0; sb < nsub; ++sb) {
+      const int slm = tab[sb][fm];
+      if (slm < 0) continue;
+      double *tb = tiles + (slm - s0) * narr * TILE;
+      for (int sp = 0; sp < NSPIN; ++sp) {
+        double *zt = tb + (narr - NSPIN + sp) * TILE;
+        for (int e = threadIdx.x; e < TILE; e += 256) {
+          const int g = e % PT;
+          double z = cf[sp][sb][0][g] * tb[e];
+          if (gga)
+            z += cf[sp][sb][1][g] * tb[TILE + e] + cf[sp][sb][2][g] * tb[2 * TILE + e] +
+                 cf[sp][sb][3][g] * tb[3 * TILE + e];
+          zt[e] = z;
+        }
       }
     }
   }
@@ -405,14 +464,17 @@ void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, co
                 const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
                 double *coef, double *blockEN, double *epoint) {
   if (b1 <= b0) return;
+  NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
+              "more than 1024 function blocks (65536 functions)");
+  const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
   if (nspin == 2)
-    hipLaunchKernelGGL(density_fused_f64<2>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
-                       pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN, epoint);
+    hipLaunchKernelGGL(density_fused_f64<2>, dim3(nwg), dim3(256), 0, s, pl.nbf_pad, pl.nfb, Pd,
+                       pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p, xl, coef,
+                       blockEN, epoint);
   else
-    hipLaunchKernelGGL(density_fused_f64<1>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s,
-                       pl.nbf_pad, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN, epoint);
+    hipLaunchKernelGGL(density_fused_f64<1>, dim3(nwg), dim3(256), 0, s, pl.nbf_pad, pl.nfb, Pd,
+                       pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p, xl, coef,
+                       blockEN, epoint);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -1,4 +1,5 @@
-// Batched V_xc build on MI355X (gfx950), mixed form.
+// Batched V_xc build on MI355X (gfx950), split-bf16 mixed form (selected
+// with NKX_PREC_MIXED_BF16; the default mixed form is kernels_vxc_i8.hip).
 //
 // Collocation values, Z and P are FP64 numbers split into three bf16 terms,
 // x ~ x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1),
@@ -118,7 +119,7 @@ __host__ __device__ inline MixedLayout mixed_layout(int gga) {
 }
 }  // namespace
 
-int64_t mixed_slot_bytes(int gga) { return mixed_layout(gga).slot_elems * 2; }
+int64_t bf16_slot_bytes(int gga) { return mixed_layout(gga).slot_elems * 2; }
 
 // ---------------------------------------------------------------------------
 // Collocation: one workgroup per live slot.  Values are computed in FP64,
@@ -417,7 +418,7 @@ __global__ void __launch_bounds__(256) vgemm_mixed(
 }
 
 // ---------------------------------------------------------------------------
-void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
+void vkb_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
                   uint16_t *Ps) {
   const int64_t n = (int64_t)nfb * FB * nfb * FB;
   hipLaunchKernelGGL(gather_p_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nbf,
@@ -425,7 +426,7 @@ void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const doubl
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
+void vkb_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
                 uint16_t *store) {
   if (s1 <= s0) return;
   hipLaunchKernelGGL(colloc_tiles_mixed, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
@@ -434,7 +435,7 @@ void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vkb_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const uint16_t *Ps, int gga, uint16_t *store, const XcList &xl, double *coef,
                  double *blockEN) {
   if (b1 <= b0) return;
@@ -449,7 +450,7 @@ void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, c
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vkm_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vkb_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                int nsplit, int gga, const uint16_t *store, double *vpart) {
   if (b1 <= b0 || pl.npairs == 0) return;
   if (nprod == 3)

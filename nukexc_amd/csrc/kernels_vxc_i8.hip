@@ -1,0 +1,544 @@
+// Batched V_xc build on MI355X (gfx950), mixed form: int8 slices with exact
+// integer MFMA accumulation.
+//
+// Every operand tile of the two contractions (P blocks, collocation values,
+// Z) is stored as four int8 slices with one power-of-two scale per tile row
+// (an Ozaki-type splitting; Ozaki, Ogita & Oishi, Numer. Algorithms 2012):
+//   x = 2^(e-6) (a0 + 2^-7 a1 + 2^-14 a2 + 2^-21 a3),   |a_s| <= 64,
+// e = the exponent of the row's largest |x| (|x| < 2^e), each slice rounded
+// to nearest from the exact FP64 remainder, so |x - sum| <= 2^(e-28) with
+// random sign.  A tile product is
+//   sum_k x_ik y_kj = 2^(eA_i + eB_j - 12) sum_d 2^(-7d) sum_(s+t=d) sum_k a_s,ik b_t,kj
+// and every inner sum runs on v_mfma_i32_16x16x64_i8 in exact int32
+// arithmetic (|sum| <= 4 * 64^3 < 2^31).  The ten slice products with
+// s + t <= 3 are kept; the dropped ones are below 2^-28 of the row-times-
+// column scale, of random sign.  The int32 sums of one tile product are
+// combined and scaled in FP64, per 64-function block (density) and per
+// 64-point batch (V).  The functional, rho, grad rho, E_xc, N_el and the V
+// accumulation are FP64; collocation gradients are FP32 (rounded once from
+// FP64).  Exact accumulation is the point of the integer form:
+// v_mfma_f32_*_bf16 rounds its FP32 sums relative to the largest term with a
+// mean pulled toward zero, a bias that shows up in N_el on large grids.
+//
+// Tile layout: a 64 x 64 tile T[row][k] is four row fragments of 16 rows;
+// lane l of fragment rt holds T[16 rt + l % 16][16 (l / 16) + e], e = 0..15,
+// as 16 consecutive bytes -- the A-operand order of v_mfma_i32_16x16x64_i8,
+// and, read as B[k][j] with j the row of T, its B-operand order.  A section
+// is the four slice planes (4 KB each) followed by the 64 row exponents.
+//
+// Per live slot (bytes):   F   Phi, rows = functions, k = points   16.25 KB
+//                          Pm  Phi, rows = points, k = functions   16.25 KB
+//                          Z   Z,   rows = functions, k = points   16.25 KB
+//                          G   dPhi/dx, dy, dz, FP32 [f][p]         48 KB (GGA)
+// P is gathered into one section per 64 x 64 block pair (rows mu, k = nu).
+#include "nkx_internal.hpp"
+#include "sto_eval.hpp"
+#include "vxc_kernels.hpp"
+#include "xc_functionals.hpp"
+
+namespace nkx {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// v_mfma_i32_16x16x64_i8 lane layout (checked by tests/gpu_probe): lane l
+// supplies A[l%16][16(l/16) .. +15] and B[16(l/16) .. +15][l%16]; result
+// register r of lane l holds D[4(l/16) + r][l%16].
+
+namespace {
+constexpr int FB = 64;
+constexpr int PT = 64;
+constexpr int TILE = FB * PT;
+constexpr int PLANE = TILE;                  // bytes per slice plane
+constexpr int SEC = 4 * PLANE + 256;         // section: 4 planes + exponents (padded)
+constexpr int OFF_F = 0, OFF_PM = SEC, OFF_Z = 2 * SEC, OFF_G = 3 * SEC;
+constexpr int GB = 4;
+constexpr int MAXNFB = 1024;
+
+__host__ __device__ inline int64_t slot_bytes_i8(int gga) {
+  return 3 * (int64_t)SEC + (gga ? 3 * (int64_t)TILE * 4 : 0);
+}
+
+// byte offset of T[row][k] inside one plane
+__device__ __forceinline__ int frag8(int row, int k) {
+  return (((row >> 4) * 64) + (row & 15) + 16 * (k >> 4)) * 16 + (k & 15);
+}
+
+// exponent e with max < 2^e; rows whose largest |x| is below 2^-120 are
+// stored as zero
+__device__ __forceinline__ int row_exp(double m) {
+  if (!(m >= 0x1p-120)) return -127;
+  int e;
+  (void)frexp(m, &e);
+  return e;
+}
+
+__device__ __forceinline__ void slice4(double x, int e, signed char s[4]) {
+  if (e == -127) {
+    s[0] = s[1] = s[2] = s[3] = 0;
+    return;
+  }
+  double y = ldexp(x, 6 - e);  // |y| < 64
+  double a = rint(y);
+  s[0] = (signed char)a;
+  double r = y - a;  // |r| <= 1/2, exact
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    y = r * 128.0;
+    a = rint(y);
+    s[i] = (signed char)a;
+    r = y - a;
+  }
+}
+
+__device__ __forceinline__ double unslice(const signed char *plane0, int off, int e) {
+  if (e == -127) return 0.0;
+  const double v = (double)plane0[off] + ldexp((double)plane0[PLANE + off], -7) +
+                   ldexp((double)plane0[2 * PLANE + off], -14) +
+                   ldexp((double)plane0[3 * PLANE + off], -21);
+  return ldexp(v, e - 6);
+}
+
+__device__ __forceinline__ v4i ld16(const signed char *p) {
+  return *reinterpret_cast<const v4i *>(p);
+}
+
+// The ten slice products of one 16 x 16 x 64 tile product, combined into the
+// FP64 value scaled by 2^(e - 19), e = eA + eB: hi = 128 d0 + d1,
+// lo = 128 d2 + d3, value = hi + 2^-14 lo.
+__device__ __forceinline__ void tile_product(const v4i *A, const v4i *B, int hi[4], int lo[4]) {
+  const v4i z = {0, 0, 0, 0};
+  v4i d0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[0], z, 0, 0, 0);
+  v4i d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[1], z, 0, 0, 0);
+  d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[0], d1, 0, 0, 0);
+  v4i d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[2], z, 0, 0, 0);
+  d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[1], d2, 0, 0, 0);
+  d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[0], d2, 0, 0, 0);
+  v4i d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[3], z, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[2], d3, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[1], d3, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[3], B[0], d3, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    hi[r] = d0[r] * 128 + d1[r];
+    lo[r] = d2[r] * 128 + d3[r];
+  }
+}
+
+__device__ __forceinline__ double combine(int hi, int lo, int e) {
+  return ldexp((double)hi + ldexp((double)lo, -14), e - 19);
+}
+
+// Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65) with
+// per-row exponents ex[row] into a section at dst (fragment order); one
+// thread per (row fragment, lane) item, 256 threads.
+__device__ void store_section(const double (*v)[65], const int *ex, signed char *dst,
+                              bool transpose) {
+  const int t = threadIdx.x;
+  const int rt = t >> 6, l = t & 63;
+  const int row = 16 * rt + (l & 15), k0 = 16 * (l >> 4);
+  __attribute__((aligned(16))) signed char out[4][16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const double x = transpose ? v[k0 + e][row] : v[row][k0 + e];
+    signed char s[4];
+    slice4(x, ex[row], s);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[p][e] = s[p];
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    *reinterpret_cast<v4i *>(dst + p * PLANE + t * 16) = *reinterpret_cast<const v4i *>(out[p]);
+  if (t < 64) dst[4 * PLANE + t] = (signed char)ex[t];
+}
+}  // namespace
+
+int64_t mixed_slot_bytes(int gga) { return slot_bytes_i8(gga); }
+
+// ---------------------------------------------------------------------------
+// Collocation: one workgroup per live slot; FP64 values in LDS, row maxima
+// along functions (F) and along points (Pm), slices in both orders.
+__global__ void __launch_bounds__(256) colloc_tiles_i8(
+    int64_t s0, const int *__restrict__ slot_b, const int *__restrict__ b_fbl,
+    const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
+    const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, int gga, signed char *__restrict__ store) {
+  const int64_t s = s0 + blockIdx.x;
+  const int b = slot_b[s];
+  const int fb = b_fbl[s];
+  __shared__ FuncRec fr[FB];
+  __shared__ double v[FB][65];
+  __shared__ int eF[FB], eP[PT];
+  if (threadIdx.x < FB) fr[threadIdx.x] = F[fb * FB + threadIdx.x];
+  __syncthreads();
+  const int g = threadIdx.x & (PT - 1);
+  const int row = threadIdx.x / PT;
+  const int64_t gp = (int64_t)b * PT + g;
+  const double x = px[gp], y = py[gp], z = pz[gp];
+  signed char *base = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
+  float *G = reinterpret_cast<float *>(base + OFF_G);
+#pragma unroll 1
+  for (int f = row; f < FB; f += 256 / PT) {
+    const FuncRec rec = fr[f];
+    double val;
+    if (gga) {
+      float a, bb, c;
+      sto_value_grad_mixed(rec, T, x, y, z, val, a, bb, c);
+      G[f * PT + g] = a;
+      G[TILE + f * PT + g] = bb;
+      G[2 * TILE + f * PT + g] = c;
+    } else {
+      val = sto_value_mixed(rec, T, x, y, z);
+    }
+    v[f][g] = val;
+  }
+  __syncthreads();
+  if (threadIdx.x < FB) {
+    double m = 0.0;
+    for (int k = 0; k < PT; ++k) m = fmax(m, fabs(v[threadIdx.x][k]));
+    eF[threadIdx.x] = row_exp(m);
+  } else if (threadIdx.x < FB + PT) {
+    const int gg = threadIdx.x - FB;
+    double m = 0.0;
+    for (int k = 0; k < FB; ++k) m = fmax(m, fabs(v[k][gg]));
+    eP[gg] = row_exp(m);
+  }
+  __syncthreads();
+  store_section(v, eF, base + OFF_F, false);
+  store_section(v, eP, base + OFF_PM, true);
+}
+
+// P (FP64, original order) -> one section per 64 x 64 block pair (fm, fk),
+// permuted and zero-padded; one workgroup per block pair.
+__global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
+                                                   const int *__restrict__ fperm,
+                                                   const double *__restrict__ P,
+                                                   signed char *__restrict__ Ps) {
+  const int fm = blockIdx.x / nfb, fk = blockIdx.x % nfb;
+  __shared__ double v[FB][65];
+  __shared__ int ex[FB];
+  for (int e = threadIdx.x; e < TILE; e += 256) {
+    const int r = e / FB, c = e % FB;
+    const int i = fm * FB + r, j = fk * FB + c;
+    v[r][c] = (i < nbf && j < nbf) ? P[(int64_t)fperm[i] * nbf + fperm[j]] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < FB) {
+    double m = 0.0;
+    for (int k = 0; k < FB; ++k) m = fmax(m, fabs(v[threadIdx.x][k]));
+    ex[threadIdx.x] = row_exp(m);
+  }
+  __syncthreads();
+  store_section(v, ex, Ps + (int64_t)blockIdx.x * SEC, false);
+}
+
+// ---------------------------------------------------------------------------
+// Density contraction, functional and Z.  One workgroup per GB batches; wave
+// w owns rows 16 w .. 16 w + 15 of the current M block; every P block
+// section is read once per group and applied to each batch of the group in
+// which both blocks are live (K blocks ascending per batch).
+__global__ void __launch_bounds__(256) density_fused_i8(
+    int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int gga,
+    signed char *__restrict__ store, const double *__restrict__ pw, XcList xl,
+    double *__restrict__ coef, double *__restrict__ blockEN) {
+  const int64_t bg = b0 + (int64_t)blockIdx.x * GB;
+  const int nsub = (int)(b1 - bg < GB ? b1 - bg : GB);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l16 = lane & 15, q = lane >> 4;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+
+  __shared__ double red[4][4][PT];
+  __shared__ double colsum[GB][4][PT];
+  __shared__ double cf[GB][4][PT];
+  __shared__ double zt[FB][65];
+  __shared__ int ez[FB];
+  __shared__ int tab[GB][MAXNFB];
+  __shared__ int ublk[MAXNFB];
+  __shared__ int nu_s;
+
+  for (int e = threadIdx.x; e < GB * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+  for (int e = threadIdx.x; e < GB * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
+  __syncthreads();
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
+    for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nfb; base += 64) {
+      const int f = base + lane;
+      bool live = false;
+      if (f < nfb)
+        for (int sb = 0; sb < GB; ++sb) live = live || tab[sb][f] >= 0;
+      const unsigned long long m = __ballot(live);
+      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nu_s = cnt;
+  }
+  __syncthreads();
+  const int nu = nu_s;
+  const int nq = gga ? 4 : 1;
+
+  for (int mi = 0; mi < nu; ++mi) {
+    const int fm = ublk[mi];
+    double x64[GB][4][4];
+#pragma unroll
+    for (int sb = 0; sb < GB; ++sb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x64[sb][j][r] = 0.0;
+    for (int ki = 0; ki < nu; ++ki) {
+      const int fk = ublk[ki];
+      bool any = false;
+      for (int sb = 0; sb < nsub; ++sb) any = any || (tab[sb][fk] >= 0 && tab[sb][fm] >= 0);
+      if (!any) continue;
+      const signed char *pa = Ps + ((int64_t)fm * nfb + fk) * SEC;
+      v4i A[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) A[p] = ld16(pa + p * PLANE + (wv * 64 + lane) * 16);
+      int eA[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) eA[r] = pa[4 * PLANE + 16 * wv + 4 * q + r];
+#pragma unroll
+      for (int sb = 0; sb < GB; ++sb) {
+        if (sb >= nsub) break;
+        const int slk = tab[sb][fk];
+        if (slk < 0 || tab[sb][fm] < 0) continue;
+        const signed char *pb = store + (slk - s0) * sb_bytes + OFF_PM;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v4i B[4];
+#pragma unroll
+          for (int p = 0; p < 4; ++p) B[p] = ld16(pb + p * PLANE + (j * 64 + lane) * 16);
+          const int eB = pb[4 * PLANE + 16 * j + l16];
+          int hi[4], lo[4];
+          tile_product(A, B, hi, lo);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (eA[r] == -127 || eB == -127) continue;
+            x64[sb][j][r] += combine(hi[r], lo[r], eA[r] + eB);
+          }
+        }
+      }
+    }
+    // Epilogue in FP64: column partial sums of Phi X (and grad Phi X).
+#pragma unroll
+    for (int sb = 0; sb < GB; ++sb) {
+      if (sb >= nsub) break;
+      const int slm = tab[sb][fm];
+      if (slm < 0) continue;
+      const signed char *tb = store + (slm - s0) * sb_bytes;
+      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+      double part[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int g = 16 * j + l16;
+        double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = 16 * wv + 4 * q + r;
+          const double x = x64[sb][j][r];
+          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+          if (gga) {
+            sm[1] += (double)G[mu * PT + g] * x;
+            sm[2] += (double)G[TILE + mu * PT + g] * x;
+            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = sm[c];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          part[j][c] = v;
+        }
+      }
+      if (q == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
+      }
+      __syncthreads();
+      {
+        const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+        if (c < nq)
+          colsum[sb][c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
+      }
+      __syncthreads();
+    }
+  }
+
+  // Functional and Z coefficients: wave sb handles the points of batch sb.
+  if (wv < nsub) {
+    const int sb = wv, g = lane;
+    const int64_t b = bg + sb;
+    const int64_t gp = b * PT + g;
+    const double w = pw[gp];
+    const double rho = colsum[sb][0][g];
+    const double gxr = 2.0 * colsum[sb][1][g], gyr = 2.0 * colsum[sb][2][g],
+                 gzr = 2.0 * colsum[sb][3][g];
+    const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
+    const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
+    const double a = 0.5 * w * o.vrho;
+    const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
+    cf[sb][0][g] = a;
+    cf[sb][1][g] = bs * gxr;
+    cf[sb][2][g] = bs * gyr;
+    cf[sb][3][g] = bs * gzr;
+    coef[4 * gp + 0] = a;
+    coef[4 * gp + 1] = bs * gxr;
+    coef[4 * gp + 2] = bs * gyr;
+    coef[4 * gp + 3] = bs * gzr;
+    double e = w * o.e, n = w * rho;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      e += __shfl_down(e, o2);
+      n += __shfl_down(n, o2);
+    }
+    if (g == 0) {  // (E, N_alpha, N_beta) per batch, N_beta = 0 here
+      blockEN[3 * b] = e;
+      blockEN[3 * b + 1] = n;
+      blockEN[3 * b + 2] = 0.0;
+    }
+  }
+  __syncthreads();
+
+  // Z = a Phi + b . grad Phi in FP64, per-function exponents, slices (F order).
+  for (int mi = 0; mi < nu; ++mi) {
+    const int fm = ublk[mi];
+    for (int sb = 0; sb < nsub; ++sb) {
+      const int slm = tab[sb][fm];
+      if (slm < 0) continue;
+      signed char *tb = store + (slm - s0) * sb_bytes;
+      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+      for (int e = threadIdx.x; e < TILE; e += 256) {
+        const int f = e / PT, g = e % PT;
+        double zz = cf[sb][0][g] * unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
+        if (gga)
+          zz += cf[sb][1][g] * (double)G[e] + cf[sb][2][g] * (double)G[TILE + e] +
+                cf[sb][3][g] * (double)G[2 * TILE + e];
+        zt[f][g] = zz;
+      }
+      __syncthreads();
+      if (threadIdx.x < FB) {
+        double m = 0.0;
+        for (int k = 0; k < PT; ++k) m = fmax(m, fabs(zt[threadIdx.x][k]));
+        ez[threadIdx.x] = row_exp(m);
+      }
+      __syncthreads();
+      store_section(zt, ez, tb + OFF_Z, false);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// acc(I, J) += Phi_I Z_J^T + Z_I Phi_J^T over batches where I and J are live.
+// Wave w owns rows 16 w .. 16 w + 15 of block I and all 64 columns of block J.
+__global__ void __launch_bounds__(256) vgemm_i8(
+    int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
+    const int *__restrict__ tile_slot, int64_t nb, int64_t b0, int64_t b1, int64_t s0,
+    int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
+  const int p = blockIdx.x % npairs;
+  const int split = blockIdx.x / npairs;
+  const int I = pair_I[p], J = pair_J[p];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l16 = lane & 15, q = lane >> 4;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+  double v64[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v64[j][r] = 0.0;
+  const int *tsI = tile_slot + (int64_t)I * nb;
+  const int *tsJ = tile_slot + (int64_t)J * nb;
+  for (int64_t b = b0 + split; b < b1; b += nsplit) {
+    const int sI = tsI[b], sJ = tsJ[b];
+    if (sI < 0 || sJ < 0) continue;
+    const signed char *tI = store + (sI - s0) * sb_bytes;
+    const signed char *tJ = store + (sJ - s0) * sb_bytes;
+    v4i AF[4], AZ[4];
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) {
+      AF[pl] = ld16(tI + OFF_F + pl * PLANE + (wv * 64 + lane) * 16);
+      AZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (wv * 64 + lane) * 16);
+    }
+    int eFI[4], eZI[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      eFI[r] = tI[OFF_F + 4 * PLANE + 16 * wv + 4 * q + r];
+      eZI[r] = tI[OFF_Z + 4 * PLANE + 16 * wv + 4 * q + r];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v4i BZ[4], BF[4];
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) {
+        BZ[pl] = ld16(tJ + OFF_Z + pl * PLANE + (j * 64 + lane) * 16);
+        BF[pl] = ld16(tJ + OFF_F + pl * PLANE + (j * 64 + lane) * 16);
+      }
+      const int eZJ = tJ[OFF_Z + 4 * PLANE + 16 * j + l16];
+      const int eFJ = tJ[OFF_F + 4 * PLANE + 16 * j + l16];
+      int h1[4], o1[4], h2[4], o2[4];
+      tile_product(AF, BZ, h1, o1);
+      tile_product(AZ, BF, h2, o2);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (eFI[r] != -127 && eZJ != -127) v64[j][r] += combine(h1[r], o1[r], eFI[r] + eZJ);
+        if (eZI[r] != -127 && eFJ != -127) v64[j][r] += combine(h2[r], o2[r], eZI[r] + eFJ);
+      }
+    }
+  }
+  double *out = vpart + ((int64_t)split * npairs + p) * (FB * FB);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mu = 16 * wv + 4 * q + r, nu = 16 * j + l16;
+      out[mu * FB + nu] += v64[j][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
+                  signed char *Ps) {
+  hipLaunchKernelGGL(gather_p_i8, dim3((unsigned)(nfb * nfb)), dim3(256), 0, s, nbf, nfb, fperm,
+                     P, Ps);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+int64_t mixed_p_bytes(int nfb) { return (int64_t)nfb * nfb * SEC; }
+
+void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
+                signed char *store) {
+  if (s1 <= s0) return;
+  hipLaunchKernelGGL(colloc_tiles_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+                     pl.slot_b.p, pl.b_fbl.p, pl.funcs.p, pl.terms.p, pl.px.p, pl.py.p, pl.pz.p,
+                     gga, store);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkm_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                 const signed char *Ps, int gga, signed char *store, const XcList &xl,
+                 double *coef, double *blockEN) {
+  if (b1 <= b0) return;
+  NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
+              "more than 1024 function blocks (65536 functions)");
+  const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
+  hipLaunchKernelGGL(density_fused_i8, dim3(nwg), dim3(256), 0, s, pl.nfb, Ps, pl.b_off.p,
+                     pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
+               int gga, const signed char *store, double *vpart) {
+  if (b1 <= b0 || pl.npairs == 0) return;
+  hipLaunchKernelGGL(vgemm_i8, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
+                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0, b1,
+                     s0, gga, store, vpart);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace nkx

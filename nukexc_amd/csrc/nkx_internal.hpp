@@ -124,14 +124,13 @@ struct Options {
   double screen_tol = 1e-12;
   int batch_points = 64;
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
-  int mixed_products = 6;       // bf16 products per split contraction (3 or 6)
 };
 
 struct Workspace {
   DevBuf<double> Pd;               // permuted padded density matrix (FP64)
-  DevBuf<uint16_t> Ps;             // three bf16 planes of the permuted P, fragment order
+  DevBuf<signed char> Ps;          // int8 slice sections of the permuted P
   DevBuf<double> tiles64;          // Phi, gx, gy, gz, Z tiles (FP64)
-  DevBuf<uint16_t> mstore;         // mixed-form tile store (bf16 planes + FP32 gradients)
+  DevBuf<signed char> mstore;      // mixed-form tile store (int8 slices + FP32 gradients)
   DevBuf<double> coef;             // per padded point and spin: a, bx, by, bz
   DevBuf<double> blockEN;          // per batch: E, N_alpha, N_beta partial sums
   DevBuf<double> vpart;            // split-K partial V tiles
@@ -216,6 +215,7 @@ void launch_functional_eval(hipStream_t s, XcList xl, int64_t npts, const double
                             const double *sigma, double *exc, double *vrho,
                             double *vsigma);
 
+void launch_copy16(hipStream_t s, void *dst, const void *src, size_t bytes);
 void launch_functional_eval_pol(hipStream_t s, XcList xl, int64_t npts, const double *rho,
                                 const double *sigma, double *exc, double *vrho,
                                 double *vsigma);

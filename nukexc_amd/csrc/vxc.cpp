@@ -324,12 +324,13 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   const int nval = gga ? 4 : 1;
   const int narr = nval + nspin;
   const int nsplit = choose_nsplit(pl);
-  const bool mixed = precision == NKX_PREC_MIXED;
+  const bool bf16 = precision == NKX_PREC_MIXED_BF16;
+  const bool mixed = precision == NKX_PREC_MIXED || bf16;
   NKX_REQUIRE(!mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
               "the spin-polarised build is available in the FP64 form only");
-  const int nprod = c.opt.mixed_products;
-  const size_t slot_bytes = mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
-                                  : (size_t)narr * FB * PT * sizeof(double);
+  const size_t slot_bytes = bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
+                            : mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
+                                    : (size_t)narr * FB * PT * sizeof(double);
   int64_t max_slots = chunk_slot_budget(slot_bytes);
   // at least the largest single batch
   for (int64_t b = 0; b < pl.nb; ++b)
@@ -337,8 +338,8 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   const size_t psz = (size_t)pl.nbf_pad * pl.nbf_pad;
   if (mixed) {
-    ws.mstore.reserve((size_t)slots_alloc * slot_bytes / sizeof(uint16_t));
-    ws.Ps.reserve((size_t)3 * pl.nbf_pad * pl.nbf_pad);
+    ws.mstore.reserve((size_t)slots_alloc * slot_bytes);
+    ws.Ps.reserve(bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)mixed_p_bytes(pl.nfb));
   } else {
     ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
     ws.Pd.reserve(nspin * psz);
@@ -352,8 +353,12 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   Timing &tm = c.timing;
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
+  uint16_t *bstore = reinterpret_cast<uint16_t *>(ws.mstore.p);
+  uint16_t *bPs = reinterpret_cast<uint16_t *>(ws.Ps.p);
   for (int sp = 0; sp < nspin; ++sp) {
-    if (mixed)
+    if (bf16)
+      vkb_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P[sp], bPs);
+    else if (mixed)
       vkm_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P[sp], ws.Ps.p);
     else
       vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P[sp], ws.Pd.p + sp * psz);
@@ -364,22 +369,27 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     const int ig = gga ? 1 : 0;
     int tt = tm.begin(kStageColloc, s);
-    if (mixed)
+    if (bf16)
+      vkb_colloc(s, s0, s1, pl, ig, bstore);
+    else if (mixed)
       vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
     else
       vk_colloc(s, s0, s1, pl, narr, ig, ws.tiles64.p);
     tm.end(tt, s);
     tt = tm.begin(kStageDensity, s);
-    if (mixed)
-      vkm_density(s, nprod, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
-                  ws.blockEN.p);
+    if (bf16)
+      vkb_density(s, 6, b0, b1, s0, pl, bPs, ig, bstore, xl, ws.coef.p, ws.blockEN.p);
+    else if (mixed)
+      vkm_density(s, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p, ws.blockEN.p);
     else
       vk_density(s, nspin, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
                  ws.blockEN.p, nullptr);
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
-    if (mixed)
-      vkm_vgemm(s, nprod, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p);
+    if (bf16)
+      vkb_vgemm(s, 6, b0, b1, s0, pl, nsplit, ig, bstore, ws.vpart.p);
+    else if (mixed)
+      vkm_vgemm(s, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p);
     else
       vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, nspin, ws.tiles64.p, ws.vpart.p);
     tm.end(tt, s);

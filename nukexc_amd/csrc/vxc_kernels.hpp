@@ -22,16 +22,31 @@ void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
 void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V);
 void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out);
 
-// mixed form (kernels_vxc_mixed.hip): pre-split bf16 planes in MFMA fragment
-// order; store holds mixed_slot_bytes(gga) per live slot
+// mixed form (kernels_vxc_i8.hip): int8 slices with per-row exponents,
+// exact integer MFMA; the store holds mixed_slot_bytes(gga) per live slot and
+// the gathered P mixed_p_bytes(nfb)
 int64_t mixed_slot_bytes(int gga);
+int64_t mixed_p_bytes(int nfb);
 void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
+                  signed char *Ps);
+void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
+                signed char *store);
+void vkm_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                 const signed char *Ps, int gga, signed char *store, const XcList &xl,
+                 double *coef, double *blockEN);
+void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
+               int gga, const signed char *store, double *vpart);
+
+// split-bf16 mixed form (kernels_vxc_bf16.hip): three bf16 planes per value,
+// six products; bf16_slot_bytes(gga) per slot, 3 nbf_pad^2 bf16 for P
+int64_t bf16_slot_bytes(int gga);
+void vkb_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const double *P,
                   uint16_t *Ps);
-void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga, uint16_t *store);
-void vkm_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vkb_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga, uint16_t *store);
+void vkb_density(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const uint16_t *Ps, int gga, uint16_t *store, const XcList &xl, double *coef,
                  double *blockEN);
-void vkm_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vkb_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                int nsplit, int gga, const uint16_t *store, double *vpart);
 
 // nuclear gradient (kernels_grad.hip)
