@@ -235,6 +235,23 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, 
    x, y, z) are written on return. */
 int nkx_xc_grad(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                 int partition, double *E_xc, double *grad_host);
+/* Molecule set (no counterpart in the reference, whose driver handles one
+   molecule): nmol molecules of different sizes, each with its own grid built
+   as nkx_make_flat_grid does (shared grid parameters) and its own basis.
+   natoms[k]; xyz (3 per atom) and Z concatenated over molecules; nbf[k]; the
+   basis arrays (as nkx_set_basis) concatenated.  npts_out (nmol, may be NULL)
+   receives each grid's size.  Replaces any previous set on the ctx. */
+int nkx_molset_create(nkx_ctx *ctx, int nmol, const int *natoms, const double *xyz,
+                      const int *Z, int nrad, int nang_order, int radial_scheme,
+                      double ta_alpha, double w_thr, int radial_sizing, const int *nbf,
+                      const int *n, const int *l, const int *m, const double *zeta,
+                      const double *norm, const double *centers, int64_t *npts_out);
+/* One V_xc build over every molecule of the set: all batches of all molecules
+   go through each kernel launch together (FP64 or NKX_PREC_MIXED).
+   P_dev[k] and V_dev[k] (host arrays of device pointers, nbf[k]^2 each),
+   E_xc[k], N_el[k] (may be NULL). */
+int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P_dev, int nfunc, const int *func_ids,
+                   int precision, double *E_xc, double *N_el, double *const *V_dev);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);

@@ -297,6 +297,52 @@ class Context:
                                     ctypes.byref(E), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return E.value, g.reshape(natoms, 3)
 
+    # -- molecule sets ---------------------------------------------------------
+    def molset_create(self, mols, nrad=75, nang_order=29, weight_threshold=1e-30,
+                      ta_alpha=TA_M4, radial="ta", sizing="uniform"):
+        """Grids and bases of many molecules for one-launch builds.  mols: list
+        of dicts with xyz (natoms, 3), Z, basis (STOBasisSet).  Returns the
+        grid size of each molecule."""
+        nmol = len(mols)
+        natoms = np.array([len(m["Z"]) for m in mols], np.int32)
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(m["xyz"], float).reshape(-1, 3)
+                                                   for m in mols]))
+        Z = np.ascontiguousarray(np.concatenate([np.asarray(m["Z"], np.int32) for m in mols]))
+        bs = [m["basis"] for m in mols]
+        nbf = np.array([b.nbf() for b in bs], np.int32)
+        cat = lambda f, t: np.ascontiguousarray(np.concatenate([np.asarray(f(b), t) for b in bs]))
+        n, l, mm = cat(lambda b: b.n, np.int32), cat(lambda b: b.l, np.int32), cat(lambda b: b.m, np.int32)
+        zeta, norm = cat(lambda b: b.zeta, np.float64), cat(lambda b: b.norm, np.float64)
+        ctr = cat(lambda b: np.asarray(b.O).reshape(-1, 3), np.float64)
+        npts = np.zeros(nmol, np.int64)
+        scheme = {"ta": _capi.RADIAL_TA, "becke": _capi.RADIAL_BECKE}[radial]
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._chk(lib().nkx_molset_create(
+            self.h, nmol, ip(natoms), dp(xyz), ip(Z), int(nrad), int(nang_order), scheme,
+            float(ta_alpha), float(weight_threshold), {"uniform": 0, "pyscf": 1}[sizing],
+            ip(nbf), ip(n), ip(l), ip(mm), dp(zeta), dp(norm), dp(ctr),
+            npts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        self._molset_nbf = [int(x) for x in nbf]
+        return npts
+
+    def molset_vxc(self, Ps, funcs=PBE, precision="fp64"):
+        """One build over all molecules of the set.  Ps: list of density
+        matrices.  Returns a list of XC_result."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        nmol = len(self._molset_nbf)
+        assert len(Ps) == nmol
+        Pd = [_dev_f64(P, self.device).contiguous() for P in Ps]
+        Vs = [torch.empty((k, k), dtype=torch.float64, device=self.device)
+              for k in self._molset_nbf]
+        Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Pd])
+        Vp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Vs])
+        E = (ctypes.c_double * nmol)()
+        N = (ctypes.c_double * nmol)()
+        ids, nf = _ids(funcs)
+        self._chk(lib().nkx_molset_vxc(self.h, Pp, nf, ids, prec, E, N, Vp))
+        return [XC_result(E[k], Vs[k], N[k]) for k in range(nmol)]
+
     STAGES = ("collocation", "density", "vgemm", "final", "build")
 
     def set_timing(self, on=True):

@@ -220,63 +220,155 @@ int nkx_radial_scale(int scheme, int Z, double *R) {
   });
 }
 
+namespace {
+// make_flat_grid (grid.hpp:172-348) into out: TA / Becke radial x Lebedev per
+// atom, capped Becke partition, compaction of w > w_thr.
+void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nrad,
+                int nang_order, int radial_scheme, double ta_alpha, double w_thr,
+                int radial_sizing, nkx::GridState &out) {
+  NKX_REQUIRE(natoms > 0 && xyz && Z && nrad > 0, NKX_ERR_INVALID_ARGUMENT,
+              "make_flat_grid: need atoms, coordinates, charges and nrad > 0");
+  NKX_REQUIRE(radial_sizing == 0 || radial_sizing == 1, NKX_ERR_INVALID_ARGUMENT,
+              "unknown radial sizing");
+  int nang = 0, actual = 0;
+  nkx::lebedev_lookup(nang_order, &nang, &actual);
+  std::vector<double> ang(4 * (size_t)nang);
+  nkx::lebedev_expand(actual, ang.data());
+  std::vector<double> P, W;
+  std::vector<int> O;
+  for (int a = 0; a < natoms; ++a) {
+    int nr = nrad;
+    if (radial_sizing == NKX_SIZING_PYSCF) {
+      static const double ratio[7] = {1.0, 1.5, 1.6, 1.8, 1.9, 2.0, 2.1};
+      nr = (int)(nrad * ratio[nkx::periodic_row(Z[a]) - 1] + 0.5);
+    }
+    std::vector<double> r(nr), wr(nr);
+    nkx::radial_rule(radial_scheme, nr, nkx::radial_scale(radial_scheme, Z[a]), ta_alpha,
+                     r.data(), wr.data());
+    for (int i = 0; i < nr; ++i)
+      for (int j = 0; j < nang; ++j) {
+        P.push_back(xyz[3 * a] + r[i] * ang[4 * j]);
+        P.push_back(xyz[3 * a + 1] + r[i] * ang[4 * j + 1]);
+        P.push_back(xyz[3 * a + 2] + r[i] * ang[4 * j + 2]);
+        const double ww = wr[i] * ang[4 * j + 3];
+        NKX_REQUIRE(ww >= 0, NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
+        W.push_back(ww);
+        O.push_back(a);
+      }
+  }
+  const int64_t n = (int64_t)W.size();
+  hipStream_t s = c.stream;
+  nkx::DevBuf<double> p0, w0;
+  nkx::DevBuf<int> o0;
+  nkx::DevBuf<int64_t> cnt;
+  p0.upload(P.data(), P.size(), s);
+  w0.upload(W.data(), W.size(), s);
+  o0.upload(O.data(), O.size(), s);
+  out.centers_host.assign(xyz, xyz + 3 * natoms);
+  out.centers.upload(xyz, 3 * (size_t)natoms, s);
+  out.natoms = natoms;
+  nkx::launch_becke(s, natoms, out.centers.p, n, p0.p, o0.p, w0.p, 1);
+  cnt.reserve((n + 1023) / 1024 + 1);
+  out.pts.reserve(3 * n);
+  out.w.reserve(n);
+  out.owner.reserve(n);
+  out.npts = nkx::compact_points(s, n, w_thr, p0.p, w0.p, o0.p, out.pts.p, out.w.p,
+                                 out.owner.p, nullptr, cnt.p);
+  out.version++;
+}
+
+nkx::BasisHost make_basis(int nbf, const int *n, const int *l, const int *m, const double *zeta,
+                          const double *norm, const double *centers, const double *cutoff) {
+  NKX_REQUIRE(nbf > 0 && n && l && m && zeta && norm && centers, NKX_ERR_INVALID_ARGUMENT,
+              "set_basis: bad arguments");
+  nkx::BasisHost B;
+  B.nbf = nbf;
+  for (int i = 0; i < nbf; ++i) {
+    NKX_REQUIRE(n[i] >= 1 && l[i] >= 0 && l[i] <= 7 && l[i] < n[i] && std::abs(m[i]) <= l[i] &&
+                    zeta[i] > 0,
+                NKX_ERR_INVALID_ARGUMENT,
+                "set_basis: need n >= 1, 0 <= l < n, l <= 7, |m| <= l, zeta > 0");
+    B.n.push_back(n[i]);
+    B.l.push_back(l[i]);
+    B.m.push_back(m[i]);
+    B.zeta.push_back(zeta[i]);
+    B.norm.push_back(norm[i]);
+    B.cx.push_back(centers[3 * i]);
+    B.cy.push_back(centers[3 * i + 1]);
+    B.cz.push_back(centers[3 * i + 2]);
+    B.cutoff.push_back(cutoff ? cutoff[i] : 0.0);
+    (void)nkx::solid_harmonic_terms(l[i], m[i]);
+  }
+  return B;
+}
+}  // namespace
+
 int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z, int nrad,
                        int nang_order, int radial_scheme, double ta_alpha, double w_thr,
                        int radial_sizing, int64_t *npts_out) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
+    build_grid(ctx->c, natoms, xyz, Z, nrad, nang_order, radial_scheme, ta_alpha, w_thr,
+               radial_sizing, ctx->c.grid);
+    if (npts_out) *npts_out = ctx->c.grid.npts;
+  });
+}
+
+int nkx_molset_create(nkx_ctx *ctx, int nmol, const int *natoms, const double *xyz,
+                      const int *Z, int nrad, int nang_order, int radial_scheme,
+                      double ta_alpha, double w_thr, int radial_sizing, const int *nbf,
+                      const int *n, const int *l, const int *m, const double *zeta,
+                      const double *norm, const double *centers, int64_t *npts_out) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
     nkx::Ctx &c = ctx->c;
-    NKX_REQUIRE(natoms > 0 && xyz && Z && nrad > 0, NKX_ERR_INVALID_ARGUMENT,
-                "make_flat_grid: need atoms, coordinates, charges and nrad > 0");
-    NKX_REQUIRE(radial_sizing == 0 || radial_sizing == 1, NKX_ERR_INVALID_ARGUMENT,
-                "unknown radial sizing");
-    int nang = 0, actual = 0;
-    nkx::lebedev_lookup(nang_order, &nang, &actual);
-    std::vector<double> ang(4 * (size_t)nang);
-    nkx::lebedev_expand(actual, ang.data());
-    std::vector<double> P, W;
-    std::vector<int> O;
-    for (int a = 0; a < natoms; ++a) {
-      int nr = nrad;
-      if (radial_sizing == NKX_SIZING_PYSCF) {
-        static const double ratio[7] = {1.0, 1.5, 1.6, 1.8, 1.9, 2.0, 2.1};
-        nr = (int)(nrad * ratio[nkx::periodic_row(Z[a]) - 1] + 0.5);
-      }
-      std::vector<double> r(nr), wr(nr);
-      nkx::radial_rule(radial_scheme, nr, nkx::radial_scale(radial_scheme, Z[a]), ta_alpha,
-                       r.data(), wr.data());
-      for (int i = 0; i < nr; ++i)
-        for (int j = 0; j < nang; ++j) {
-          P.push_back(xyz[3 * a] + r[i] * ang[4 * j]);
-          P.push_back(xyz[3 * a + 1] + r[i] * ang[4 * j + 1]);
-          P.push_back(xyz[3 * a + 2] + r[i] * ang[4 * j + 2]);
-          const double ww = wr[i] * ang[4 * j + 3];
-          NKX_REQUIRE(ww >= 0, NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
-          W.push_back(ww);
-          O.push_back(a);
-        }
+    NKX_REQUIRE(nmol > 0 && natoms && xyz && Z && nbf, NKX_ERR_INVALID_ARGUMENT,
+                "molset_create: bad arguments");
+    std::vector<std::vector<double>> pts(nmol), wts(nmol);
+    std::vector<nkx::BasisHost> bases;
+    size_t aoff = 0, foff = 0;
+    for (int k = 0; k < nmol; ++k) {
+      nkx::GridState g;
+      build_grid(c, natoms[k], xyz + 3 * aoff, Z + aoff, nrad, nang_order, radial_scheme,
+                 ta_alpha, w_thr, radial_sizing, g);
+      pts[k].resize(3 * (size_t)g.npts);
+      wts[k].resize(g.npts);
+      NKX_HIP_CHECK(hipMemcpyAsync(pts[k].data(), g.pts.p, pts[k].size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, c.stream));
+      NKX_HIP_CHECK(hipMemcpyAsync(wts[k].data(), g.w.p, wts[k].size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, c.stream));
+      NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+      NKX_REQUIRE(g.npts > 0, NKX_ERR_INVALID_ARGUMENT, "molset_create: a molecule has no points");
+      if (npts_out) npts_out[k] = g.npts;
+      bases.push_back(make_basis(nbf[k], n + foff, l + foff, m + foff, zeta + foff, norm + foff,
+                                 centers + 3 * foff, nullptr));
+      aoff += natoms[k];
+      foff += nbf[k];
     }
-    const int64_t n = (int64_t)W.size();
-    hipStream_t s = c.stream;
-    nkx::DevBuf<double> p0, w0;
-    nkx::DevBuf<int> o0;
-    nkx::DevBuf<int64_t> cnt;
-    p0.upload(P.data(), P.size(), s);
-    w0.upload(W.data(), W.size(), s);
-    o0.upload(O.data(), O.size(), s);
-    c.grid.centers_host.assign(xyz, xyz + 3 * natoms);
-    c.grid.centers.upload(xyz, 3 * (size_t)natoms, s);
-    c.grid.natoms = natoms;
-    nkx::launch_becke(s, natoms, c.grid.centers.p, n, p0.p, o0.p, w0.p, 1);
-    cnt.reserve((n + 1023) / 1024 + 1);
-    c.grid.pts.reserve(3 * n);
-    c.grid.w.reserve(n);
-    c.grid.owner.reserve(n);
-    const int64_t kept = nkx::compact_points(s, n, w_thr, p0.p, w0.p, o0.p, c.grid.pts.p,
-                                             c.grid.w.p, c.grid.owner.p, nullptr, cnt.p);
-    c.grid.npts = kept;
-    c.grid.version++;
-    if (npts_out) *npts_out = kept;
+    nkx::build_molset_plan(c, pts, wts, bases);
+  });
+}
+
+int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P, int nfunc, const int *ids,
+                   int precision, double *E, double *N, double *const *V) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(c.mplan.nmol > 0, NKX_ERR_STATE, "no molecule set: call nkx_molset_create");
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_UNSUPPORTED, "molecule sets build in the FP64 or the int8 mixed form");
+    NKX_REQUIRE(P && E && V, NKX_ERR_INVALID_ARGUMENT, "molset_vxc: bad arguments");
+    const int nmol = c.mplan.nmol;
+    for (int k = 0; k < nmol; ++k)
+      NKX_REQUIRE(P[k] && V[k], NKX_ERR_INVALID_ARGUMENT, "molset_vxc: null P or V");
+    std::vector<double> nel(nmol);
+    nkx::build_on_plan(c, c.mplan, 1, P, xl, gga, precision, E, nel.data(), V);
+    if (N)
+      for (int k = 0; k < nmol; ++k) N[k] = nel[k];
   });
 }
 
@@ -363,27 +455,7 @@ int nkx_set_basis(nkx_ctx *ctx, int nbf, const int *n, const int *l, const int *
                   const double *cutoff) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
-    NKX_REQUIRE(nbf > 0 && n && l && m && zeta && norm && centers, NKX_ERR_INVALID_ARGUMENT,
-                "set_basis: bad arguments");
-    nkx::BasisHost B;
-    B.nbf = nbf;
-    for (int i = 0; i < nbf; ++i) {
-      NKX_REQUIRE(n[i] >= 1 && l[i] >= 0 && l[i] <= 7 && l[i] < n[i] && std::abs(m[i]) <= l[i] &&
-                      zeta[i] > 0,
-                  NKX_ERR_INVALID_ARGUMENT,
-                  "set_basis: need n >= 1, 0 <= l < n, l <= 7, |m| <= l, zeta > 0");
-      B.n.push_back(n[i]);
-      B.l.push_back(l[i]);
-      B.m.push_back(m[i]);
-      B.zeta.push_back(zeta[i]);
-      B.norm.push_back(norm[i]);
-      B.cx.push_back(centers[3 * i]);
-      B.cy.push_back(centers[3 * i + 1]);
-      B.cz.push_back(centers[3 * i + 2]);
-      B.cutoff.push_back(cutoff ? cutoff[i] : 0.0);
-      (void)nkx::solid_harmonic_terms(l[i], m[i]);
-    }
-    ctx->c.basis = std::move(B);
+    ctx->c.basis = make_basis(nbf, n, l, m, zeta, norm, centers, cutoff);
     ctx->c.basis_version++;
   });
 }

@@ -19,6 +19,8 @@
 //                    both blocks are live (MFMA-bound).
 // and once per build: vreduce (split sum + un-permute + mirror) and the FP64
 // energy / electron-count reduction.
+#include <algorithm>
+
 #include "nkx_internal.hpp"
 #include "sto_eval.hpp"
 #include "vxc_kernels.hpp"
@@ -88,7 +90,7 @@ constexpr int MAXNFB = 1024;   // function blocks the union table holds
 
 template <int NSPIN>
 __global__ void __launch_bounds__(256) density_fused_f64(
-    int nbf_pad, int nfb, const double *__restrict__ Pd, const int *__restrict__ b_off,
+    MolTab mt, int64_t pspin, int nfb, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int narr, int gga,
     double *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
@@ -132,9 +134,11 @@ __global__ void __launch_bounds__(256) density_fused_f64(
 
   const int nq = gga ? 4 : 1;
   for (int sp = 0; sp < NSPIN; ++sp) {
-    const double *Psp = Pd + (int64_t)sp * nbf_pad * nbf_pad;
+    const double *Psp = Pd + (int64_t)sp * pspin;
     for (int mi = 0; mi < nu; ++mi) {
       const int fm = ublk[mi];
+      const int mol = mt.blk_mol[fm], ld = mt.ld[mol];
+      const double *Pmol = Psp + mt.P_off[mol] + mt.blk_local[fm] * FB;
       v4d acc[GB][4];
 #pragma unroll
       for (int sb = 0; sb < GB; ++sb)
@@ -146,13 +150,15 @@ __global__ void __launch_bounds__(256) density_fused_f64(
         for (int sb = 0; sb < nsub; ++sb) any = any || (tab[sb][fk] >= 0 && tab[sb][fm] >= 0);
         if (!any) continue;
         {
-          const double *psrc = Psp + (int64_t)(fk * FB) * nbf_pad + fm * FB;
+          // rows nu of block fk, columns mu of block fm (P symmetric); both
+          // blocks belong to molecule mol
+          const double *psrc = Pmol + (int64_t)(mt.blk_local[fk] * FB) * ld;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int e = 2 * (u * 256 + (int)threadIdx.x);
             const int r = e >> 6, c = e & 63;
             *reinterpret_cast<double2 *>(&Pk[r][c]) =
-                *reinterpret_cast<const double2 *>(psrc + (int64_t)r * nbf_pad + c);
+                *reinterpret_cast<const double2 *>(psrc + (int64_t)r * ld + c);
           }
         }
 #pragma unroll
@@ -317,7 +323,7 @@ constexpr int LDP = HP + 1; // padded row
 // zbase + zs, and the partials go to vpart + zs * nsplit * npairs * FB * FB.
 __global__ void __launch_bounds__(256) vgemm_f64(
     int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
-    const int *__restrict__ tile_slot, int64_t nb, int64_t b0, int64_t b1, int64_t s0,
+    const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1, int64_t s0,
     int narr, int zbase, const double *__restrict__ tiles, double *__restrict__ vpart) {
   const int p = blockIdx.x % npairs;
   const int split = (blockIdx.x / npairs) % nsplit;
@@ -330,9 +336,13 @@ __global__ void __launch_bounds__(256) vgemm_f64(
   __shared__ double A0[FB][LDP], A1[FB][LDP], B0[FB][LDP], B1[FB][LDP];
   v4d acc[4];
   for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-  const int *tsI = tile_slot + (int64_t)I * nb;
-  const int *tsJ = tile_slot + (int64_t)J * nb;
-  for (int64_t b = b0 + split; b < b1; b += nsplit) {
+  // batches of the pair's molecule inside [b0, b1)
+  const int mol = mt.blk_mol[I];
+  const int64_t mb0 = mt.b0[mol], mnb = mt.nb[mol];
+  const int *tsI = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[I] * mnb - mb0;
+  const int *tsJ = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[J] * mnb - mb0;
+  const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
+  for (int64_t b = lo + split; b < hi; b += nsplit) {
     const int sI = tsI[b], sJ = tsJ[b];
     if (sI < 0 || sJ < 0) continue;
     const double *tI = tiles + (sI - s0) * narr * TILE;
@@ -382,33 +392,41 @@ __global__ void __launch_bounds__(256) vgemm_f64(
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) vreduce_kernel(int npairs, int nsplit, int nbf,
+__global__ void __launch_bounds__(256) vreduce_kernel(int npairs, int nsplit,
                                                       const int *__restrict__ pair_I,
                                                       const int *__restrict__ pair_J,
-                                                      const int *__restrict__ fperm,
+                                                      const int *__restrict__ fperm, MolTab mt,
+                                                      const int *__restrict__ mol_nbf,
                                                       const double *__restrict__ vpart,
-                                                      double *__restrict__ V) {
+                                                      double *const *__restrict__ Vmol) {
   const int p = blockIdx.x;
   const int I = pair_I[p], J = pair_J[p];
+  const int mol = mt.blk_mol[I];
+  const int nbf = mol_nbf[mol];
+  double *V = Vmol[mol];
+  const int lI = mt.blk_local[I], lJ = mt.blk_local[J];
   for (int e = threadIdx.x; e < FB * FB; e += 256) {
     const int mu = e / FB, nu = e % FB;
     if (I == J && mu > nu) continue;
-    const int i = I * FB + mu, j = J * FB + nu;
+    const int i = lI * FB + mu, j = lJ * FB + nu;
     if (i >= nbf || j >= nbf) continue;
     double v = 0.0;
     for (int s = 0; s < nsplit; ++s) v += vpart[((int64_t)s * npairs + p) * (FB * FB) + e];
-    const int oi = fperm[i], oj = fperm[j];
+    const int oi = fperm[I * FB + mu], oj = fperm[J * FB + nu];
     V[(int64_t)oi * nbf + oj] = v;
     V[(int64_t)oj * nbf + oi] = v;
   }
 }
 
-// Fixed-order sum of the per-batch (E, N_alpha, N_beta) triples.
-__global__ void __launch_bounds__(256) en_reduce_kernel(int64_t nb, const double *__restrict__ en,
+// Fixed-order sum of the per-batch (E, N_alpha, N_beta) triples, one
+// workgroup per molecule over its batches.
+__global__ void __launch_bounds__(256) en_reduce_kernel(MolTab mt, const double *__restrict__ en,
                                                         double *__restrict__ out) {
   __shared__ double sv[3][256];
+  const int mol = blockIdx.x;
+  const int64_t b0 = mt.b0[mol], b1 = b0 + mt.nb[mol];
   double v[3] = {0.0, 0.0, 0.0};
-  for (int64_t b = threadIdx.x; b < nb; b += 256)
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += 256)
     for (int c = 0; c < 3; ++c) v[c] += en[3 * b + c];
   for (int c = 0; c < 3; ++c) sv[c][threadIdx.x] = v[c];
   __syncthreads();
@@ -418,7 +436,14 @@ __global__ void __launch_bounds__(256) en_reduce_kernel(int64_t nb, const double
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int c = 0; c < 3; ++c) out[c] = sv[c][0];
+    for (int c = 0; c < 3; ++c) out[3 * mol + c] = sv[c][0];
+}
+
+// Zero fill (stream-ordered kernel, also valid inside a captured graph).
+__global__ void __launch_bounds__(256) fill0_kernel(int64_t n, double *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
 }
 
 __global__ void __launch_bounds__(256) gather_p_f64(int nbf, int nbf_pad,
@@ -468,13 +493,13 @@ void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, co
               "more than 1024 function blocks (65536 functions)");
   const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
   if (nspin == 2)
-    hipLaunchKernelGGL(density_fused_f64<2>, dim3(nwg), dim3(256), 0, s, pl.nbf_pad, pl.nfb, Pd,
-                       pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p, xl, coef,
-                       blockEN, epoint);
+    hipLaunchKernelGGL(density_fused_f64<2>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.P_total,
+                       pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p,
+                       xl, coef, blockEN, epoint);
   else
-    hipLaunchKernelGGL(density_fused_f64<1>, dim3(nwg), dim3(256), 0, s, pl.nbf_pad, pl.nfb, Pd,
-                       pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p, xl, coef,
-                       blockEN, epoint);
+    hipLaunchKernelGGL(density_fused_f64<1>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.P_total,
+                       pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p,
+                       xl, coef, blockEN, epoint);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
@@ -490,20 +515,29 @@ void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
               int narr, int nz, const double *tiles, double *vpart) {
   if (b1 <= b0 || pl.npairs == 0) return;
   hipLaunchKernelGGL(vgemm_f64, dim3((unsigned)(pl.npairs * nsplit * nz)), dim3(256), 0, s,
-                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0, b1,
+                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.tab(), b0, b1,
                      s0, narr, narr - nz, tiles, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V) {
+void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart,
+                double *const *Vmol_dev, const int *mol_nbf_dev) {
   if (pl.npairs == 0) return;
-  hipLaunchKernelGGL(vreduce_kernel, dim3(pl.npairs), dim3(256), 0, s, pl.npairs, nsplit, pl.nbf,
-                     pl.pair_I.p, pl.pair_J.p, pl.fperm_d.p, vpart, V);
+  hipLaunchKernelGGL(vreduce_kernel, dim3(pl.npairs), dim3(256), 0, s, pl.npairs, nsplit,
+                     pl.pair_I.p, pl.pair_J.p, pl.fperm_d.p, pl.tab(), mol_nbf_dev, vpart,
+                     Vmol_dev);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out) {
-  hipLaunchKernelGGL(en_reduce_kernel, dim3(1), dim3(256), 0, s, nb, en, out);
+void vk_en_reduce(hipStream_t s, const Plan &pl, const double *en, double *out) {
+  hipLaunchKernelGGL(en_reduce_kernel, dim3(pl.nmol), dim3(256), 0, s, pl.tab(), en, out);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vk_fill0(hipStream_t s, double *x, int64_t n) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill0_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n, x);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 // section is read once per group and applied to each batch of the group in
 // which both blocks are live (K blocks ascending per batch).
 __global__ void __launch_bounds__(256) density_fused_i8(
-    int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int gga,
     signed char *__restrict__ store, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN) {
@@ -283,6 +283,9 @@ __global__ void __launch_bounds__(256) density_fused_i8(
 
   for (int mi = 0; mi < nu; ++mi) {
     const int fm = ublk[mi];
+    const int mol = mt.blk_mol[fm];
+    const signed char *Pmol =
+        Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
     double x64[GB][4][4];
 #pragma unroll
     for (int sb = 0; sb < GB; ++sb)
@@ -295,7 +298,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
       bool any = false;
       for (int sb = 0; sb < nsub; ++sb) any = any || (tab[sb][fk] >= 0 && tab[sb][fm] >= 0);
       if (!any) continue;
-      const signed char *pa = Ps + ((int64_t)fm * nfb + fk) * SEC;
+      const signed char *pa = Pmol + (int64_t)mt.blk_local[fk] * SEC;  // same molecule
       v4i A[4];
 #pragma unroll
       for (int p = 0; p < 4; ++p) A[p] = ld16(pa + p * PLANE + (wv * 64 + lane) * 16);
@@ -439,7 +442,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
 // Wave w owns rows 16 w .. 16 w + 15 of block I and all 64 columns of block J.
 __global__ void __launch_bounds__(256) vgemm_i8(
     int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
-    const int *__restrict__ tile_slot, int64_t nb, int64_t b0, int64_t b1, int64_t s0,
+    const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1, int64_t s0,
     int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
   const int p = blockIdx.x % npairs;
   const int split = blockIdx.x / npairs;
@@ -452,9 +455,13 @@ __global__ void __launch_bounds__(256) vgemm_i8(
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) v64[j][r] = 0.0;
-  const int *tsI = tile_slot + (int64_t)I * nb;
-  const int *tsJ = tile_slot + (int64_t)J * nb;
-  for (int64_t b = b0 + split; b < b1; b += nsplit) {
+  // batches of the pair's molecule inside [b0, b1)
+  const int mol = mt.blk_mol[I];
+  const int64_t mb0 = mt.b0[mol], mnb = mt.nb[mol];
+  const int *tsI = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[I] * mnb - mb0;
+  const int *tsJ = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[J] * mnb - mb0;
+  const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
+  for (int64_t b = lo + split; b < hi; b += nsplit) {
     const int sI = tsI[b], sJ = tsJ[b];
     if (sI < 0 || sJ < 0) continue;
     const signed char *tI = store + (sI - s0) * sb_bytes;
@@ -527,8 +534,8 @@ void vkm_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
   const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
-  hipLaunchKernelGGL(density_fused_i8, dim3(nwg), dim3(256), 0, s, pl.nfb, Ps, pl.b_off.p,
-                     pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
+  hipLaunchKernelGGL(density_fused_i8, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.nfb, Ps,
+                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
@@ -536,8 +543,8 @@ void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl
                int gga, const signed char *store, double *vpart) {
   if (b1 <= b0 || pl.npairs == 0) return;
   hipLaunchKernelGGL(vgemm_i8, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
-                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.nb, b0, b1,
-                     s0, gga, store, vpart);
+                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.tab(), b0,
+                     b1, s0, gga, store, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

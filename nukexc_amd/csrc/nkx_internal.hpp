@@ -92,32 +92,57 @@ struct GridState {
   uint64_t version = 0;              // bumped on every change
 };
 
+// Per-molecule tables of a plan, passed to kernels by value.  Blocks and
+// batches of all molecules are numbered globally (molecule by molecule);
+// a block pair and a batch's live blocks always belong to one molecule.
+struct MolTab {
+  const int *blk_mol, *blk_local;  // per global block: molecule, block index in it
+  const int *ld;                   // per molecule: padded function count (P row stride)
+  const int *nfb;                  // per molecule: function blocks
+  const int64_t *b0, *nb;          // per molecule: first global batch, batch count
+  const int64_t *ts_off;           // per molecule: offset of its tile_slot table [nfb][nb]
+  const int64_t *P_off;            // per molecule: offset of its gathered FP64 P
+  const int64_t *Ps_off;           // per molecule: byte offset of its int8 P sections
+};
+
 // Launch plan of the batched V_xc build; rebuilt when grid, basis or the
-// screening options change.
+// screening options change.  A plan holds one molecule (the ctx grid and
+// basis) or a molecule set.
 struct Plan {
   uint64_t grid_version = ~0ull, basis_version = ~0ull;
   double screen_tol = -1;
   int pt = 0;                 // points per batch
   int fb = 64;                // functions per function block
-  int nbf = 0, nbf_pad = 0, nfb = 0;
+  int nbf = 0, nbf_pad = 0, nfb = 0;       // totals over molecules
   int64_t npts = 0, npts_pad = 0, nb = 0;  // nb = number of point batches
-  std::vector<int> fperm;     // sorted position -> original function index
+  std::vector<int> fperm;     // global sorted position -> original function index in its molecule
   DevBuf<int> fperm_d;
   DevBuf<FuncRec> funcs;      // nbf_pad records in sorted order (padding: norm 0)
   DevBuf<HarmTerm> terms;
   DevBuf<double> px, py, pz, pw;  // sorted, padded point SoA
-  DevBuf<int> pidx;               // sorted position -> original point (-1: pad)
+  DevBuf<int> pidx;               // sorted position -> original point in its molecule (-1: pad)
   // batch -> active function blocks (CSR)
   std::vector<int> b_off_h;
   DevBuf<int> b_off, b_fbl, slot_b;  // slot s = CSR position; slot_b[s] = its batch
-  // function-block pair (I <= J) -> batches where both are active (CSR)
+  // function-block pairs (I <= J, same molecule) with a common live batch
   int npairs = 0;
   std::vector<int> pair_I_h, pair_J_h;
-  DevBuf<int> pair_I, pair_J, pair_off, pair_bl;
-  DevBuf<int> tile_slot;          // (fb, batch) -> slot in the tile store or -1
+  DevBuf<int> pair_I, pair_J;
+  DevBuf<int> tile_slot;          // per molecule [block][batch] -> slot in the tile store or -1
   int64_t nslots = 0;
   int64_t pairs_kept = 0;
   double work_nb2g = 0, work_nbg = 0;  // sum_b n_b^2 g_b, sum_b n_b g_b
+  // molecules
+  int nmol = 0;
+  std::vector<int> mol_nbf_h, mol_ld_h, mol_nfb_h, mol_blk0_h;
+  std::vector<int64_t> mol_b0_h, mol_nb_h, mol_npts_h, mol_P_off_h, mol_Ps_off_h;
+  int64_t P_total = 0;            // sum of ld^2 (one spin of gathered FP64 P)
+  DevBuf<int> blk_mol, blk_local, mol_ld, mol_nfb, mol_nbf;
+  DevBuf<int64_t> mol_b0, mol_nb, mol_ts_off, mol_P_off, mol_Ps_off;
+  MolTab tab() const {
+    return MolTab{blk_mol.p, blk_local.p, mol_ld.p, mol_nfb.p, mol_b0.p,
+                  mol_nb.p,  mol_ts_off.p, mol_P_off.p, mol_Ps_off.p};
+  }
 };
 
 struct Options {
@@ -138,6 +163,8 @@ struct Workspace {
   DevBuf<double> scratch;
   DevBuf<double> gbuf;             // nuclear-gradient buffers
   DevBuf<int> gint;
+  std::vector<double *> vptr_h;    // output pointer table of a build (host copy)
+  DevBuf<double *> vptr;
 };
 
 // Per-stage kernel timing with HIP events on the ctx stream
@@ -178,6 +205,7 @@ struct Ctx {
   BasisHost basis;
   uint64_t basis_version = 0;
   Plan plan;
+  Plan mplan;                    // plan of the molecule set (nkx_molset_create)
   Workspace ws;
 };
 
@@ -241,6 +269,11 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
                double *E, double *N, double *V);
 void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
                     int precision, double *E, double *N, double *const *V);
+void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
+                   bool gga, int precision, double *E, double *N, double *const *V);
+void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
+                       const std::vector<std::vector<double>> &mol_w,
+                       const std::vector<BasisHost> &bases);
 void overlap_build(Ctx &c, double *S);
 void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, double *E,
              double *grad);

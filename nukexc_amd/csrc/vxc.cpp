@@ -68,24 +68,29 @@ constexpr int PT = 64;
 
 }  // namespace
 
-void ensure_plan(Ctx &c) {
-  Plan &pl = c.plan;
-  NKX_REQUIRE(c.grid.npts > 0, NKX_ERR_STATE, "no grid: call nkx_make_flat_grid or nkx_set_grid");
-  NKX_REQUIRE(c.basis.nbf > 0, NKX_ERR_STATE, "no basis: call nkx_set_basis");
-  if (pl.grid_version == c.grid.version && pl.basis_version == c.basis_version &&
-      pl.screen_tol == c.opt.screen_tol && pl.pt == PT)
-    return;
-  hipStream_t s = c.stream;
-  const BasisHost &B = c.basis;
-  const int64_t npts = c.grid.npts;
+namespace {
 
+// Plan of one molecule in local numbering (blocks, batches, slots from 0).
+struct PlanPart {
+  int nbf = 0, nfb = 0;
+  int64_t npts = 0, nb = 0;
+  std::vector<int> fperm;  // sorted -> original function
+  std::vector<FuncRec> fr;
+  std::vector<HarmTerm> terms;
+  std::vector<double> px, py, pz, pw;
+  std::vector<int> pidx;
+  std::vector<int> b_off, b_fbl, slot_b, tile_slot;
+  std::vector<int> pair_I, pair_J;
+  double nb2g = 0, nbg = 0;
+};
+
+// Points P (3 npts) and weights W of one molecule, its basis B.
+PlanPart make_part(const std::vector<double> &P, const std::vector<double> &W,
+                   const BasisHost &B, double tol) {
+  PlanPart pt;
+  const int64_t npts = (int64_t)W.size();
+  pt.npts = npts;
   // ---- points ------------------------------------------------------------
-  std::vector<double> P(3 * npts), W(npts);
-  NKX_HIP_CHECK(hipMemcpyAsync(P.data(), c.grid.pts.p, 3 * npts * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  NKX_HIP_CHECK(hipMemcpyAsync(W.data(), c.grid.w.p, npts * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  NKX_HIP_CHECK(hipStreamSynchronize(s));
   Box bb{{1e300, 1e300, 1e300}, {-1e300, -1e300, -1e300}};
   for (int64_t g = 0; g < npts; ++g)
     for (int d = 0; d < 3; ++d) {
@@ -97,8 +102,12 @@ void ensure_plan(Ctx &c) {
   std::sort(key.begin(), key.end());
   const int64_t nb = (npts + PT - 1) / PT;
   const int64_t npad = nb * PT;
-  std::vector<double> px(npad), py(npad), pz(npad), pw(npad);
-  std::vector<int> pidx(npad);
+  pt.nb = nb;
+  pt.px.resize(npad);
+  pt.py.resize(npad);
+  pt.pz.resize(npad);
+  pt.pw.resize(npad);
+  pt.pidx.resize(npad);
   std::vector<Box> boxes(nb);
   for (int64_t b = 0; b < nb; ++b) {
     Box &x = boxes[b];
@@ -109,11 +118,11 @@ void ensure_plan(Ctx &c) {
     for (int i = 0; i < PT; ++i) {
       const int64_t sp = b * PT + i;
       const int64_t src = sp < npts ? key[sp].second : key[npts - 1].second;
-      px[sp] = P[3 * src];
-      py[sp] = P[3 * src + 1];
-      pz[sp] = P[3 * src + 2];
-      pw[sp] = sp < npts ? W[src] : 0.0;
-      pidx[sp] = sp < npts ? (int)src : -1;
+      pt.px[sp] = P[3 * src];
+      pt.py[sp] = P[3 * src + 1];
+      pt.pz[sp] = P[3 * src + 2];
+      pt.pw[sp] = sp < npts ? W[src] : 0.0;
+      pt.pidx[sp] = sp < npts ? (int)src : -1;
       for (int d = 0; d < 3; ++d) {
         x.lo[d] = std::min(x.lo[d], P[3 * src + d]);
         x.hi[d] = std::max(x.hi[d], P[3 * src + d]);
@@ -125,17 +134,19 @@ void ensure_plan(Ctx &c) {
   const int nbf = B.nbf;
   const int nfb = (nbf + FB - 1) / FB;
   const int nbf_pad = nfb * FB;
+  pt.nbf = nbf;
+  pt.nfb = nfb;
   std::vector<double> rad(nbf);
   Box cb{{1e300, 1e300, 1e300}, {-1e300, -1e300, -1e300}};
   for (int i = 0; i < nbf; ++i) {
     double nrm, cut;
-    sto_norm_cutoff(B.n[i], B.zeta[i], c.opt.screen_tol, &nrm, &cut);
+    sto_norm_cutoff(B.n[i], B.zeta[i], tol, &nrm, &cut);
     // Same radial bound with the stored normalisation (callers may scale it).
     if (B.norm[i] != nrm && B.norm[i] > 0) {
       double r = cut;
       for (int it = 0; it < 60; ++it) {
         const double v = std::log(B.norm[i]) + (B.n[i] - 1) * std::log(r) - B.zeta[i] * r -
-                         std::log(c.opt.screen_tol);
+                         std::log(tol);
         if (v <= 0) break;
         r *= 1.25;
       }
@@ -155,15 +166,14 @@ void ensure_plan(Ctx &c) {
     fkey[i] = std::make_tuple(cls, morton(ctr, cb), i);
   }
   std::sort(fkey.begin(), fkey.end());
-  pl.fperm.resize(nbf);
-  for (int i = 0; i < nbf; ++i) pl.fperm[i] = std::get<2>(fkey[i]);
+  pt.fperm.resize(nbf);
+  for (int i = 0; i < nbf; ++i) pt.fperm[i] = std::get<2>(fkey[i]);
 
-  std::vector<FuncRec> fr(nbf_pad);
-  std::vector<HarmTerm> terms;
+  pt.fr.resize(nbf_pad);
   for (int sp = 0; sp < nbf_pad; ++sp) {
     FuncRec r{};
     if (sp < nbf) {
-      const int i = pl.fperm[sp];
+      const int i = pt.fperm[sp];
       const auto &t = solid_harmonic_terms(B.l[i], B.m[i]);
       r.cx = B.cx[i];
       r.cy = B.cy[i];
@@ -171,76 +181,163 @@ void ensure_plan(Ctx &c) {
       r.zeta = B.zeta[i];
       r.norm = B.norm[i];
       r.k = B.n[i] - B.l[i] - 1;
-      r.term_off = (int)terms.size();
+      r.term_off = (int)pt.terms.size();
       r.term_cnt = (int)t.size();
-      terms.insert(terms.end(), t.begin(), t.end());
+      pt.terms.insert(pt.terms.end(), t.begin(), t.end());
     } else {
       r.zeta = 1.0;
       r.norm = 0.0;
       r.term_off = 0;
       r.term_cnt = 0;
     }
-    fr[sp] = r;
+    pt.fr[sp] = r;
   }
-  if (terms.empty()) terms.push_back(HarmTerm{0, 0, 0, 0, 0});
 
   // ---- screening: live (block, batch) pairs -------------------------------
-  std::vector<int> b_off(nb + 1, 0), b_fbl, slot_b;
-  b_fbl.reserve((size_t)nb * 4);
+  pt.b_off.assign(nb + 1, 0);
+  pt.b_fbl.reserve((size_t)nb * 4);
   for (int64_t b = 0; b < nb; ++b) {
-    b_off[b] = (int)b_fbl.size();
+    pt.b_off[b] = (int)pt.b_fbl.size();
     for (int f = 0; f < nfb; ++f) {
       bool live = false;
       for (int k = f * FB; k < std::min(nbf, (f + 1) * FB) && !live; ++k) {
-        const int i = pl.fperm[k];
+        const int i = pt.fperm[k];
         live = box_dist2(boxes[b], B.cx[i], B.cy[i], B.cz[i]) < rad[i] * rad[i];
       }
       if (live) {
-        b_fbl.push_back(f);
-        slot_b.push_back((int)b);
+        pt.b_fbl.push_back(f);
+        pt.slot_b.push_back((int)b);
       }
     }
-    NKX_REQUIRE(b_fbl.size() < (size_t)INT32_MAX, NKX_ERR_UNSUPPORTED,
+    NKX_REQUIRE(pt.b_fbl.size() < (size_t)INT32_MAX, NKX_ERR_UNSUPPORTED,
                 "too many live (block, batch) pairs for 32-bit slot numbers");
   }
-  b_off[nb] = (int)b_fbl.size();
+  pt.b_off[nb] = (int)pt.b_fbl.size();
   // algorithmic work: true functions of the live blocks, true points
-  {
-    double nb2g = 0, nbg = 0;
-    for (int64_t b = 0; b < nb; ++b) {
-      double nfun = 0;
-      for (int k = b_off[b]; k < b_off[b + 1]; ++k)
-        nfun += std::min(nbf, (b_fbl[k] + 1) * FB) - b_fbl[k] * FB;
-      const double g = (double)std::min<int64_t>(PT, npts - b * PT);
-      nb2g += nfun * nfun * g;
-      nbg += nfun * g;
-    }
-    pl.work_nb2g = nb2g;
-    pl.work_nbg = nbg;
+  for (int64_t b = 0; b < nb; ++b) {
+    double nfun = 0;
+    for (int k = pt.b_off[b]; k < pt.b_off[b + 1]; ++k)
+      nfun += std::min(nbf, (pt.b_fbl[k] + 1) * FB) - pt.b_fbl[k] * FB;
+    const double g = (double)std::min<int64_t>(PT, npts - b * PT);
+    pt.nb2g += nfun * nfun * g;
+    pt.nbg += nfun * g;
   }
+  pt.tile_slot.assign((size_t)nfb * nb, -1);
+  std::vector<char> pair_live((size_t)nfb * nfb, 0);
+  for (int64_t b = 0; b < nb; ++b) {
+    for (int k = pt.b_off[b]; k < pt.b_off[b + 1]; ++k) {
+      pt.tile_slot[(size_t)pt.b_fbl[k] * nb + b] = k;
+      for (int k2 = k; k2 < pt.b_off[b + 1]; ++k2)
+        pair_live[(size_t)pt.b_fbl[k] * nfb + pt.b_fbl[k2]] = 1;
+    }
+  }
+  for (int I = 0; I < nfb; ++I)
+    for (int J = I; J < nfb; ++J)
+      if (pair_live[(size_t)I * nfb + J]) {
+        pt.pair_I.push_back(I);
+        pt.pair_J.push_back(J);
+      }
+  return pt;
+}
+
+// Concatenate the parts into global numbering and upload.
+void upload_plan(Plan &pl, const std::vector<PlanPart> &parts, hipStream_t s) {
+  const int nmol = (int)parts.size();
+  pl.nmol = nmol;
+  pl.mol_nbf_h.resize(nmol);
+  pl.mol_ld_h.resize(nmol);
+  pl.mol_nfb_h.resize(nmol);
+  pl.mol_blk0_h.resize(nmol);
+  pl.mol_b0_h.resize(nmol);
+  pl.mol_nb_h.resize(nmol);
+  pl.mol_npts_h.resize(nmol);
+  pl.mol_P_off_h.resize(nmol);
+  pl.mol_Ps_off_h.resize(nmol);
+  std::vector<int64_t> ts_off(nmol);
+  int nfb = 0;
+  int64_t nb = 0, nslots = 0, ts = 0, poff = 0, psoff = 0, npts = 0;
+  double nb2g = 0, nbg = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const PlanPart &pt = parts[m];
+    pl.mol_nbf_h[m] = pt.nbf;
+    pl.mol_ld_h[m] = pt.nfb * FB;
+    pl.mol_nfb_h[m] = pt.nfb;
+    pl.mol_blk0_h[m] = nfb;
+    pl.mol_b0_h[m] = nb;
+    pl.mol_nb_h[m] = pt.nb;
+    pl.mol_npts_h[m] = pt.npts;
+    pl.mol_P_off_h[m] = poff;
+    pl.mol_Ps_off_h[m] = psoff;
+    ts_off[m] = ts;
+    nfb += pt.nfb;
+    nb += pt.nb;
+    nslots += pt.b_off[pt.nb];
+    ts += (int64_t)pt.nfb * pt.nb;
+    poff += (int64_t)pt.nfb * FB * pt.nfb * FB;
+    psoff += mixed_p_bytes(pt.nfb);
+    npts += pt.npts;
+    nb2g += pt.nb2g;
+    nbg += pt.nbg;
+  }
+  NKX_REQUIRE(nslots < (int64_t)INT32_MAX, NKX_ERR_UNSUPPORTED,
+              "too many live (block, batch) pairs for 32-bit slot numbers");
+  const int nbf_pad = nfb * FB;
+  const int64_t npad = nb * PT;
+  std::vector<int> fperm(nbf_pad, 0), pidx(npad), b_off(nb + 1), b_fbl, slot_b, blk_mol(nfb),
+      blk_local(nfb), tile_slot(std::max<int64_t>(ts, 1), -1);
+  std::vector<FuncRec> fr(nbf_pad);
+  std::vector<HarmTerm> terms;
+  std::vector<double> px(npad), py(npad), pz(npad), pw(npad);
+  pl.pair_I_h.clear();
+  pl.pair_J_h.clear();
+  b_fbl.reserve(nslots);
+  slot_b.reserve(nslots);
+  int nbf_total = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const PlanPart &pt = parts[m];
+    const int blk0 = pl.mol_blk0_h[m];
+    const int64_t b0 = pl.mol_b0_h[m];
+    const int s0 = (int)b_fbl.size();
+    const int toff = (int)terms.size();
+    for (int f = 0; f < pt.nfb; ++f) {
+      blk_mol[blk0 + f] = m;
+      blk_local[blk0 + f] = f;
+    }
+    for (int i = 0; i < pt.nbf; ++i) fperm[blk0 * FB + i] = pt.fperm[i];
+    for (int i = 0; i < pt.nfb * FB; ++i) {
+      FuncRec r = pt.fr[i];
+      r.term_off += toff;
+      fr[blk0 * FB + i] = r;
+    }
+    terms.insert(terms.end(), pt.terms.begin(), pt.terms.end());
+    for (int64_t i = 0; i < pt.nb * PT; ++i) {
+      px[b0 * PT + i] = pt.px[i];
+      py[b0 * PT + i] = pt.py[i];
+      pz[b0 * PT + i] = pt.pz[i];
+      pw[b0 * PT + i] = pt.pw[i];
+      pidx[b0 * PT + i] = pt.pidx[i];
+    }
+    for (int64_t b = 0; b < pt.nb; ++b) b_off[b0 + b] = s0 + pt.b_off[b];
+    for (size_t k = 0; k < pt.b_fbl.size(); ++k) {
+      b_fbl.push_back(blk0 + pt.b_fbl[k]);
+      slot_b.push_back((int)(b0 + pt.slot_b[k]));
+    }
+    for (size_t k = 0; k < pt.tile_slot.size(); ++k)
+      if (pt.tile_slot[k] >= 0) tile_slot[ts_off[m] + k] = s0 + pt.tile_slot[k];
+    for (size_t k = 0; k < pt.pair_I.size(); ++k) {
+      pl.pair_I_h.push_back(blk0 + pt.pair_I[k]);
+      pl.pair_J_h.push_back(blk0 + pt.pair_J[k]);
+    }
+    nbf_total += pt.nbf;
+  }
+  b_off[nb] = (int)b_fbl.size();
+  if (terms.empty()) terms.push_back(HarmTerm{0, 0, 0, 0, 0});
   if (b_fbl.empty()) {  // keep device buffers non-empty
     b_fbl.push_back(0);
     slot_b.push_back(0);
   }
-  std::vector<int> tile_slot((size_t)nfb * nb, -1);
-  std::vector<char> pair_live((size_t)nfb * nfb, 0);
-  for (int64_t b = 0; b < nb; ++b) {
-    for (int k = b_off[b]; k < b_off[b + 1]; ++k) {
-      tile_slot[(size_t)b_fbl[k] * nb + b] = k;
-      for (int k2 = k; k2 < b_off[b + 1]; ++k2) pair_live[(size_t)b_fbl[k] * nfb + b_fbl[k2]] = 1;
-    }
-  }
-  pl.pair_I_h.clear();
-  pl.pair_J_h.clear();
-  for (int I = 0; I < nfb; ++I)
-    for (int J = I; J < nfb; ++J)
-      if (pair_live[(size_t)I * nfb + J]) {
-        pl.pair_I_h.push_back(I);
-        pl.pair_J_h.push_back(J);
-      }
 
-  // ---- upload -------------------------------------------------------------
-  pl.nbf = nbf;
+  pl.nbf = nmol == 1 ? parts[0].nbf : nbf_total;
   pl.nbf_pad = nbf_pad;
   pl.nfb = nfb;
   pl.fb = FB;
@@ -248,10 +345,14 @@ void ensure_plan(Ctx &c) {
   pl.npts_pad = npad;
   pl.nb = nb;
   pl.b_off_h = b_off;
-  pl.nslots = b_off[nb];
-  pl.pairs_kept = b_off[nb];
+  pl.nslots = nslots;
+  pl.pairs_kept = nslots;
   pl.npairs = (int)pl.pair_I_h.size();
-  pl.fperm_d.upload(pl.fperm.data(), nbf, s);
+  pl.work_nb2g = nb2g;
+  pl.work_nbg = nbg;
+  pl.P_total = poff;
+  pl.fperm = fperm;
+  pl.fperm_d.upload(fperm.data(), fperm.size(), s);
   pl.funcs.upload(fr.data(), fr.size(), s);
   pl.terms.upload(terms.data(), terms.size(), s);
   pl.px.upload(px.data(), npad, s);
@@ -267,11 +368,55 @@ void ensure_plan(Ctx &c) {
     pl.pair_I.upload(pl.pair_I_h.data(), pl.npairs, s);
     pl.pair_J.upload(pl.pair_J_h.data(), pl.npairs, s);
   }
+  pl.blk_mol.upload(blk_mol.data(), nfb, s);
+  pl.blk_local.upload(blk_local.data(), nfb, s);
+  pl.mol_ld.upload(pl.mol_ld_h.data(), nmol, s);
+  pl.mol_nbf.upload(pl.mol_nbf_h.data(), nmol, s);
+  pl.mol_nfb.upload(pl.mol_nfb_h.data(), nmol, s);
+  pl.mol_b0.upload(pl.mol_b0_h.data(), nmol, s);
+  pl.mol_nb.upload(pl.mol_nb_h.data(), nmol, s);
+  pl.mol_ts_off.upload(ts_off.data(), nmol, s);
+  pl.mol_P_off.upload(pl.mol_P_off_h.data(), nmol, s);
+  pl.mol_Ps_off.upload(pl.mol_Ps_off_h.data(), nmol, s);
   NKX_HIP_CHECK(hipStreamSynchronize(s));
+  pl.pt = PT;
+}
+
+}  // namespace
+
+void ensure_plan(Ctx &c) {
+  Plan &pl = c.plan;
+  NKX_REQUIRE(c.grid.npts > 0, NKX_ERR_STATE, "no grid: call nkx_make_flat_grid or nkx_set_grid");
+  NKX_REQUIRE(c.basis.nbf > 0, NKX_ERR_STATE, "no basis: call nkx_set_basis");
+  if (pl.grid_version == c.grid.version && pl.basis_version == c.basis_version &&
+      pl.screen_tol == c.opt.screen_tol && pl.pt == PT)
+    return;
+  hipStream_t s = c.stream;
+  const int64_t npts = c.grid.npts;
+  std::vector<double> P(3 * npts), W(npts);
+  NKX_HIP_CHECK(hipMemcpyAsync(P.data(), c.grid.pts.p, 3 * npts * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipMemcpyAsync(W.data(), c.grid.w.p, npts * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  NKX_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<PlanPart> parts;
+  parts.push_back(make_part(P, W, c.basis, c.opt.screen_tol));
+  upload_plan(pl, parts, s);
   pl.grid_version = c.grid.version;
   pl.basis_version = c.basis_version;
   pl.screen_tol = c.opt.screen_tol;
-  pl.pt = PT;
+}
+
+// Plan of a molecule set: molecule m has points mol_pts[m] (3 npts), weights
+// mol_w[m] and basis bases[m].
+void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
+                       const std::vector<std::vector<double>> &mol_w,
+                       const std::vector<BasisHost> &bases) {
+  std::vector<PlanPart> parts;
+  for (size_t m = 0; m < bases.size(); ++m)
+    parts.push_back(make_part(mol_pts[m], mol_w[m], bases[m], c.opt.screen_tol));
+  upload_plan(c.mplan, parts, c.stream);
+  c.mplan.screen_tol = c.opt.screen_tol;
 }
 
 namespace {
@@ -312,15 +457,24 @@ void for_each_chunk(const Plan &pl, int64_t max_slots, F &&fn) {
 
 }  // namespace
 
-// One V_xc build.  nspin = 1: P is the total density matrix, V the potential.
-// nspin = 2 (FP64 only): P[0], P[1] are the spin density matrices and V[0],
-// V[1] the spin potentials.  N[0..nspin-1] receives the electron counts.
-void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
-                    int precision, double *E, double *N, double *const *V) {
-  ensure_plan(c);
-  Plan &pl = c.plan;
+namespace {
+// Device table of output pointers (kept alive in the workspace until the
+// build synchronises).
+double *const *upload_ptrs(Ctx &c, const std::vector<double *> &ptrs) {
+  c.ws.vptr_h = ptrs;
+  c.ws.vptr.upload(c.ws.vptr_h.data(), c.ws.vptr_h.size(), c.stream);
+  return c.ws.vptr.p;
+}
+}  // namespace
+
+// One V_xc build on plan pl.  Molecule m, spin sp: density matrix
+// P[sp * nmol + m] and potential V[sp * nmol + m] (device, nbf_m x nbf_m,
+// row-major); E[m] and N[m * nspin + sp] on return.  nspin = 2 is FP64 only.
+void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
+                   bool gga, int precision, double *E, double *N, double *const *V) {
   Workspace &ws = c.ws;
   hipStream_t s = c.stream;
+  const int nmol = pl.nmol;
   const int nval = gga ? 4 : 1;
   const int narr = nval + nspin;
   const int nsplit = choose_nsplit(pl);
@@ -328,6 +482,8 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   const bool mixed = precision == NKX_PREC_MIXED || bf16;
   NKX_REQUIRE(!mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
               "the spin-polarised build is available in the FP64 form only");
+  NKX_REQUIRE(!bf16 || nmol == 1, NKX_ERR_UNSUPPORTED,
+              "the split-bf16 form builds one molecule at a time");
   const size_t slot_bytes = bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
                             : mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
                                     : (size_t)narr * FB * PT * sizeof(double);
@@ -336,35 +492,42 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
-  const size_t psz = (size_t)pl.nbf_pad * pl.nbf_pad;
   if (mixed) {
     ws.mstore.reserve((size_t)slots_alloc * slot_bytes);
-    ws.Ps.reserve(bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)mixed_p_bytes(pl.nfb));
+    int64_t psb = 0;
+    for (int m = 0; m < nmol; ++m) psb += mixed_p_bytes(pl.mol_nfb_h[m]);
+    ws.Ps.reserve(bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)psb);
   } else {
     ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
-    ws.Pd.reserve(nspin * psz);
+    ws.Pd.reserve((size_t)nspin * pl.P_total);
   }
   ws.coef.reserve((size_t)4 * nspin * pl.npts_pad);
   ws.blockEN.reserve((size_t)3 * pl.nb);
   const size_t vsz = (size_t)nsplit * std::max(pl.npairs, 1) * FB * FB;
   ws.vpart.reserve(nspin * vsz);
-  ws.red.reserve(3);
+  ws.red.reserve((size_t)3 * nmol);
+  double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
 
   Timing &tm = c.timing;
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
   uint16_t *bstore = reinterpret_cast<uint16_t *>(ws.mstore.p);
   uint16_t *bPs = reinterpret_cast<uint16_t *>(ws.Ps.p);
-  for (int sp = 0; sp < nspin; ++sp) {
-    if (bf16)
-      vkb_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P[sp], bPs);
-    else if (mixed)
-      vkm_gather_p(s, pl.nbf, pl.nfb, pl.fperm_d.p, P[sp], ws.Ps.p);
-    else
-      vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P[sp], ws.Pd.p + sp * psz);
-    NKX_HIP_CHECK(hipMemsetAsync(V[sp], 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
-  }
-  NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, nspin * vsz * sizeof(double), s));
+  for (int sp = 0; sp < nspin; ++sp)
+    for (int m = 0; m < nmol; ++m) {
+      const double *Pm = P[(size_t)sp * nmol + m];
+      const int nbf = pl.mol_nbf_h[m];
+      const int *fp = pl.fperm_d.p + (size_t)pl.mol_blk0_h[m] * FB;
+      if (bf16)
+        vkb_gather_p(s, nbf, pl.nfb, fp, Pm, bPs);
+      else if (mixed)
+        vkm_gather_p(s, nbf, pl.mol_nfb_h[m], fp, Pm, ws.Ps.p + pl.mol_Ps_off_h[m]);
+      else
+        vk_gather_p(s, nbf, pl.mol_ld_h[m], fp, Pm,
+                    ws.Pd.p + (size_t)sp * pl.P_total + pl.mol_P_off_h[m]);
+      vk_fill0(s, V[(size_t)sp * nmol + m], (int64_t)nbf * nbf);
+    }
+  vk_fill0(s, ws.vpart.p, (int64_t)(nspin * vsz));
   tm.end(t, s);
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     const int ig = gga ? 1 : 0;
@@ -395,19 +558,31 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
     tm.end(tt, s);
   });
   t = tm.begin(kStageFinal, s);
-  for (int sp = 0; sp < nspin; ++sp) vk_vreduce(s, pl, nsplit, ws.vpart.p + sp * vsz, V[sp]);
-  vk_en_reduce(s, pl.nb, ws.blockEN.p, ws.red.p);
+  for (int sp = 0; sp < nspin; ++sp)
+    vk_vreduce(s, pl, nsplit, ws.vpart.p + sp * vsz, Vtab + (size_t)sp * nmol, pl.mol_nbf.p);
+  vk_en_reduce(s, pl, ws.blockEN.p, ws.red.p);
   tm.end(t, s);
   tm.end(t_build, s);
-  double en[3];
-  NKX_HIP_CHECK(hipMemcpyAsync(en, ws.red.p, sizeof(en), hipMemcpyDeviceToHost, s));
+  std::vector<double> en((size_t)3 * nmol);
+  NKX_HIP_CHECK(hipMemcpyAsync(en.data(), ws.red.p, en.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
   NKX_HIP_CHECK(hipStreamSynchronize(s));
   tm.collect();
   c.work.flops_density = 2.0 * nspin * pl.work_nb2g;
   c.work.flops_v = 2.0 * nspin * pl.work_nb2g;
   c.work.colloc_elems = pl.work_nbg;
-  *E = en[0];
-  for (int sp = 0; sp < nspin; ++sp) N[sp] = en[1 + sp];
+  for (int m = 0; m < nmol; ++m) {
+    E[m] = en[3 * m];
+    for (int sp = 0; sp < nspin; ++sp) N[(size_t)m * nspin + sp] = en[3 * m + 1 + sp];
+  }
+}
+
+// One V_xc build on the ctx grid and basis.  nspin = 1: P[0] is the total
+// density matrix; nspin = 2: P[0], P[1] the spin density matrices.
+void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+                    int precision, double *E, double *N, double *const *V) {
+  ensure_plan(c);
+  build_on_plan(c, c.plan, nspin, P, xl, gga, precision, E, N, V);
 }
 
 void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *E,
@@ -465,7 +640,7 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
   ws.gint.reserve(pl.npts_pad);
 
   vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
-  NKX_HIP_CHECK(hipMemsetAsync(gacc, 0, n_ga * sizeof(double), s));
+  vk_fill0(s, gacc, (int64_t)n_ga);
   const int ig = gga ? 1 : 0;
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
@@ -475,7 +650,7 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
     vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
   });
   vkg_colsum(s, nseg, pl.nbf_pad * 3, gacc, gfun);
-  vk_en_reduce(s, pl.nb, ws.blockEN.p, ws.red.p);
+  vk_en_reduce(s, pl, ws.blockEN.p, ws.red.p);
   vkg_owners(s, pl, c.grid.owner.p, ws.gint.p);
   vkg_weights(s, pl, natoms, c.grid.centers.p, ws.gint.p, epoint, tpt, partition, nblk, wpart);
   vkg_colsum(s, nblk, 3 * natoms, wpart, wsum);
@@ -543,14 +718,15 @@ void overlap_build(Ctx &c, double *S) {
   ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
   const size_t vsz = (size_t)nsplit * std::max(pl.npairs, 1) * FB * FB;
   ws.vpart.reserve(vsz);
-  NKX_HIP_CHECK(hipMemsetAsync(ws.vpart.p, 0, vsz * sizeof(double), s));
-  NKX_HIP_CHECK(hipMemsetAsync(S, 0, (size_t)pl.nbf * pl.nbf * sizeof(double), s));
+  vk_fill0(s, ws.vpart.p, (int64_t)vsz);
+  vk_fill0(s, S, (int64_t)pl.nbf * pl.nbf);
+  double *const *Stab = upload_ptrs(c, std::vector<double *>{S});
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     vk_colloc(s, s0, s1, pl, narr, 0, ws.tiles64.p);
     vk_zoverlap(s, s0, s1, pl, narr, ws.tiles64.p);
     vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, 1, ws.tiles64.p, ws.vpart.p);
   });
-  vk_vreduce(s, pl, nsplit, ws.vpart.p, S);
+  vk_vreduce(s, pl, nsplit, ws.vpart.p, Stab, pl.mol_nbf.p);
   NKX_HIP_CHECK(hipStreamSynchronize(s));
 }
 

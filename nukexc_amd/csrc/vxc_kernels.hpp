@@ -19,8 +19,13 @@ void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr
 // vpart + k * nsplit * npairs * 64 * 64
 void vk_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
               int narr, int nz, const double *tiles, double *vpart);
-void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart, double *V);
-void vk_en_reduce(hipStream_t s, int64_t nb, const double *en, double *out);
+// V of molecule m is Vmol_dev[m] (device array of device pointers), its
+// function count mol_nbf_dev[m]
+void vk_vreduce(hipStream_t s, const Plan &pl, int nsplit, const double *vpart,
+                double *const *Vmol_dev, const int *mol_nbf_dev);
+// out[3 m + c] = (E, N_alpha, N_beta) of molecule m
+void vk_en_reduce(hipStream_t s, const Plan &pl, const double *en, double *out);
+void vk_fill0(hipStream_t s, double *x, int64_t n);
 
 // mixed form (kernels_vxc_i8.hip): int8 slices with per-row exponents,
 // exact integer MFMA; the store holds mixed_slot_bytes(gga) per live slot and

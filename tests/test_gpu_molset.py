@@ -1,0 +1,67 @@
+"""Molecule sets: one launch sequence over many molecules of different sizes
+(nkx_molset_create / nkx_molset_vxc) against the same molecules built one at
+a time on the ctx grid and basis (two paths through the plan builder, the
+per-molecule tables of every kernel and the per-molecule reductions)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import nukexc_amd as nk  # noqa: E402
+from nukexc_amd import configs  # noqa: E402
+
+GRID = dict(nrad=30, nang_order=17)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    c = nk.Context(0)
+    mols, rng = configs.molset(64, 3)
+    mols = mols[:10]
+    Ps, singles = [], []
+    for m in mols:
+        grid = c.make_flat_grid(m["xyz"], m["Z"], **GRID)
+        c.set_basis(m["basis"])
+        S = c.overlap().cpu().numpy()
+        C, P = configs.loewdin_orbitals(S, m["nocc"], rng)
+        Ps.append(P)
+        singles.append({p: c.vxc(torch.tensor(P, device="cuda"), funcs=(101, 130), precision=p)
+                        for p in ("fp64", "mixed")})
+    yield c, mols, Ps, singles
+    c.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_molset_matches_single_builds(setup, precision):
+    c, mols, Ps, singles = setup
+    npts = c.molset_create(mols, **GRID)
+    assert np.all(npts > 0)
+    res = c.molset_vxc([torch.tensor(P, device="cuda") for P in Ps], funcs=(101, 130),
+                       precision=precision)
+    worst = 0.0
+    for k, (r, one) in enumerate(zip(res, singles)):
+        ref = one[precision]
+        dV = (r.potential - ref.potential).abs().max().item()
+        worst = max(worst, dV)
+        assert abs(r.energy - ref.energy) <= 1e-11 * max(1.0, abs(ref.energy)), k
+        assert abs(r.n_el - ref.n_el) <= 1e-11 * ref.n_el, k
+        assert dV <= 1e-12, (k, dV)
+        assert abs(r.n_el - 2 * mols[k]["nocc"]) < 1e-6
+    print(f"{precision}: {len(mols)} molecules, {npts.sum()} points, worst |dV| {worst:.1e}")
+
+
+def test_molset_rejects_split_bf16(setup):
+    """The split-bf16 form builds one molecule at a time: NKX_ERR_UNSUPPORTED."""
+    c, mols, Ps, singles = setup
+    c.molset_create(mols[:2], **GRID)
+    Pd = [torch.tensor(P, device="cuda") for P in Ps[:2]]
+    Vs = [torch.empty_like(P) for P in Pd]
+    Pp = (ctypes.c_void_p * 2)(*[x.data_ptr() for x in Pd])
+    Vp = (ctypes.c_void_p * 2)(*[x.data_ptr() for x in Vs])
+    E = (ctypes.c_double * 2)()
+    ids = (ctypes.c_int * 1)(101)
+    rc = nk.lib().nkx_molset_vxc(c.h, Pp, 1, ids, nk._capi.PREC_MIXED_BF16, E, None, Vp)
+    assert nk._capi.STATUS_NAMES[rc] == "NKX_ERR_UNSUPPORTED"
