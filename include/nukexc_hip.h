@@ -252,6 +252,16 @@ int nkx_molset_create(nkx_ctx *ctx, int nmol, const int *natoms, const double *x
    E_xc[k], N_el[k] (may be NULL). */
 int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P_dev, int nfunc, const int *func_ids,
                    int precision, double *E_xc, double *N_el, double *const *V_dev);
+/* HIP graph of one nkx_vxc build (no counterpart in the reference): captures
+   every launch of a build on the ctx grid and basis for these P and V device
+   pointers (nothing runs during the capture; the graph has its own
+   workspace).  nkx_graph_launch replays it and returns E_xc and N_el (N may be
+   NULL); P's contents may change between replays, as across SCF iterations.
+   A later grid, basis or screening change makes nkx_graph_launch fail with
+   NKX_ERR_STATE until the next capture. */
+int nkx_graph_capture(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
+                      int precision, double *V_dev);
+int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);

@@ -71,6 +71,8 @@ SIGNATURES = {
                                           _vp, _vp],
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
+    "nkx_graph_capture": [_vp, _vp, _c_int, _pi, _c_int, _vp],
+    "nkx_graph_launch": [_vp, _pd, _pd],
     "nkx_molset_create": [_vp, _c_int, _pi, _pd, _pi, _c_int, _c_int, _c_int, _c_double,
                           _c_double, _c_int, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi64],
     "nkx_molset_vxc": [_vp, ctypes.POINTER(_vp), _c_int, _pi, _c_int, _pd, _pd,

@@ -297,6 +297,23 @@ class Context:
                                     ctypes.byref(E), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return E.value, g.reshape(natoms, 3)
 
+    # -- HIP graph of one build ----------------------------------------------
+    def graph_capture(self, P, V, funcs=PBE, precision="fp64"):
+        """Capture one build for the device tensors P (input) and V (output);
+        replay with graph_launch() after changing P in place."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
+                "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
+        assert P.is_contiguous() and V.is_contiguous() and P.dtype == torch.float64
+        ids, nf = _ids(funcs)
+        self._graph_keep = (P, V)
+        self._chk(lib().nkx_graph_capture(self.h, _ptr(P), nf, ids, prec, _ptr(V)))
+
+    def graph_launch(self):
+        """Replay the captured build; returns (E_xc, N_el)."""
+        E, N = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().nkx_graph_launch(self.h, ctypes.byref(E), ctypes.byref(N)))
+        return E.value, N.value
+
     # -- molecule sets ---------------------------------------------------------
     def molset_create(self, mols, nrad=75, nang_order=29, weight_threshold=1e-30,
                       ta_alpha=TA_M4, radial="ta", sizing="uniform"):

@@ -705,6 +705,30 @@ int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int pa
   });
 }
 
+int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,
+                      double *V) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED ||
+                    precision == NKX_PREC_MIXED_BF16,
+                NKX_ERR_INVALID_ARGUMENT, "unknown precision");
+    NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "graph_capture: bad arguments");
+    nkx::graph_capture(ctx->c, P, xl, gga, precision, V);
+  });
+}
+
+int nkx_graph_launch(nkx_ctx *ctx, double *E, double *N) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(E, NKX_ERR_INVALID_ARGUMENT, "graph_launch: E is NULL");
+    nkx::graph_launch(ctx->c, E, N);
+  });
+}
+
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

@@ -193,6 +193,29 @@ struct Work {
   double flops_density = 0, flops_v = 0, colloc_elems = 0;
 };
 
+// A captured build (nkx_graph_capture): its own workspace, so that builds
+// outside the graph never reallocate memory the graph's kernels use.
+struct GraphState {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double *en_pinned = nullptr;  // (E, N_alpha, N_beta), written by the graph
+  std::vector<double *> vtab_h;
+  DevBuf<double *> vtab;
+  Workspace ws;
+  uint64_t grid_version = ~0ull, basis_version = ~0ull;
+  double screen_tol = -1;
+  void release();
+  ~GraphState() { release(); }
+};
+
+// Sizes of one build (prepare_build).
+struct BuildSetup {
+  int nspin = 1, narr = 0, nsplit = 1;
+  bool gga = false, bf16 = false, mixed = false;
+  int64_t slots_alloc = 0;
+  size_t vsz = 0;
+};
+
 struct Ctx {
   int device = 0;
   Timing timing;
@@ -207,6 +230,7 @@ struct Ctx {
   Plan plan;
   Plan mplan;                    // plan of the molecule set (nkx_molset_create)
   Workspace ws;
+  GraphState graph;
 };
 
 // ---- host helpers (basis.cpp, lebedev.cpp, grid.cpp) --------------------
@@ -271,6 +295,9 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
                     int precision, double *E, double *N, double *const *V);
 void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
                    bool gga, int precision, double *E, double *N, double *const *V);
+void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
+                   double *V);
+void graph_launch(Ctx &c, double *E, double *N);
 void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
                        const std::vector<std::vector<double>> &mol_w,
                        const std::vector<BasisHost> &bases);

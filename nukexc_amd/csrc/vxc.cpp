@@ -467,47 +467,56 @@ double *const *upload_ptrs(Ctx &c, const std::vector<double *> &ptrs) {
 }
 }  // namespace
 
-// One V_xc build on plan pl.  Molecule m, spin sp: density matrix
-// P[sp * nmol + m] and potential V[sp * nmol + m] (device, nbf_m x nbf_m,
-// row-major); E[m] and N[m * nspin + sp] on return.  nspin = 2 is FP64 only.
-void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
-                   bool gga, int precision, double *E, double *N, double *const *V) {
-  Workspace &ws = c.ws;
-  hipStream_t s = c.stream;
+// Sizes and workspace of one build on plan pl (allocations happen here, so a
+// captured build performs none).
+BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int precision) {
+  BuildSetup b;
+  b.nspin = nspin;
+  b.gga = gga;
   const int nmol = pl.nmol;
-  const int nval = gga ? 4 : 1;
-  const int narr = nval + nspin;
-  const int nsplit = choose_nsplit(pl);
-  const bool bf16 = precision == NKX_PREC_MIXED_BF16;
-  const bool mixed = precision == NKX_PREC_MIXED || bf16;
-  NKX_REQUIRE(!mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
+  b.narr = (gga ? 4 : 1) + nspin;
+  b.nsplit = choose_nsplit(pl);
+  b.bf16 = precision == NKX_PREC_MIXED_BF16;
+  b.mixed = precision == NKX_PREC_MIXED || b.bf16;
+  NKX_REQUIRE(!b.mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
               "the spin-polarised build is available in the FP64 form only");
-  NKX_REQUIRE(!bf16 || nmol == 1, NKX_ERR_UNSUPPORTED,
+  NKX_REQUIRE(!b.bf16 || nmol == 1, NKX_ERR_UNSUPPORTED,
               "the split-bf16 form builds one molecule at a time");
-  const size_t slot_bytes = bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
-                            : mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
-                                    : (size_t)narr * FB * PT * sizeof(double);
+  const size_t slot_bytes = b.bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
+                            : b.mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
+                                      : (size_t)b.narr * FB * PT * sizeof(double);
   int64_t max_slots = chunk_slot_budget(slot_bytes);
   // at least the largest single batch
-  for (int64_t b = 0; b < pl.nb; ++b)
-    max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
-  const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
-  if (mixed) {
-    ws.mstore.reserve((size_t)slots_alloc * slot_bytes);
+  for (int64_t bb = 0; bb < pl.nb; ++bb)
+    max_slots = std::max<int64_t>(max_slots, pl.b_off_h[bb + 1] - pl.b_off_h[bb]);
+  b.slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
+  if (b.mixed) {
+    ws.mstore.reserve((size_t)b.slots_alloc * slot_bytes);
     int64_t psb = 0;
     for (int m = 0; m < nmol; ++m) psb += mixed_p_bytes(pl.mol_nfb_h[m]);
-    ws.Ps.reserve(bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)psb);
+    ws.Ps.reserve(b.bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)psb);
   } else {
-    ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
+    ws.tiles64.reserve((size_t)b.slots_alloc * b.narr * FB * PT);
     ws.Pd.reserve((size_t)nspin * pl.P_total);
   }
   ws.coef.reserve((size_t)4 * nspin * pl.npts_pad);
   ws.blockEN.reserve((size_t)3 * pl.nb);
-  const size_t vsz = (size_t)nsplit * std::max(pl.npairs, 1) * FB * FB;
-  ws.vpart.reserve(nspin * vsz);
-  ws.red.reserve((size_t)3 * nmol);
-  double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
+  b.vsz = (size_t)b.nsplit * std::max(pl.npairs, 1) * FB * FB;
+  ws.vpart.reserve(nspin * b.vsz);
+  ws.red.reserve((size_t)3 * pl.nmol);
+  return b;
+}
 
+// Every launch of one build, stream-ordered, ending with the copy of the
+// (E, N_alpha, N_beta) triples of all molecules into en_dst (host).  No
+// allocation, no memset, no synchronisation: the sequence can be captured.
+void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
+                   const double *const *P, const XcList &xl, double *const *V,
+                   double *const *Vtab, double *en_dst) {
+  hipStream_t s = c.stream;
+  const int nmol = pl.nmol, nspin = b.nspin, narr = b.narr, nsplit = b.nsplit;
+  const bool bf16 = b.bf16, mixed = b.mixed;
+  const size_t vsz = b.vsz;
   Timing &tm = c.timing;
   const int t_build = tm.begin(kStageBuild, s);
   int t = tm.begin(kStageFinal, s);
@@ -529,8 +538,8 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
     }
   vk_fill0(s, ws.vpart.p, (int64_t)(nspin * vsz));
   tm.end(t, s);
-  for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
-    const int ig = gga ? 1 : 0;
+  for_each_chunk(pl, b.slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
+    const int ig = b.gga ? 1 : 0;
     int tt = tm.begin(kStageColloc, s);
     if (bf16)
       vkb_colloc(s, s0, s1, pl, ig, bstore);
@@ -563,11 +572,22 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
   vk_en_reduce(s, pl, ws.blockEN.p, ws.red.p);
   tm.end(t, s);
   tm.end(t_build, s);
-  std::vector<double> en((size_t)3 * nmol);
-  NKX_HIP_CHECK(hipMemcpyAsync(en.data(), ws.red.p, en.size() * sizeof(double),
+  NKX_HIP_CHECK(hipMemcpyAsync(en_dst, ws.red.p, (size_t)3 * nmol * sizeof(double),
                                hipMemcpyDeviceToHost, s));
-  NKX_HIP_CHECK(hipStreamSynchronize(s));
-  tm.collect();
+}
+
+// One V_xc build on plan pl.  Molecule m, spin sp: density matrix
+// P[sp * nmol + m] and potential V[sp * nmol + m] (device, nbf_m x nbf_m,
+// row-major); E[m] and N[m * nspin + sp] on return.  nspin = 2 is FP64 only.
+void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
+                   bool gga, int precision, double *E, double *N, double *const *V) {
+  const int nmol = pl.nmol;
+  const BuildSetup b = prepare_build(c.ws, pl, nspin, gga, precision);
+  double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
+  std::vector<double> en((size_t)3 * nmol);
+  enqueue_build(c, c.ws, pl, b, P, xl, V, Vtab, en.data());
+  NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+  c.timing.collect();
   c.work.flops_density = 2.0 * nspin * pl.work_nb2g;
   c.work.flops_v = 2.0 * nspin * pl.work_nb2g;
   c.work.colloc_elems = pl.work_nbg;
@@ -575,6 +595,60 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
     E[m] = en[3 * m];
     for (int sp = 0; sp < nspin; ++sp) N[(size_t)m * nspin + sp] = en[3 * m + 1 + sp];
   }
+}
+
+// ---- HIP graph of one build on the ctx grid and basis ----------------------
+void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
+                   double *V) {
+  ensure_plan(c);
+  GraphState &gs = c.graph;
+  gs.release();
+  Plan &pl = c.plan;
+  const BuildSetup b = prepare_build(gs.ws, pl, 1, gga, precision);
+  NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned), 3 * sizeof(double),
+                              hipHostMallocDefault));
+  gs.vtab_h.assign(1, V);
+  gs.vtab.upload(gs.vtab_h.data(), 1, c.stream);
+  NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+  const bool timing = c.timing.on;
+  c.timing.on = false;  // no event records inside the graph
+  NKX_HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_build(c, gs.ws, pl, b, &P, xl, &V, gs.vtab.p, gs.en_pinned);
+  } catch (...) {
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(c.stream, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    c.timing.on = timing;
+    throw;
+  }
+  c.timing.on = timing;
+  NKX_HIP_CHECK(hipStreamEndCapture(c.stream, &gs.graph));
+  NKX_HIP_CHECK(hipGraphInstantiate(&gs.exec, gs.graph, nullptr, nullptr, 0));
+  gs.grid_version = c.grid.version;
+  gs.basis_version = c.basis_version;
+  gs.screen_tol = c.opt.screen_tol;
+}
+
+void graph_launch(Ctx &c, double *E, double *N) {
+  GraphState &gs = c.graph;
+  NKX_REQUIRE(gs.exec, NKX_ERR_STATE, "no captured build: call nkx_graph_capture");
+  NKX_REQUIRE(gs.grid_version == c.grid.version && gs.basis_version == c.basis_version &&
+                  gs.screen_tol == c.opt.screen_tol,
+              NKX_ERR_STATE, "grid, basis or screening changed since the capture");
+  NKX_HIP_CHECK(hipGraphLaunch(gs.exec, c.stream));
+  NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+  *E = gs.en_pinned[0];
+  if (N) *N = gs.en_pinned[1];
+}
+
+void GraphState::release() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (en_pinned) (void)hipHostFree(en_pinned);
+  exec = nullptr;
+  graph = nullptr;
+  en_pinned = nullptr;
 }
 
 // One V_xc build on the ctx grid and basis.  nspin = 1: P[0] is the total
