@@ -729,6 +729,23 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E, double *N) {
   });
 }
 
+int nkx_autotune(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,
+                 double *V, int *vgemm_split, double *best_ms) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED ||
+                    precision == NKX_PREC_MIXED_BF16,
+                NKX_ERR_INVALID_ARGUMENT, "unknown precision");
+    NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "autotune: bad arguments");
+    const double ms = nkx::autotune(ctx->c, P, xl, gga, precision, V, vgemm_split);
+    if (best_ms) *best_ms = ms;
+  });
+}
+
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

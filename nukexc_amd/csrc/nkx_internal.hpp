@@ -148,6 +148,7 @@ struct Plan {
 struct Options {
   double screen_tol = 1e-12;
   int batch_points = 64;
+  int vgemm_split = 0;          // V-contraction split count; 0 = automatic
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
 };
 
@@ -298,6 +299,8 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
 void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
                    double *V);
 void graph_launch(Ctx &c, double *E, double *N);
+double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
+                int *best_split);
 void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
                        const std::vector<std::vector<double>> &mol_w,
                        const std::vector<BasisHost> &bases);

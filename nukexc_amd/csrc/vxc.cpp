@@ -430,9 +430,11 @@ int64_t chunk_slot_budget(size_t slot_bytes) {
   return (int64_t)(budget / slot_bytes);
 }
 
-int choose_nsplit(const Plan &pl) {
+// Split count of the V contraction over batches: enough workgroups to fill
+// the chip (about 2048), or the autotuned value.
+int choose_nsplit(const Plan &pl, int tuned = 0) {
   if (pl.npairs == 0) return 1;
-  int64_t ns = (2048 + pl.npairs - 1) / pl.npairs;
+  int64_t ns = tuned > 0 ? tuned : (2048 + pl.npairs - 1) / pl.npairs;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, pl.nb));
   return (int)ns;
 }
@@ -469,13 +471,14 @@ double *const *upload_ptrs(Ctx &c, const std::vector<double *> &ptrs) {
 
 // Sizes and workspace of one build on plan pl (allocations happen here, so a
 // captured build performs none).
-BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int precision) {
+BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int precision,
+                         int tuned_split) {
   BuildSetup b;
   b.nspin = nspin;
   b.gga = gga;
   const int nmol = pl.nmol;
   b.narr = (gga ? 4 : 1) + nspin;
-  b.nsplit = choose_nsplit(pl);
+  b.nsplit = choose_nsplit(pl, tuned_split);
   b.bf16 = precision == NKX_PREC_MIXED_BF16;
   b.mixed = precision == NKX_PREC_MIXED || b.bf16;
   NKX_REQUIRE(!b.mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
@@ -582,7 +585,7 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
 void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
                    bool gga, int precision, double *E, double *N, double *const *V) {
   const int nmol = pl.nmol;
-  const BuildSetup b = prepare_build(c.ws, pl, nspin, gga, precision);
+  const BuildSetup b = prepare_build(c.ws, pl, nspin, gga, precision, c.opt.vgemm_split);
   double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
   std::vector<double> en((size_t)3 * nmol);
   enqueue_build(c, c.ws, pl, b, P, xl, V, Vtab, en.data());
@@ -597,6 +600,58 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
   }
 }
 
+// ---- autotuner ------------------------------------------------------------
+// Times one build per candidate split count of the V contraction (after one
+// untimed build) and keeps the fastest in the ctx options.
+double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
+                int *best_split) {
+  ensure_plan(c);
+  const int base = choose_nsplit(c.plan, 0);
+  std::vector<int> cand;
+  for (int k : {base / 4, base / 2, base, 2 * base, 4 * base}) {
+    const int v = std::max(1, std::min<int>(k, (int)std::max<int64_t>(c.plan.nb, 1)));
+    if (std::find(cand.begin(), cand.end(), v) == cand.end()) cand.push_back(v);
+  }
+  double E, N;
+  hipEvent_t a, b;
+  NKX_HIP_CHECK(hipEventCreate(&a));
+  NKX_HIP_CHECK(hipEventCreate(&b));
+  double best = 1e300;
+  int best_k = base;
+  const int saved = c.opt.vgemm_split;
+  const bool timing = c.timing.on;
+  c.timing.on = false;
+  try {
+    c.opt.vgemm_split = cand[0];
+    vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
+    for (int k : cand) {
+      c.opt.vgemm_split = k;
+      NKX_HIP_CHECK(hipEventRecord(a, c.stream));
+      vxc_build(c, P, xl, gga, precision, &E, &N, V);
+      NKX_HIP_CHECK(hipEventRecord(b, c.stream));
+      NKX_HIP_CHECK(hipEventSynchronize(b));
+      float ms = 0.f;
+      NKX_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) {
+        best = ms;
+        best_k = k;
+      }
+    }
+  } catch (...) {
+    c.opt.vgemm_split = saved;
+    c.timing.on = timing;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  c.timing.on = timing;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c.opt.vgemm_split = best_k;
+  if (best_split) *best_split = best_k;
+  return best;
+}
+
 // ---- HIP graph of one build on the ctx grid and basis ----------------------
 void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
                    double *V) {
@@ -604,7 +659,7 @@ void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int prec
   GraphState &gs = c.graph;
   gs.release();
   Plan &pl = c.plan;
-  const BuildSetup b = prepare_build(gs.ws, pl, 1, gga, precision);
+  const BuildSetup b = prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split);
   NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned), 3 * sizeof(double),
                               hipHostMallocDefault));
   gs.vtab_h.assign(1, V);

@@ -56,3 +56,17 @@ def test_graph_invalidated_by_new_grid(setup):
     with pytest.raises(nk.NukexcError) as ei:
         c.graph_launch()
     assert ei.value.status == "NKX_ERR_STATE"
+
+
+def test_autotune_keeps_results(setup):
+    """The autotuner picks a split count among its candidates; builds after it
+    agree with builds before it to summation-order rounding."""
+    c, w, P1, P2 = setup
+    c.make_flat_grid(w["xyz"], w["Z"], nrad=30, nang_order=17)
+    r0 = c.vxc(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
+    k, ms = c.autotune(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
+    r1 = c.vxc(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
+    print(f"autotune: split {k}, {ms:.2f} ms per build")
+    assert k >= 1 and ms > 0
+    assert abs(r1.energy - r0.energy) <= 1e-12 * abs(r0.energy)
+    assert (r1.potential - r0.potential).abs().max().item() <= 1e-13
