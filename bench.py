@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the FP64 reference build of a mixed run (profiling)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (density batch group, V split count)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="timed FP64 builds for the mixed-vs-FP64 speedup (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -154,6 +156,12 @@ def main():
             torch.cuda.synchronize()
             fp64_ms = 1e3 * (time.perf_counter() - t0) / args.fp64_steps
             log(f"FP64 form: {fp64_ms:.2f} ms/build over {args.fp64_steps} builds")
+
+    tuned = None
+    if not args.no_autotune:
+        k, g, ms = ctx.autotune(P, funcs, precision=args.precision)
+        tuned = {"vgemm_split": k, "density_group": g, "ms": ms}
+        log(f"autotune: V split {k}, density group {g}, {ms:.1f} ms/build")
 
     for i in range(args.warmup):
         r = ctx.vxc(P, funcs, precision=args.precision)
@@ -231,6 +239,7 @@ def main():
                                f"{'PBE' if max(funcs) > 100 else 'LDA_X'}",
                    "precision": args.precision, "builds_per_s": 1e3 / ms_step,
                    "median_ms_per_build": float(np.median(step_ms)),
+                   "autotuned": tuned,
                    "E_xc": r.energy, "N_el": r.n_el},
         "roofline": roof,
         "cpu_baseline": cpu,

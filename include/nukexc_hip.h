@@ -262,13 +262,16 @@ int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P_dev, int nfunc, const in
 int nkx_graph_capture(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                       int precision, double *V_dev);
 int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
-/* Autotuner (no counterpart in the reference): times one nkx_vxc build per
-   candidate split count of the V contraction over batches (the automatic
-   choice x 1/4, 1/2, 1, 2, 4) on the ctx grid and basis for this P, keeps
-   the fastest for later builds and captures, and returns it with its build
-   time.  V receives the last build's potential. */
+/* Autotuner (no counterpart in the reference): on the ctx grid and basis for
+   this P, times one nkx_vxc build per candidate -- first the number of
+   point batches per density workgroup (1, 2, 4), then the split count of the
+   V contraction over batches (the automatic choice x 1/4 .. 4) -- keeps the
+   fastest for later builds and captures, and returns them (either pointer
+   may be NULL) with the best build time.  V receives the last build's
+   potential. */
 int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
-                 int precision, double *V_dev, int *vgemm_split, double *best_ms);
+                 int precision, double *V_dev, int *vgemm_split, int *density_group,
+                 double *best_ms);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);

@@ -298,17 +298,17 @@ class Context:
         return E.value, g.reshape(natoms, 3)
 
     def autotune(self, P, funcs=PBE, precision="fp64"):
-        """Pick the fastest V-contraction split count for this workload;
-        returns (split, ms per build)."""
+        """Pick the fastest density batch group and V-contraction split count
+        for this workload; returns (split, group, ms per build)."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
         P = _dev_f64(P, self.device).contiguous()
         V = torch.empty_like(P)
         ids, nf = _ids(funcs)
-        k, ms = ctypes.c_int(), ctypes.c_double()
+        k, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._chk(lib().nkx_autotune(self.h, _ptr(P), nf, ids, prec, _ptr(V), ctypes.byref(k),
-                                     ctypes.byref(ms)))
-        return k.value, ms.value
+                                     ctypes.byref(g), ctypes.byref(ms)))
+        return k.value, g.value, ms.value
 
     # -- HIP graph of one build ----------------------------------------------
     def graph_capture(self, P, V, funcs=PBE, precision="fp64"):

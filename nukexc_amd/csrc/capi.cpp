@@ -730,7 +730,7 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E, double *N) {
 }
 
 int nkx_autotune(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,
-                 double *V, int *vgemm_split, double *best_ms) {
+                 double *V, int *vgemm_split, int *density_group, double *best_ms) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
     nkx::XcList xl;
@@ -741,7 +741,7 @@ int nkx_autotune(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int p
                     precision == NKX_PREC_MIXED_BF16,
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
     NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "autotune: bad arguments");
-    const double ms = nkx::autotune(ctx->c, P, xl, gga, precision, V, vgemm_split);
+    const double ms = nkx::autotune(ctx->c, P, xl, gga, precision, V, vgemm_split, density_group);
     if (best_ms) *best_ms = ms;
   });
 }

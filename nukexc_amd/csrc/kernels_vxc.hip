@@ -74,10 +74,11 @@ __global__ void __launch_bounds__(256) colloc_tiles_f64(
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup per group of GB consecutive (Morton-adjacent) batches: every
+// One workgroup per group of GBT consecutive (Morton-adjacent) batches: every
 // 64 x 64 block of P is loaded into LDS once and applied to all batches of
 // the group in which both of its blocks are live, so P is streamed from the
-// cache hierarchy once per GB batches.  For each batch the K blocks are
+// cache hierarchy once per GBT batches (1, 2 or 4, chosen at launch).  For
+// each batch the K blocks are
 // visited in ascending order, as with one batch per workgroup, so every sum
 // is formed in the same order.
 //
@@ -85,31 +86,30 @@ __global__ void __launch_bounds__(256) colloc_tiles_f64(
 // P_beta (each nbf_pad^2); the functional is evaluated spin-polarised and the
 // tile store carries Z_alpha, Z_beta as its last two arrays.  blockEN holds
 // (E, N_alpha, N_beta) per batch (N_beta = 0 for NSPIN = 1, N_alpha = N).
-constexpr int GB = 4;          // batches per workgroup
 constexpr int MAXNFB = 1024;   // function blocks the union table holds
 
-template <int NSPIN>
+template <int NSPIN, int GBT>
 __global__ void __launch_bounds__(256) density_fused_f64(
     MolTab mt, int64_t pspin, int nfb, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int narr, int gga,
     double *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
-  const int64_t bg = b0 + (int64_t)blockIdx.x * GB;
-  const int nsub = (int)(b1 - bg < GB ? b1 - bg : GB);
+  const int64_t bg = b0 + (int64_t)blockIdx.x * GBT;
+  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
 
   __shared__ double Pk[FB][FB];   // Pk[k][i] = P(mu = i, nu = k) of the block pair
   __shared__ double Ph[FB][PT];   // Phi tile of the K block
   __shared__ double red[4][4][PT];
-  __shared__ double colsum[NSPIN][GB][4][PT];  // rho, d/dx, d/dy, d/dz column sums
-  __shared__ double cf[NSPIN][GB][4][PT];      // Z coefficients
-  __shared__ int tab[GB][MAXNFB];              // slot of (sub-batch, block) or -1
+  __shared__ double colsum[NSPIN][GBT][4][PT];  // rho, d/dx, d/dy, d/dz column sums
+  __shared__ double cf[NSPIN][GBT][4][PT];      // Z coefficients
+  __shared__ int tab[GBT][MAXNFB];              // slot of (sub-batch, block) or -1
   __shared__ int ublk[MAXNFB];                 // union of live blocks, ascending
   __shared__ int nu_s;
 
-  for (int e = threadIdx.x; e < GB * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
-  for (int e = threadIdx.x; e < NSPIN * GB * 4 * PT; e += 256) (&colsum[0][0][0][0])[e] = 0.0;
+  for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+  for (int e = threadIdx.x; e < NSPIN * GBT * 4 * PT; e += 256) (&colsum[0][0][0][0])[e] = 0.0;
   __syncthreads();
   for (int sb = 0; sb < nsub; ++sb) {
     const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       const int f = base + lane;
       bool live = false;
       if (f < nfb)
-        for (int sb = 0; sb < GB; ++sb) live = live || tab[sb][f] >= 0;
+        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
       const unsigned long long m = __ballot(live);
       if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
       cnt += __popcll(m);
@@ -139,9 +139,9 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       const int fm = ublk[mi];
       const int mol = mt.blk_mol[fm], ld = mt.ld[mol];
       const double *Pmol = Psp + mt.P_off[mol] + mt.blk_local[fm] * FB;
-      v4d acc[GB][4];
+      v4d acc[GBT][4];
 #pragma unroll
-      for (int sb = 0; sb < GB; ++sb)
+      for (int sb = 0; sb < GBT; ++sb)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[sb][j] = v4d{0.0, 0.0, 0.0, 0.0};
       for (int ki = 0; ki < nu; ++ki) {
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
           }
         }
 #pragma unroll
-        for (int sb = 0; sb < GB; ++sb) {
+        for (int sb = 0; sb < GBT; ++sb) {
           if (sb >= nsub) break;
           const int slk = tab[sb][fk];
           if (slk < 0 || tab[sb][fm] < 0) continue;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       }
       // Epilogue: column partial sums sum_mu Phi(mu,g) X(mu,g) (and gradients).
 #pragma unroll
-      for (int sb = 0; sb < GB; ++sb) {
+      for (int sb = 0; sb < GBT; ++sb) {
         if (sb >= nsub) break;
         const int slm = tab[sb][fm];
         if (slm < 0) continue;
@@ -485,21 +485,34 @@ void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, 
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
-                const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
-                double *coef, double *blockEN, double *epoint) {
+template <int NSPIN, int GBT>
+static void launch_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                           const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
+                           double *coef, double *blockEN, double *epoint) {
+  const unsigned nwg = (unsigned)((b1 - b0 + GBT - 1) / GBT);
+  hipLaunchKernelGGL((density_fused_f64<NSPIN, GBT>), dim3(nwg), dim3(256), 0, s, pl.tab(),
+                     pl.P_total, pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles,
+                     pl.pw.p, xl, coef, blockEN, epoint);
+}
+
+void vk_density(hipStream_t s, int nspin, int group, int64_t b0, int64_t b1, int64_t s0,
+                const Plan &pl, const double *Pd, int narr, int gga, double *tiles,
+                const XcList &xl, double *coef, double *blockEN, double *epoint) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
-  const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
-  if (nspin == 2)
-    hipLaunchKernelGGL(density_fused_f64<2>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.P_total,
-                       pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN, epoint);
-  else
-    hipLaunchKernelGGL(density_fused_f64<1>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.P_total,
-                       pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles, pl.pw.p,
-                       xl, coef, blockEN, epoint);
+  const int g = group == 1 || group == 2 ? group : 4;
+#define NKX_DENS(NS, G) launch_density<NS, G>(s, b0, b1, s0, pl, Pd, narr, gga, tiles, xl, coef, blockEN, epoint)
+  if (nspin == 2) {
+    if (g == 1) NKX_DENS(2, 1);
+    else if (g == 2) NKX_DENS(2, 2);
+    else NKX_DENS(2, 4);
+  } else {
+    if (g == 1) NKX_DENS(1, 1);
+    else if (g == 2) NKX_DENS(1, 2);
+    else NKX_DENS(1, 4);
+  }
+#undef NKX_DENS
   NKX_HIP_CHECK(hipGetLastError());
 }
 

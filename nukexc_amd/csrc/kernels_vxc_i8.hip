@@ -51,7 +51,6 @@ constexpr int TILE = FB * PT;
 constexpr int PLANE = TILE;                  // bytes per slice plane
 constexpr int SEC = 4 * PLANE + 256;         // section: 4 planes + exponents (padded)
 constexpr int OFF_F = 0, OFF_PM = SEC, OFF_Z = 2 * SEC, OFF_G = 3 * SEC;
-constexpr int GB = 4;
 constexpr int MAXNFB = 1024;
 
 __host__ __device__ inline int64_t slot_bytes_i8(int gga) {
@@ -232,32 +231,34 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 }
 
 // ---------------------------------------------------------------------------
-// Density contraction, functional and Z.  One workgroup per GB batches; wave
+// Density contraction, functional and Z.  One workgroup per GBT (1, 2 or 4)
+// batches; wave
 // w owns rows 16 w .. 16 w + 15 of the current M block; every P block
 // section is read once per group and applied to each batch of the group in
 // which both blocks are live (K blocks ascending per batch).
+template <int GBT>
 __global__ void __launch_bounds__(256) density_fused_i8(
     MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int gga,
     signed char *__restrict__ store, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN) {
-  const int64_t bg = b0 + (int64_t)blockIdx.x * GB;
-  const int nsub = (int)(b1 - bg < GB ? b1 - bg : GB);
+  const int64_t bg = b0 + (int64_t)blockIdx.x * GBT;
+  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
 
   __shared__ double red[4][4][PT];
-  __shared__ double colsum[GB][4][PT];
-  __shared__ double cf[GB][4][PT];
+  __shared__ double colsum[GBT][4][PT];
+  __shared__ double cf[GBT][4][PT];
   __shared__ double zt[FB][65];
   __shared__ int ez[FB];
-  __shared__ int tab[GB][MAXNFB];
+  __shared__ int tab[GBT][MAXNFB];
   __shared__ int ublk[MAXNFB];
   __shared__ int nu_s;
 
-  for (int e = threadIdx.x; e < GB * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
-  for (int e = threadIdx.x; e < GB * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
+  for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+  for (int e = threadIdx.x; e < GBT * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
   __syncthreads();
   for (int sb = 0; sb < nsub; ++sb) {
     const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
       const int f = base + lane;
       bool live = false;
       if (f < nfb)
-        for (int sb = 0; sb < GB; ++sb) live = live || tab[sb][f] >= 0;
+        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
       const unsigned long long m = __ballot(live);
       if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
       cnt += __popcll(m);
@@ -286,9 +287,9 @@ __global__ void __launch_bounds__(256) density_fused_i8(
     const int mol = mt.blk_mol[fm];
     const signed char *Pmol =
         Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
-    double x64[GB][4][4];
+    double x64[GBT][4][4];
 #pragma unroll
-    for (int sb = 0; sb < GB; ++sb)
+    for (int sb = 0; sb < GBT; ++sb)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
 #pragma unroll
       for (int r = 0; r < 4; ++r) eA[r] = pa[4 * PLANE + 16 * wv + 4 * q + r];
 #pragma unroll
-      for (int sb = 0; sb < GB; ++sb) {
+      for (int sb = 0; sb < GBT; ++sb) {
         if (sb >= nsub) break;
         const int slk = tab[sb][fk];
         if (slk < 0 || tab[sb][fm] < 0) continue;
@@ -329,7 +330,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
     }
     // Epilogue in FP64: column partial sums of Phi X (and grad Phi X).
 #pragma unroll
-    for (int sb = 0; sb < GB; ++sb) {
+    for (int sb = 0; sb < GBT; ++sb) {
       if (sb >= nsub) break;
       const int slm = tab[sb][fm];
       if (slm < 0) continue;
@@ -527,15 +528,27 @@ void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-void vkm_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+template <int GBT>
+static void launch_density_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                              const signed char *Ps, int gga, signed char *store,
+                              const XcList &xl, double *coef, double *blockEN) {
+  const unsigned nwg = (unsigned)((b1 - b0 + GBT - 1) / GBT);
+  hipLaunchKernelGGL(density_fused_i8<GBT>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.nfb, Ps,
+                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
+}
+
+void vkm_density(hipStream_t s, int group, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
-  const unsigned nwg = (unsigned)((b1 - b0 + GB - 1) / GB);
-  hipLaunchKernelGGL(density_fused_i8, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.nfb, Ps,
-                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
+  if (group == 1)
+    launch_density_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
+  else if (group == 2)
+    launch_density_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
+  else
+    launch_density_i8<4>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -149,6 +149,7 @@ struct Options {
   double screen_tol = 1e-12;
   int batch_points = 64;
   int vgemm_split = 0;          // V-contraction split count; 0 = automatic
+  int density_group = 0;        // batches per density workgroup (1, 2, 4); 0 = form default
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
 };
 
@@ -211,7 +212,7 @@ struct GraphState {
 
 // Sizes of one build (prepare_build).
 struct BuildSetup {
-  int nspin = 1, narr = 0, nsplit = 1;
+  int nspin = 1, narr = 0, nsplit = 1, group = 4;
   bool gga = false, bf16 = false, mixed = false;
   int64_t slots_alloc = 0;
   size_t vsz = 0;
@@ -300,7 +301,7 @@ void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int prec
                    double *V);
 void graph_launch(Ctx &c, double *E, double *N);
 double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
-                int *best_split);
+                int *best_split, int *best_group);
 void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
                        const std::vector<std::vector<double>> &mol_w,
                        const std::vector<BasisHost> &bases);

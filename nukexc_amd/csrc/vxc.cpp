@@ -472,9 +472,11 @@ double *const *upload_ptrs(Ctx &c, const std::vector<double *> &ptrs) {
 // Sizes and workspace of one build on plan pl (allocations happen here, so a
 // captured build performs none).
 BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int precision,
-                         int tuned_split) {
+                         int tuned_split, int tuned_group) {
   BuildSetup b;
   b.nspin = nspin;
+  // batches per density workgroup: tuned, else 4 (FP64) / 2 (int8 form)
+  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 2 : 4);
   b.gga = gga;
   const int nmol = pl.nmol;
   b.narr = (gga ? 4 : 1) + nspin;
@@ -555,9 +557,11 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
     if (bf16)
       vkb_density(s, 6, b0, b1, s0, pl, bPs, ig, bstore, xl, ws.coef.p, ws.blockEN.p);
     else if (mixed)
-      vkm_density(s, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p, ws.blockEN.p);
+      vkm_density(s, b.group, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
+                  ws.blockEN.p);
     else
-      vk_density(s, nspin, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+      vk_density(s, nspin, b.group, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl,
+                 ws.coef.p,
                  ws.blockEN.p, nullptr);
     tm.end(tt, s);
     tt = tm.begin(kStageVgemm, s);
@@ -585,7 +589,8 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
 void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
                    bool gga, int precision, double *E, double *N, double *const *V) {
   const int nmol = pl.nmol;
-  const BuildSetup b = prepare_build(c.ws, pl, nspin, gga, precision, c.opt.vgemm_split);
+  const BuildSetup b =
+      prepare_build(c.ws, pl, nspin, gga, precision, c.opt.vgemm_split, c.opt.density_group);
   double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
   std::vector<double> en((size_t)3 * nmol);
   enqueue_build(c, c.ws, pl, b, P, xl, V, Vtab, en.data());
@@ -601,44 +606,66 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
 }
 
 // ---- autotuner ------------------------------------------------------------
-// Times one build per candidate split count of the V contraction (after one
-// untimed build) and keeps the fastest in the ctx options.
+// Times one build per candidate (after one untimed build): first the batches
+// per density workgroup (1, 2, 4), then the split count of the V contraction
+// (1/4 .. 4 x the automatic choice), and keeps the fastest in the ctx options.
 double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
-                int *best_split) {
+                int *best_split, int *best_group) {
   ensure_plan(c);
   const int base = choose_nsplit(c.plan, 0);
-  std::vector<int> cand;
+  std::vector<int> splits;
   for (int k : {base / 4, base / 2, base, 2 * base, 4 * base}) {
     const int v = std::max(1, std::min<int>(k, (int)std::max<int64_t>(c.plan.nb, 1)));
-    if (std::find(cand.begin(), cand.end(), v) == cand.end()) cand.push_back(v);
+    if (std::find(splits.begin(), splits.end(), v) == splits.end()) splits.push_back(v);
   }
+  const bool bf16 = precision == NKX_PREC_MIXED_BF16;  // one batch per workgroup
   double E, N;
   hipEvent_t a, b;
   NKX_HIP_CHECK(hipEventCreate(&a));
   NKX_HIP_CHECK(hipEventCreate(&b));
-  double best = 1e300;
-  int best_k = base;
-  const int saved = c.opt.vgemm_split;
+  const int saved_split = c.opt.vgemm_split, saved_group = c.opt.density_group;
   const bool timing = c.timing.on;
   c.timing.on = false;
+  auto timed = [&]() {
+    NKX_HIP_CHECK(hipEventRecord(a, c.stream));
+    vxc_build(c, P, xl, gga, precision, &E, &N, V);
+    NKX_HIP_CHECK(hipEventRecord(b, c.stream));
+    NKX_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    NKX_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
+  };
+  double best = 1e300;
   try {
-    c.opt.vgemm_split = cand[0];
+    c.opt.vgemm_split = base;
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
-    for (int k : cand) {
+    int best_g = bf16 ? 1 : 4;
+    if (!bf16)
+      for (int g : {1, 2, 4}) {
+        c.opt.density_group = g;
+        const double ms = timed();
+        if (ms < best) {
+          best = ms;
+          best_g = g;
+        }
+      }
+    c.opt.density_group = best_g;
+    int best_k = base;
+    best = 1e300;
+    for (int k : splits) {
       c.opt.vgemm_split = k;
-      NKX_HIP_CHECK(hipEventRecord(a, c.stream));
-      vxc_build(c, P, xl, gga, precision, &E, &N, V);
-      NKX_HIP_CHECK(hipEventRecord(b, c.stream));
-      NKX_HIP_CHECK(hipEventSynchronize(b));
-      float ms = 0.f;
-      NKX_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+      const double ms = timed();
       if (ms < best) {
         best = ms;
         best_k = k;
       }
     }
+    c.opt.vgemm_split = best_k;
+    if (best_split) *best_split = best_k;
+    if (best_group) *best_group = best_g;
   } catch (...) {
-    c.opt.vgemm_split = saved;
+    c.opt.vgemm_split = saved_split;
+    c.opt.density_group = saved_group;
     c.timing.on = timing;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -647,8 +674,6 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
   c.timing.on = timing;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  c.opt.vgemm_split = best_k;
-  if (best_split) *best_split = best_k;
   return best;
 }
 
@@ -659,7 +684,8 @@ void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int prec
   GraphState &gs = c.graph;
   gs.release();
   Plan &pl = c.plan;
-  const BuildSetup b = prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split);
+  const BuildSetup b =
+      prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split, c.opt.density_group);
   NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned), 3 * sizeof(double),
                               hipHostMallocDefault));
   gs.vtab_h.assign(1, V);
@@ -773,7 +799,7 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
   const int ig = gga ? 1 : 0;
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
-    vk_density(s, 1, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+    vk_density(s, 1, 4, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
                ws.blockEN.p, epoint);
     vkg_basis(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, ws.coef.p, cpart, tpt);
     vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);

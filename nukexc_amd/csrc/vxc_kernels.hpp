@@ -9,10 +9,11 @@ void vk_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const do
                  double *Pd);
 void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
                double *tiles);
-// epoint (may be NULL): w e_xc per sorted point
-void vk_density(hipStream_t s, int nspin, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
-                const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
-                double *coef, double *blockEN, double *epoint);
+// group: batches per workgroup (1, 2 or 4); epoint (may be NULL): w e_xc
+// per sorted point
+void vk_density(hipStream_t s, int nspin, int group, int64_t b0, int64_t b1, int64_t s0,
+                const Plan &pl, const double *Pd, int narr, int gga, double *tiles,
+                const XcList &xl, double *coef, double *blockEN, double *epoint);
 void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr,
                  double *tiles);
 // nz Z arrays (the last nz of the narr per slot), partials for Z array k at
@@ -36,7 +37,7 @@ void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const doubl
                   signed char *Ps);
 void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
                 signed char *store);
-void vkm_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+void vkm_density(hipStream_t s, int group, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN);
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,

@@ -64,9 +64,9 @@ def test_autotune_keeps_results(setup):
     c, w, P1, P2 = setup
     c.make_flat_grid(w["xyz"], w["Z"], nrad=30, nang_order=17)
     r0 = c.vxc(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
-    k, ms = c.autotune(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
+    k, g, ms = c.autotune(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
     r1 = c.vxc(torch.tensor(P1, device="cuda"), funcs=w["funcs"])
-    print(f"autotune: split {k}, {ms:.2f} ms per build")
-    assert k >= 1 and ms > 0
+    print(f"autotune: split {k}, group {g}, {ms:.2f} ms per build")
+    assert k >= 1 and g in (1, 2, 4) and ms > 0
     assert abs(r1.energy - r0.energy) <= 1e-12 * abs(r0.energy)
     assert (r1.potential - r0.potential).abs().max().item() <= 1e-13
