@@ -107,6 +107,9 @@ def main():
                     help="skip the FP64 reference build of a mixed run (profiling)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (density batch group, V split count)")
+    ap.add_argument("--launch", default=None, metavar="SPLIT,SHAPE",
+                    help="fixed launch parameters (V split count, density shape) instead "
+                         "of the autotuner")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="timed FP64 builds for the mixed-vs-FP64 speedup (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -158,7 +161,12 @@ def main():
             log(f"FP64 form: {fp64_ms:.2f} ms/build over {args.fp64_steps} builds")
 
     tuned = None
-    if not args.no_autotune:
+    if args.launch:
+        k, g = (int(x) for x in args.launch.split(","))
+        ctx.set_launch(k, g)
+        tuned = {"vgemm_split": k, "density_group": g, "ms": None}
+        log(f"launch parameters: V split {k}, density shape {g}")
+    elif not args.no_autotune:
         k, g, ms = ctx.autotune(P, funcs, precision=args.precision)
         tuned = {"vgemm_split": k, "density_group": g, "ms": ms}
         log(f"autotune: V split {k}, density group {g}, {ms:.1f} ms/build")

@@ -72,6 +72,8 @@ SIGNATURES = {
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_autotune": [_vp, _vp, _c_int, _pi, _c_int, _vp, _pi, _pi, _pd],
+    "nkx_set_launch": [_vp, _c_int, _c_int],
+    "nkx_get_launch": [_vp, _pi, _pi],
     "nkx_graph_capture": [_vp, _vp, _c_int, _pi, _c_int, _vp],
     "nkx_graph_launch": [_vp, _pd, _pd],
     "nkx_molset_create": [_vp, _c_int, _pi, _pd, _pi, _c_int, _c_int, _c_int, _c_double,

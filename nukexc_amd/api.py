@@ -298,8 +298,8 @@ class Context:
         return E.value, g.reshape(natoms, 3)
 
     def autotune(self, P, funcs=PBE, precision="fp64"):
-        """Pick the fastest density batch group and V-contraction split count
-        for this workload; returns (split, group, ms per build)."""
+        """Pick the fastest density workgroup shape and V-contraction split
+        count for this workload; returns (split, shape, ms per build)."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
         P = _dev_f64(P, self.device).contiguous()
@@ -309,6 +309,17 @@ class Context:
         self._chk(lib().nkx_autotune(self.h, _ptr(P), nf, ids, prec, _ptr(V), ctypes.byref(k),
                                      ctypes.byref(g), ctypes.byref(ms)))
         return k.value, g.value, ms.value
+
+    def set_launch(self, vgemm_split=0, density_shape=0):
+        """Set the launch parameters the autotuner picks (0 = automatic /
+        default); density shapes 1, 2, 4 (FP64) or 1, 2, 17, 18 (int8 form)."""
+        self._chk(lib().nkx_set_launch(self.h, int(vgemm_split), int(density_shape)))
+
+    def get_launch(self):
+        """Current (vgemm_split, density_shape)."""
+        k, g = ctypes.c_int(), ctypes.c_int()
+        self._chk(lib().nkx_get_launch(self.h, ctypes.byref(k), ctypes.byref(g)))
+        return k.value, g.value
 
     # -- HIP graph of one build ----------------------------------------------
     def graph_capture(self, P, V, funcs=PBE, precision="fp64"):

@@ -746,6 +746,26 @@ int nkx_autotune(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int p
   });
 }
 
+int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(vgemm_split >= 0, NKX_ERR_INVALID_ARGUMENT, "vgemm_split must be >= 0");
+    NKX_REQUIRE(density_shape == 0 || density_shape == 1 || density_shape == 2 ||
+                    density_shape == 4 || density_shape == 17 || density_shape == 18,
+                NKX_ERR_INVALID_ARGUMENT, "density shape must be 0, 1, 2, 4, 17 or 18");
+    ctx->c.opt.vgemm_split = vgemm_split;
+    ctx->c.opt.density_group = density_shape;
+  });
+}
+
+int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    if (vgemm_split) *vgemm_split = ctx->c.opt.vgemm_split;
+    if (density_shape) *density_shape = ctx->c.opt.density_group;
+  });
+}
+
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

@@ -123,10 +123,6 @@ __device__ __forceinline__ void tile_product(const v4i *A, const v4i *B, int hi[
   }
 }
 
-__device__ __forceinline__ double combine(int hi, int lo, int e) {
-  return ldexp((double)hi + ldexp((double)lo, -14), e - 19);
-}
-
 // Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65) with
 // per-row exponents ex[row] into a section at dst (fragment order); one
 // thread per (row fragment, lane) item, 256 threads.
@@ -231,38 +227,72 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 }
 
 // ---------------------------------------------------------------------------
-// Density contraction, functional and Z.  One workgroup per GBT (1, 2 or 4)
-// batches; wave
-// w owns rows 16 w .. 16 w + 15 of the current M block; every P block
-// section is read once per group and applied to each batch of the group in
-// which both blocks are live (K blocks ascending per batch).
-template <int GBT>
-__global__ void __launch_bounds__(256) density_fused_i8(
+// Direct global -> LDS copy of one section (4 planes + the 256-byte exponent
+// area) by the waves w0, w0 + nw, ... of a workgroup: 16 pieces of 1 KB
+// (global_load_lds_dwordx4: lane l writes its 16 bytes at dst + 16 l) and
+// one of 256 bytes (dword form).  The copies are tracked by vmcnt only; the
+// consumer waits with wait_vm0() before the barrier that publishes the LDS.
+__device__ __forceinline__ void dma_section(const signed char *src, signed char *dst_lds,
+                                            int w0, int nw, int lane) {
+  for (int pc = w0; pc < 17; pc += nw) {
+    if (pc < 16)
+      __builtin_amdgcn_global_load_lds(src + pc * 1024 + lane * 16, dst_lds + pc * 1024, 16, 0,
+                                       0);
+    else
+      __builtin_amdgcn_global_load_lds(src + 16384 + lane * 4, dst_lds + 16384, 4, 0, 0);
+  }
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// 2^e as a double, e in the normal range
+__device__ __forceinline__ double pow2(int e) {
+  return __longlong_as_double((long long)(e + 1023) << 52);
+}
+// 2^e, or 0 for the zero-row exponent marker -127
+__device__ __forceinline__ double pow2_or_zero(int e) { return e == -127 ? 0.0 : pow2(e); }
+
+// Density contraction, functional and Z.  A workgroup of 4 MB waves handles
+// GBT point batches.  Its steps run over (m group, k): MB consecutive entries
+// m of the union list of live function blocks, k over the same list in
+// ascending order; a step does the tile products P[m][k] Phi_k for every
+// batch of the group in which both blocks are live.  The Pm sections of
+// block k for the group's batches are staged in LDS (two buffers): the next
+// active step's sections are copied into the other buffer while the current
+// one computes, one barrier per step.
+// Wave w works on block m = w / 4 of the group, rows 16 (w % 4) .. +15; its
+// P fragment is loaded into registers one step ahead.
+template <int GBT, int MB>
+__global__ void __launch_bounds__(256 * MB) density_fused_i8(
     MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int gga,
     signed char *__restrict__ store, const double *__restrict__ pw, XcList xl,
     double *__restrict__ coef, double *__restrict__ blockEN) {
+  constexpr int NT = 256 * MB, NW = 4 * MB;
   const int64_t bg = b0 + (int64_t)blockIdx.x * GBT;
   const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int mb = wv >> 2, rw = wv & 3;
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
 
-  __shared__ double red[4][4][PT];
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
+  __shared__ double red[NW][4][PT];
   __shared__ double colsum[GBT][4][PT];
   __shared__ double cf[GBT][4][PT];
-  __shared__ double zt[FB][65];
   __shared__ int ez[FB];
   __shared__ int tab[GBT][MAXNFB];
   __shared__ int ublk[MAXNFB];
   __shared__ int nu_s;
+  static_assert(sizeof(bbuf) >= sizeof(double) * FB * 65, "zt alias");
+  double(*zt)[65] = reinterpret_cast<double(*)[65]>(&bbuf[0][0][0]);
 
-  for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
-  for (int e = threadIdx.x; e < GBT * 4 * PT; e += 256) (&colsum[0][0][0])[e] = 0.0;
+  for (int e = threadIdx.x; e < GBT * nfb; e += NT) tab[e / nfb][e % nfb] = -1;
+  for (int e = threadIdx.x; e < GBT * 4 * PT; e += NT) (&colsum[0][0][0])[e] = 0.0;
   __syncthreads();
   for (int sb = 0; sb < nsub; ++sb) {
     const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
-    for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
+    for (int k = lo + (int)threadIdx.x; k < hi; k += NT) tab[sb][b_fbl[k]] = k;
   }
   __syncthreads();
   if (wv == 0) {
@@ -281,12 +311,71 @@ __global__ void __launch_bounds__(256) density_fused_i8(
   __syncthreads();
   const int nu = nu_s;
   const int nq = gga ? 4 : 1;
+  const int ngrp = (nu + MB - 1) / MB;
+  const int total = ngrp * nu;
 
-  for (int mi = 0; mi < nu; ++mi) {
-    const int fm = ublk[mi];
-    const int mol = mt.blk_mol[fm];
-    const signed char *Pmol =
-        Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
+  // bit (m * GBT + sb): blocks m (of the group) and k both live in batch sb
+  auto pairmask = [&](int st) -> unsigned {
+    const int mg = st / nu, fk = ublk[st - mg * nu];
+    unsigned pm = 0;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const int mi = mg * MB + m;
+      if (mi < nu) {
+        const int fm = ublk[mi];
+#pragma unroll
+        for (int sb = 0; sb < GBT; ++sb)
+          if (sb < nsub && tab[sb][fk] >= 0 && tab[sb][fm] >= 0) pm |= 1u << (m * GBT + sb);
+      }
+    }
+    return pm;
+  };
+  auto next_active = [&](int st, unsigned &pm) -> int {
+    pm = 0;
+    while (st < total) {
+      pm = pairmask(st);
+      if (pm) break;
+      ++st;
+    }
+    return st;
+  };
+  auto colbit = [](unsigned pm, int sb) -> bool {
+    unsigned c = 0;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) c |= pm >> (m * GBT + sb);
+    return c & 1u;
+  };
+  constexpr unsigned ROW = (1u << GBT) - 1u;
+
+  v4i An[4];
+  int eAn = 0;
+  // Pm sections of block k for the live batches -> LDS buffer dbuf (direct
+  // copies), and this wave's P fragment of block pair (m, k) -> registers
+  auto issue = [&](int st, unsigned pm, int dbuf) {
+    const int mg = st / nu, fk = ublk[st - mg * nu];
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!colbit(pm, sb)) continue;
+      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                  (wv + 5 * sb) % NW, NW, lane);
+    }
+    if ((pm >> (mb * GBT)) & ROW) {
+      const int fm = ublk[mg * MB + mb];
+      const int mol = mt.blk_mol[fm];
+      const signed char *pa =
+          Ps + mt.Ps_off[mol] +
+          ((int64_t)mt.blk_local[fm] * mt.nfb[mol] + mt.blk_local[fk]) * SEC;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
+      eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
+    }
+  };
+
+  unsigned pmc = 0;
+  int cur = next_active(0, pmc);
+  if (cur < total) issue(cur, pmc, 0);
+  int buf = 0;
+  for (int mg = 0; mg < ngrp; ++mg) {
     double x64[GBT][4][4];
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb)
@@ -294,70 +383,81 @@ __global__ void __launch_bounds__(256) density_fused_i8(
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) x64[sb][j][r] = 0.0;
-    for (int ki = 0; ki < nu; ++ki) {
-      const int fk = ublk[ki];
-      bool any = false;
-      for (int sb = 0; sb < nsub; ++sb) any = any || (tab[sb][fk] >= 0 && tab[sb][fm] >= 0);
-      if (!any) continue;
-      const signed char *pa = Pmol + (int64_t)mt.blk_local[fk] * SEC;  // same molecule
+    while (cur < total && cur / nu == mg) {
+      wait_vm0();
+      __syncthreads();  // buffer buf complete, buf ^ 1 free
       v4i A[4];
 #pragma unroll
-      for (int p = 0; p < 4; ++p) A[p] = ld16(pa + p * PLANE + (wv * 64 + lane) * 16);
-      int eA[4];
+      for (int p = 0; p < 4; ++p) A[p] = An[p];
+      const int eAc = eAn;
+      unsigned pmn = 0;
+      const int nxt = next_active(cur + 1, pmn);
+      if (nxt < total) issue(nxt, pmn, buf ^ 1);
+      double sA[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) eA[r] = pa[4 * PLANE + 16 * wv + 4 * q + r];
+      for (int r = 0; r < 4; ++r) sA[r] = pow2_or_zero((int)(signed char)(eAc >> (8 * r)));
 #pragma unroll
       for (int sb = 0; sb < GBT; ++sb) {
-        if (sb >= nsub) break;
-        const int slk = tab[sb][fk];
-        if (slk < 0 || tab[sb][fm] < 0) continue;
-        const signed char *pb = store + (slk - s0) * sb_bytes + OFF_PM;
+        if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
+        const signed char *bb = &bbuf[buf][sb][0];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           v4i B[4];
 #pragma unroll
-          for (int p = 0; p < 4; ++p) B[p] = ld16(pb + p * PLANE + (j * 64 + lane) * 16);
-          const int eB = pb[4 * PLANE + 16 * j + l16];
+          for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+          const int eB = bb[4 * PLANE + 16 * j + l16];
+          const double sB = eB == -127 ? 0.0 : pow2(eB - 19);
           int hi[4], lo[4];
           tile_product(A, B, hi, lo);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if (eA[r] == -127 || eB == -127) continue;
-            x64[sb][j][r] += combine(hi[r], lo[r], eA[r] + eB);
-          }
+          for (int r = 0; r < 4; ++r)
+            x64[sb][j][r] =
+                fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[r] * sB, x64[sb][j][r]);
         }
       }
+      buf ^= 1;
+      cur = nxt;
+      pmc = pmn;
     }
-    // Epilogue in FP64: column partial sums of Phi X (and grad Phi X).
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (sb >= nsub) break;
-      const int slm = tab[sb][fm];
-      if (slm < 0) continue;
-      const signed char *tb = store + (slm - s0) * sb_bytes;
-      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+    // Epilogue in FP64: column partial sums of Phi X (and grad Phi X) of
+    // every block of the group, summed over waves in a fixed order.
+    const int mi = mg * MB + mb;
+    const int fm = mi < nu ? ublk[mi] : -1;
+    for (int sb = 0; sb < nsub; ++sb) {
+      const int slm = fm >= 0 ? tab[sb][fm] : -1;
       double part[4][4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int g = 16 * j + l16;
-        double sm[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 16 * wv + 4 * q + r;
-          const double x = x64[sb][j][r];
-          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
-          if (gga) {
-            sm[1] += (double)G[mu * PT + g] * x;
-            sm[2] += (double)G[TILE + mu * PT + g] * x;
-            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+        for (int c = 0; c < 4; ++c) part[j][c] = 0.0;
+      if (slm >= 0) {
+        const signed char *tb = store + (slm - s0) * sb_bytes;
+        const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int g = 16 * j + l16;
+          double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mu = 16 * rw + 4 * q + r;
+            double x = 0.0;
+#pragma unroll
+            for (int s2 = 0; s2 < GBT; ++s2)
+              if (s2 == sb) x = x64[s2][j][r];
+            sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+            if (gga) {
+              sm[1] += (double)G[mu * PT + g] * x;
+              sm[2] += (double)G[TILE + mu * PT + g] * x;
+              sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+            }
           }
-        }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = sm[c];
-          v += __shfl_xor(v, 16);
-          v += __shfl_xor(v, 32);
-          part[j][c] = v;
+          for (int c = 0; c < 4; ++c) {
+            double v = sm[c];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            part[j][c] = v;
+          }
         }
       }
       if (q == 0) {
@@ -366,10 +466,14 @@ __global__ void __launch_bounds__(256) density_fused_i8(
           for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
       }
       __syncthreads();
-      {
-        const int c = threadIdx.x / PT, g = threadIdx.x % PT;
-        if (c < nq)
-          colsum[sb][c][g] += red[0][c][g] + red[1][c][g] + red[2][c][g] + red[3][c][g];
+      for (int e = threadIdx.x; e < 4 * PT; e += NT) {
+        const int c = e / PT, g = e % PT;
+        if (c < nq) {
+          double acc = red[0][c][g];
+#pragma unroll
+          for (int w = 1; w < NW; ++w) acc += red[w][c][g];
+          colsum[sb][c][g] += acc;
+        }
       }
       __syncthreads();
     }
@@ -417,7 +521,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
       if (slm < 0) continue;
       signed char *tb = store + (slm - s0) * sb_bytes;
       const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-      for (int e = threadIdx.x; e < TILE; e += 256) {
+      for (int e = threadIdx.x; e < TILE; e += NT) {
         const int f = e / PT, g = e % PT;
         double zz = cf[sb][0][g] * unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
         if (gga)
@@ -432,7 +536,7 @@ __global__ void __launch_bounds__(256) density_fused_i8(
         ez[threadIdx.x] = row_exp(m);
       }
       __syncthreads();
-      store_section(zt, ez, tb + OFF_Z, false);
+      if (threadIdx.x < 256) store_section(zt, ez, tb + OFF_Z, false);
       __syncthreads();
     }
   }
@@ -440,7 +544,13 @@ __global__ void __launch_bounds__(256) density_fused_i8(
 
 // ---------------------------------------------------------------------------
 // acc(I, J) += Phi_I Z_J^T + Z_I Phi_J^T over batches where I and J are live.
-// Wave w owns rows 16 w .. 16 w + 15 of block I and all 64 columns of block J.
+// One workgroup per (block pair, split); its batches b = lo + split + nsplit i
+// are taken in windows of 256 candidates, compacted in ascending order into
+// an LDS list of (slot of I, slot of J).  The Z and F sections of J's slot
+// (shared by the four waves) are staged in LDS, two buffers: the next
+// batch's are copied into the other buffer while the current one computes.  Wave w
+// owns rows 16 w .. 16 w + 15 of block I (its F and Z fragments loaded one
+// batch ahead) and all 64 columns of block J.
 __global__ void __launch_bounds__(256) vgemm_i8(
     int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
     const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1, int64_t s0,
@@ -451,6 +561,9 @@ __global__ void __launch_bounds__(256) vgemm_i8(
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][2][SEC];  // [buffer][Z_J, F_J]
+  __shared__ int lsI[256], lsJ[256];
+  __shared__ int wcnt[4];
   double v64[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j)
@@ -462,41 +575,90 @@ __global__ void __launch_bounds__(256) vgemm_i8(
   const int *tsI = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[I] * mnb - mb0;
   const int *tsJ = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[J] * mnb - mb0;
   const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
-  for (int64_t b = lo + split; b < hi; b += nsplit) {
-    const int sI = tsI[b], sJ = tsJ[b];
-    if (sI < 0 || sJ < 0) continue;
-    const signed char *tI = store + (sI - s0) * sb_bytes;
-    const signed char *tJ = store + (sJ - s0) * sb_bytes;
-    v4i AF[4], AZ[4];
+
+  v4i AnF[4], AnZ[4];
+  int eFn = 0, eZn = 0;
+  // Z and F sections of J's slot -> LDS buffer dbuf (direct copies), this
+  // wave's F and Z fragments of I's slot -> registers
+  auto issue = [&](int i, int dbuf) {
+    const signed char *tI = store + (lsI[i] - s0) * sb_bytes;
+    const signed char *tJ = store + (lsJ[i] - s0) * sb_bytes;
+    dma_section(tJ + OFF_Z, &bbuf[dbuf][0][0], wv, 4, lane);
+    dma_section(tJ + OFF_F, &bbuf[dbuf][1][0], (wv + 3) & 3, 4, lane);
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) {
-      AF[pl] = ld16(tI + OFF_F + pl * PLANE + (wv * 64 + lane) * 16);
-      AZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (wv * 64 + lane) * 16);
+      AnF[pl] = ld16(tI + OFF_F + pl * PLANE + (wv * 64 + lane) * 16);
+      AnZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (wv * 64 + lane) * 16);
     }
-    int eFI[4], eZI[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      eFI[r] = tI[OFF_F + 4 * PLANE + 16 * wv + 4 * q + r];
-      eZI[r] = tI[OFF_Z + 4 * PLANE + 16 * wv + 4 * q + r];
+    eFn = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * wv + 4 * q);
+    eZn = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * wv + 4 * q);
+  };
+
+  for (int64_t c0 = lo + split; c0 < hi; c0 += 256 * (int64_t)nsplit) {
+    const int64_t b = c0 + (int64_t)nsplit * threadIdx.x;
+    int sI = -1, sJ = -1;
+    if (b < hi) {
+      sI = tsI[b];
+      sJ = tsJ[b];
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      v4i BZ[4], BF[4];
+    const bool act = sI >= 0 && sJ >= 0;
+    const unsigned long long m = __ballot(act);
+    __syncthreads();  // the previous window is done with the list and buffers
+    if (lane == 0) wcnt[wv] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wv; ++w) off += wcnt[w];
+    const int n = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    if (act) {
+      const int i = off + __popcll(m & ((1ull << lane) - 1ull));
+      lsI[i] = sI;
+      lsJ[i] = sJ;
+    }
+    __syncthreads();
+    if (n == 0) continue;
+    issue(0, 0);
+    int buf = 0;
+    for (int i = 0; i < n; ++i) {
+      wait_vm0();
+      __syncthreads();  // buffer buf complete, buf ^ 1 free
+      v4i AF[4], AZ[4];
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) {
-        BZ[pl] = ld16(tJ + OFF_Z + pl * PLANE + (j * 64 + lane) * 16);
-        BF[pl] = ld16(tJ + OFF_F + pl * PLANE + (j * 64 + lane) * 16);
+        AF[pl] = AnF[pl];
+        AZ[pl] = AnZ[pl];
       }
-      const int eZJ = tJ[OFF_Z + 4 * PLANE + 16 * j + l16];
-      const int eFJ = tJ[OFF_F + 4 * PLANE + 16 * j + l16];
-      int h1[4], o1[4], h2[4], o2[4];
-      tile_product(AF, BZ, h1, o1);
-      tile_product(AZ, BF, h2, o2);
+      const int eFc = eFn, eZc = eZn;
+      if (i + 1 < n) issue(i + 1, buf ^ 1);
+      double sF[4], sZ[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        if (eFI[r] != -127 && eZJ != -127) v64[j][r] += combine(h1[r], o1[r], eFI[r] + eZJ);
-        if (eZI[r] != -127 && eFJ != -127) v64[j][r] += combine(h2[r], o2[r], eZI[r] + eFJ);
+        sF[r] = pow2_or_zero((int)(signed char)(eFc >> (8 * r)));
+        sZ[r] = pow2_or_zero((int)(signed char)(eZc >> (8 * r)));
       }
+      const signed char *bz = &bbuf[buf][0][0];
+      const signed char *bf = &bbuf[buf][1][0];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4i BZ[4], BF[4];
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+          BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
+          BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
+        }
+        const int eZJ = bz[4 * PLANE + 16 * j + l16];
+        const int eFJ = bf[4 * PLANE + 16 * j + l16];
+        const double sZJ = eZJ == -127 ? 0.0 : pow2(eZJ - 19);
+        const double sFJ = eFJ == -127 ? 0.0 : pow2(eFJ - 19);
+        int h1[4], o1[4], h2[4], o2[4];
+        tile_product(AF, BZ, h1, o1);
+        tile_product(AZ, BF, h2, o2);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v64[j][r] = fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[j][r]);
+          v64[j][r] = fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[j][r]);
+        }
+      }
+      buf ^= 1;
     }
   }
   double *out = vpart + ((int64_t)split * npairs + p) * (FB * FB);
@@ -528,27 +690,31 @@ void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-template <int GBT>
+template <int GBT, int MB>
 static void launch_density_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                               const signed char *Ps, int gga, signed char *store,
                               const XcList &xl, double *coef, double *blockEN) {
   const unsigned nwg = (unsigned)((b1 - b0 + GBT - 1) / GBT);
-  hipLaunchKernelGGL(density_fused_i8<GBT>, dim3(nwg), dim3(256), 0, s, pl.tab(), pl.nfb, Ps,
-                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl, coef, blockEN);
+  hipLaunchKernelGGL((density_fused_i8<GBT, MB>), dim3(nwg), dim3(256 * MB), 0, s, pl.tab(),
+                     pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl,
+                     coef, blockEN);
 }
 
-void vkm_density(hipStream_t s, int group, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+// shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1);
+// other values take 2
+void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
-  if (group == 1)
-    launch_density_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
-  else if (group == 2)
-    launch_density_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
-  else
-    launch_density_i8<4>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN);
+  switch (shape) {
+    case 1: launch_density_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+    case 2: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+    case 17: launch_density_i8<1, 2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+    case 18: launch_density_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+    default: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+  }
   NKX_HIP_CHECK(hipGetLastError());
 }
 

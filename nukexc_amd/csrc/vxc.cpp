@@ -639,9 +639,13 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
   try {
     c.opt.vgemm_split = base;
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
-    int best_g = bf16 ? 1 : 4;
+    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 2 : 4);
+    // density shapes: batches per workgroup (+ 16 for two function blocks
+    // per step in the int8 form)
+    const std::vector<int> shapes =
+        precision == NKX_PREC_MIXED ? std::vector<int>{1, 2, 17, 18} : std::vector<int>{1, 2, 4};
     if (!bf16)
-      for (int g : {1, 2, 4}) {
+      for (int g : shapes) {
         c.opt.density_group = g;
         const double ms = timed();
         if (ms < best) {

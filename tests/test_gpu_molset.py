@@ -34,13 +34,20 @@ def setup():
     c.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_molset_matches_single_builds(setup, precision):
+@pytest.mark.parametrize("precision,shape", [("fp64", 0), ("mixed", 0), ("mixed", 1),
+                                             ("mixed", 17), ("mixed", 18)])
+def test_molset_matches_single_builds(setup, precision, shape):
+    """shape: the density workgroup shape of the set build (the single builds
+    use the default); 17 and 18 put blocks of two molecules in one step."""
     c, mols, Ps, singles = setup
     npts = c.molset_create(mols, **GRID)
     assert np.all(npts > 0)
-    res = c.molset_vxc([torch.tensor(P, device="cuda") for P in Ps], funcs=(101, 130),
-                       precision=precision)
+    c.set_launch(0, shape)
+    try:
+        res = c.molset_vxc([torch.tensor(P, device="cuda") for P in Ps], funcs=(101, 130),
+                           precision=precision)
+    finally:
+        c.set_launch(0, 0)
     worst = 0.0
     for k, (r, one) in enumerate(zip(res, singles)):
         ref = one[precision]
@@ -50,7 +57,8 @@ def test_molset_matches_single_builds(setup, precision):
         assert abs(r.n_el - ref.n_el) <= 1e-11 * ref.n_el, k
         assert dV <= 1e-12, (k, dV)
         assert abs(r.n_el - 2 * mols[k]["nocc"]) < 1e-6
-    print(f"{precision}: {len(mols)} molecules, {npts.sum()} points, worst |dV| {worst:.1e}")
+    print(f"{precision} shape {shape}: {len(mols)} molecules, {npts.sum()} points, "
+          f"worst |dV| {worst:.1e}")
 
 
 def test_molset_rejects_split_bf16(setup):

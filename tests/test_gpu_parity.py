@@ -358,6 +358,42 @@ def test_vxc_mixed_no_density_bias(ctx):
     assert dV <= 1e-7
 
 
+def test_vxc_mixed_density_shapes(ctx):
+    """The int8 density kernel's workgroup shapes on an 8-water cluster
+    (several function blocks per batch): one or two batches per workgroup
+    give bitwise the same build (same per-batch K order and sums); two
+    function blocks per step change only the order of the FP64 column sums.
+    Out-of-range shapes are rejected by nkx_set_launch."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    xyz, Z, _ = configs.water_cluster(8, rng)
+    basis = nk.make_manual_basis(configs.sto_functions(Z, xyz, rng))
+    ctx.make_flat_grid(xyz, Z, nrad=40, nang_order=23)
+    ctx.set_basis(basis)
+    S = ctx.overlap().cpu().numpy()
+    C, P = configs.loewdin_orbitals(S, 40, rng)
+    res = {}
+    try:
+        for shape in (2, 1, 17, 18):
+            ctx.set_launch(0, shape)
+            assert ctx.get_launch() == (0, shape)
+            r = ctx.vxc(dev(P), funcs=(101, 130), precision="mixed")
+            res[shape] = (r.energy, r.n_el, r.potential.cpu().numpy())
+            print(f"shape {shape}: E={r.energy:.12f} N={r.n_el:.12f}")
+        with pytest.raises(RuntimeError):
+            ctx.set_launch(0, 3)
+        with pytest.raises(RuntimeError):
+            ctx.set_launch(-1, 0)
+    finally:
+        ctx.set_launch(0, 0)
+    E2, N2, V2 = res[2]
+    assert res[1][0] == E2 and res[1][1] == N2 and np.array_equal(res[1][2], V2)
+    for shape in (17, 18):
+        E, N, V = res[shape]
+        assert abs(E - E2) <= 1e-11 * abs(E2)
+        assert abs(N - N2) <= 1e-11 * N2
+        assert np.abs(V - V2).max() <= 1e-12
+
+
 def test_mfma_bf16_accumulate_rounding():
     """Records how v_mfma_f32_16x16x32_bf16 rounds C + a*b into FP32 (the
     mixed form bounds FP32 accumulation to 64-element blocks either way)."""
