@@ -220,6 +220,7 @@ int nkx_radial_scale(int scheme, int Z, double *R) {
   });
 }
 
+extern "C++" {
 namespace {
 // make_flat_grid (grid.hpp:172-348) into out: TA / Becke radial x Lebedev per
 // atom, capped Becke partition, compaction of w > w_thr.
@@ -302,6 +303,7 @@ nkx::BasisHost make_basis(int nbf, const int *n, const int *l, const int *m, con
   return B;
 }
 }  // namespace
+}  // extern "C++"
 
 int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z, int nrad,
                        int nang_order, int radial_scheme, double ta_alpha, double w_thr,
@@ -568,6 +570,7 @@ int nkx_functional_eval(nkx_ctx *ctx, int nfunc, const int *ids, int64_t npts,
   });
 }
 
+extern "C++" {
 namespace {
 // compute_lda / compute_gga family checks (xc_integrals.hpp:85-91, :437-447)
 void family_require(int family_check, int nfunc, const int *ids, bool gga) {
@@ -586,6 +589,7 @@ void family_require(int family_check, int nfunc, const int *ids, bool gga) {
   }
 }
 }  // namespace
+}  // extern "C++"
 
 int nkx_functional_eval_polarized(nkx_ctx *ctx, int nfunc, const int *ids, int64_t npts,
                                   const double *rho, const double *sigma, double *exc,
@@ -750,9 +754,11 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
     NKX_REQUIRE(vgemm_split >= 0, NKX_ERR_INVALID_ARGUMENT, "vgemm_split must be >= 0");
-    NKX_REQUIRE(density_shape == 0 || density_shape == 1 || density_shape == 2 ||
-                    density_shape == 4 || density_shape == 17 || density_shape == 18,
-                NKX_ERR_INVALID_ARGUMENT, "density shape must be 0, 1, 2, 4, 17 or 18");
+    const int sh = density_shape;
+    NKX_REQUIRE(sh == 0 || sh == 1 || sh == 2 || sh == 4 || sh == 17 || sh == 18 || sh == 33 ||
+                    sh == 34 || sh == 49 || sh == 50,
+                NKX_ERR_INVALID_ARGUMENT,
+                "density shape must be 0, 1, 2, 4, 17, 18, 33, 34, 49 or 50");
     ctx->c.opt.vgemm_split = vgemm_split;
     ctx->c.opt.density_group = density_shape;
   });

@@ -543,76 +543,398 @@ __global__ void __launch_bounds__(256 * MB) density_fused_i8(
 }
 
 // ---------------------------------------------------------------------------
-// acc(I, J) += Phi_I Z_J^T + Z_I Phi_J^T over batches where I and J are live.
-// One workgroup per (block pair, split); its batches b = lo + split + nsplit i
-// are taken in windows of 256 candidates, compacted in ascending order into
-// an LDS list of (slot of I, slot of J).  The Z and F sections of J's slot
-// (shared by the four waves) are staged in LDS, two buffers: the next
-// batch's are copied into the other buffer while the current one computes.  Wave w
-// owns rows 16 w .. 16 w + 15 of block I (its F and Z fragments loaded one
-// batch ahead) and all 64 columns of block J.
-__global__ void __launch_bounds__(256) vgemm_i8(
-    int npairs, int nsplit, const int *__restrict__ pair_I, const int *__restrict__ pair_J,
-    const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1, int64_t s0,
-    int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
-  const int p = blockIdx.x % npairs;
-  const int split = blockIdx.x / npairs;
-  const int I = pair_I[p], J = pair_J[p];
+// Split form of the density stage: contraction (density_x_i8), reduction and
+// functional (density_f_i8), Z slices (z_slots_i8).
+//
+// density_x_i8: one workgroup of 4 MB waves per (batch group g of GBT
+// batches, m group i of MB blocks of the group's union list).  Workgroup w
+// takes g = 8 (w / 8 / nmg) + w % 8 and i = (w / 8) % nmg, so the m groups of
+// one batch group run on one XCD (workgroups are dealt to the eight XCDs in
+// turn) and share its L2 for the Pm sections.  The step loop is the fused
+// kernel's over k for one m group; the epilogue writes, per live slot
+// (m, batch), the column sums of Phi X (and grad Phi X) over the block's 64
+// functions, summed over the four waves of the block in order, to part.
+template <int GBT, int MB>
+__global__ void __launch_bounds__(256 * MB) density_x_i8(
+    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
+    const signed char *__restrict__ store, double *__restrict__ part) {
+  constexpr int NT = 256 * MB, NW = 4 * MB;
+  const int xcd = blockIdx.x & 7;
+  const int rr = blockIdx.x >> 3;
+  const int mgi = rr % nmg;
+  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
+  if (bg >= b1) return;
+  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int mb = wv >> 2, rw = wv & 3;
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][2][SEC];  // [buffer][Z_J, F_J]
-  __shared__ int lsI[256], lsJ[256];
-  __shared__ int wcnt[4];
-  double v64[4][4];
+
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
+  __shared__ double red[NW][4][PT];
+  __shared__ int tab[GBT][MAXNFB];
+  __shared__ int ublk[MAXNFB];
+  __shared__ int nu_s;
+
+  for (int e = threadIdx.x; e < GBT * nfb; e += NT) tab[e / nfb][e % nfb] = -1;
+  __syncthreads();
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
+    for (int k = lo + (int)threadIdx.x; k < hi; k += NT) tab[sb][b_fbl[k]] = k;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nfb; base += 64) {
+      const int f = base + lane;
+      bool live = false;
+      if (f < nfb)
+        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
+      const unsigned long long m = __ballot(live);
+      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nu_s = cnt;
+  }
+  __syncthreads();
+  const int nu = nu_s;
+  if (mgi * MB >= nu) return;  // uniform: no block of this m group
+  const int nq = gga ? 4 : 1;
+
+  // bit (m * GBT + sb): block m of the group and block k = ublk[ki] both live in sb
+  auto pairmask = [&](int ki) -> unsigned {
+    const int fk = ublk[ki];
+    unsigned pm = 0;
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+    for (int m = 0; m < MB; ++m) {
+      const int mi = mgi * MB + m;
+      if (mi < nu) {
+        const int fm = ublk[mi];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v64[j][r] = 0.0;
-  // batches of the pair's molecule inside [b0, b1)
-  const int mol = mt.blk_mol[I];
+        for (int sb = 0; sb < GBT; ++sb)
+          if (sb < nsub && tab[sb][fk] >= 0 && tab[sb][fm] >= 0) pm |= 1u << (m * GBT + sb);
+      }
+    }
+    return pm;
+  };
+  auto next_active = [&](int ki, unsigned &pm) -> int {
+    pm = 0;
+    while (ki < nu) {
+      pm = pairmask(ki);
+      if (pm) break;
+      ++ki;
+    }
+    return ki;
+  };
+  auto colbit = [](unsigned pm, int sb) -> bool {
+    unsigned c = 0;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) c |= pm >> (m * GBT + sb);
+    return c & 1u;
+  };
+  constexpr unsigned ROW = (1u << GBT) - 1u;
+
+  v4i An[4];
+  int eAn = 0;
+  auto issue = [&](int ki, unsigned pm, int dbuf) {
+    const int fk = ublk[ki];
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!colbit(pm, sb)) continue;
+      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                  (wv + 5 * sb) % NW, NW, lane);
+    }
+    if ((pm >> (mb * GBT)) & ROW) {
+      const int fm = ublk[mgi * MB + mb];
+      const int mol = mt.blk_mol[fm];
+      const signed char *pa =
+          Ps + mt.Ps_off[mol] +
+          ((int64_t)mt.blk_local[fm] * mt.nfb[mol] + mt.blk_local[fk]) * SEC;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
+      eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
+    }
+  };
+
+  double x64[GBT][4][4];
+#pragma unroll
+  for (int sb = 0; sb < GBT; ++sb)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x64[sb][j][r] = 0.0;
+  unsigned pmc = 0;
+  int cur = next_active(0, pmc);
+  if (cur < nu) issue(cur, pmc, 0);
+  int buf = 0;
+  while (cur < nu) {
+    wait_vm0();
+    __syncthreads();  // buffer buf complete, buf ^ 1 free
+    v4i A[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[p] = An[p];
+    const int eAc = eAn;
+    unsigned pmn = 0;
+    const int nxt = next_active(cur + 1, pmn);
+    if (nxt < nu) issue(nxt, pmn, buf ^ 1);
+    double sA[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sA[r] = pow2_or_zero((int)(signed char)(eAc >> (8 * r)));
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
+      const signed char *bb = &bbuf[buf][sb][0];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4i B[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+        const int eB = bb[4 * PLANE + 16 * j + l16];
+        const double sB = eB == -127 ? 0.0 : pow2(eB - 19);
+        int hi[4], lo[4];
+        tile_product(A, B, hi, lo);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          x64[sb][j][r] =
+              fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[r] * sB, x64[sb][j][r]);
+      }
+    }
+    buf ^= 1;
+    cur = nxt;
+    pmc = pmn;
+  }
+
+  // Epilogue: per live slot (m, sb), column sums over the block's functions.
+  const int mi = mgi * MB + mb;
+  const int fm = mi < nu ? ublk[mi] : -1;
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int slm = fm >= 0 ? tab[sb][fm] : -1;
+    double prt[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
+    if (slm >= 0) {
+      const signed char *tb = store + (slm - s0) * sb_bytes;
+      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int g = 16 * j + l16;
+        double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = 16 * rw + 4 * q + r;
+          double x = 0.0;
+#pragma unroll
+          for (int s2 = 0; s2 < GBT; ++s2)
+            if (s2 == sb) x = x64[s2][j][r];
+          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+          if (gga) {
+            sm[1] += (double)G[mu * PT + g] * x;
+            sm[2] += (double)G[TILE + mu * PT + g] * x;
+            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = sm[c];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          prt[j][c] = v;
+        }
+      }
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
+    }
+    __syncthreads();
+    // waves 4 m .. 4 m + 3 hold block m: write its slot's partial sums
+    for (int e = threadIdx.x; e < MB * 4 * PT; e += NT) {
+      const int m = e / (4 * PT), c = (e / PT) % 4, g = e % PT;
+      const int mi2 = mgi * MB + m;
+      if (mi2 >= nu) continue;
+      const int sl = tab[sb][ublk[mi2]];
+      if (sl < 0) continue;
+      double acc = 0.0;
+      if (c < nq) acc = red[4 * m][c][g] + red[4 * m + 1][c][g] + red[4 * m + 2][c][g] +
+                        red[4 * m + 3][c][g];
+      part[(sl - s0) * (4 * PT) + c * PT + g] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Per batch: column sums of the live slots in block order (the fused
+// kernel's order), functional, Z coefficients, (E, N) partial sums.
+__global__ void __launch_bounds__(256) density_f_i8(
+    int64_t s0, int64_t b0, const int *__restrict__ b_off, int gga,
+    const double *__restrict__ part, const double *__restrict__ pw, XcList xl,
+    double *__restrict__ coef, double *__restrict__ blockEN) {
+  const int64_t b = b0 + blockIdx.x;
+  __shared__ double cs[4][PT];
+  const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+  const int lo = b_off[b], hi = b_off[b + 1];
+  double acc = 0.0;
+  for (int k = lo; k < hi; ++k) acc += part[(k - s0) * (4 * PT) + c * PT + g];
+  cs[c][g] = acc;
+  __syncthreads();
+  if (threadIdx.x < PT) {
+    const int64_t gp = b * PT + g;
+    const double w = pw[gp];
+    const double rho = cs[0][g];
+    const double gxr = 2.0 * cs[1][g], gyr = 2.0 * cs[2][g], gzr = 2.0 * cs[3][g];
+    const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
+    const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
+    const double a = 0.5 * w * o.vrho;
+    const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
+    coef[4 * gp + 0] = a;
+    coef[4 * gp + 1] = bs * gxr;
+    coef[4 * gp + 2] = bs * gyr;
+    coef[4 * gp + 3] = bs * gzr;
+    double e = w * o.e, n = w * rho;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      e += __shfl_down(e, o2);
+      n += __shfl_down(n, o2);
+    }
+    if (g == 0) {
+      blockEN[3 * b] = e;
+      blockEN[3 * b + 1] = n;
+      blockEN[3 * b + 2] = 0.0;
+    }
+  }
+}
+
+// Z = a Phi + b . grad Phi of one live slot in FP64, per-function exponents,
+// slices (F order); one workgroup per slot.
+__global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restrict__ slot_b,
+                                                  int gga, const double *__restrict__ coef,
+                                                  signed char *__restrict__ store) {
+  const int64_t sl = s0 + blockIdx.x;
+  const int64_t b = slot_b[sl];
+  __shared__ double cf[4][PT];
+  __shared__ double zt[FB][65];
+  __shared__ int ez[FB];
+  {
+    const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+    cf[c][g] = coef[4 * (b * PT + g) + c];
+  }
+  __syncthreads();
+  signed char *tb = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
+  const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+  for (int e = threadIdx.x; e < TILE; e += 256) {
+    const int f = e / PT, g = e % PT;
+    double zz = cf[0][g] * unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
+    if (gga)
+      zz += cf[1][g] * (double)G[e] + cf[2][g] * (double)G[TILE + e] +
+            cf[3][g] * (double)G[2 * TILE + e];
+    zt[f][g] = zz;
+  }
+  __syncthreads();
+  if (threadIdx.x < FB) {
+    double m = 0.0;
+    for (int k = 0; k < PT; ++k) m = fmax(m, fabs(zt[threadIdx.x][k]));
+    ez[threadIdx.x] = row_exp(m);
+  }
+  __syncthreads();
+  store_section(zt, ez, tb + OFF_Z, false);
+}
+
+// ---------------------------------------------------------------------------
+// V contraction on 2 x 2 block super-tiles: acc(I, J) += Phi_I Z_J^T +
+// Z_I Phi_J^T over the batches where I and J are live, for the four tiles
+// (I_a, J_b) of a super-pair (blocks I_0 < I_1, J_0 < J_1 of one molecule;
+// a missing second block is -1; on the diagonal I_1 x J_0 is the transpose
+// of I_0 x J_1 and is skipped).  One workgroup of eight waves per
+// (super-pair, split); its batches b = lo + split + nsplit i are taken in
+// windows of 512 candidates, compacted in ascending order into an LDS list
+// of the four slots.  The Z and F sections of the J slots (shared by all
+// waves) are staged in LDS, two buffers: the next batch's are copied in
+// while the current one computes.  Wave w owns rows 16 (w % 4) .. +15 of
+// block I_(w / 4) (its F and Z fragments loaded one batch ahead) and all 128
+// columns of J_0, J_1.  Each tile is added into vpart at its pair index.
+__global__ void __launch_bounds__(512) vgemm_i8(
+    int nsp, int nsplit, const int4 *__restrict__ sp_blk, const int4 *__restrict__ sp_pidx,
+    int npairs, const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1,
+    int64_t s0, int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
+  const int sp = blockIdx.x % nsp;
+  const int split = blockIdx.x / nsp;
+  const int4 blk = sp_blk[sp];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int half = wv >> 2, rw = wv & 3;
+  const int l16 = lane & 15, q = lane >> 4;
+  const bool diag = blk.x == blk.z;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][4][SEC];  // Z_J0 F_J0 Z_J1 F_J1
+  __shared__ int ls[4][512];  // slots of I_0, I_1, J_0, J_1 per listed batch
+  __shared__ int wcnt[8];
+  double v64[2][4][4];
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v64[jb][j][r] = 0.0;
+  // batches of the super-pair's molecule inside [b0, b1)
+  const int mol = mt.blk_mol[blk.x];
   const int64_t mb0 = mt.b0[mol], mnb = mt.nb[mol];
-  const int *tsI = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[I] * mnb - mb0;
-  const int *tsJ = tile_slot + mt.ts_off[mol] + (int64_t)mt.blk_local[J] * mnb - mb0;
+  const int *ts0 = tile_slot + mt.ts_off[mol] - mb0;
+  const int bl[4] = {blk.x, blk.y, blk.z, blk.w};
+  int64_t tso[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) tso[u] = bl[u] >= 0 ? (int64_t)mt.blk_local[bl[u]] * mnb : -1;
   const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
 
   v4i AnF[4], AnZ[4];
   int eFn = 0, eZn = 0;
-  // Z and F sections of J's slot -> LDS buffer dbuf (direct copies), this
-  // wave's F and Z fragments of I's slot -> registers
+  // J sections -> LDS buffer dbuf (direct copies); this wave's F and Z
+  // fragments of its I slot -> registers
   auto issue = [&](int i, int dbuf) {
-    const signed char *tI = store + (lsI[i] - s0) * sb_bytes;
-    const signed char *tJ = store + (lsJ[i] - s0) * sb_bytes;
-    dma_section(tJ + OFF_Z, &bbuf[dbuf][0][0], wv, 4, lane);
-    dma_section(tJ + OFF_F, &bbuf[dbuf][1][0], (wv + 3) & 3, 4, lane);
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) {
-      AnF[pl] = ld16(tI + OFF_F + pl * PLANE + (wv * 64 + lane) * 16);
-      AnZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (wv * 64 + lane) * 16);
+    for (int jb = 0; jb < 2; ++jb) {
+      const int sJ = ls[2 + jb][i];
+      if (sJ < 0) continue;
+      const signed char *tJ = store + (sJ - s0) * sb_bytes;
+      dma_section(tJ + OFF_Z, &bbuf[dbuf][2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
+      dma_section(tJ + OFF_F, &bbuf[dbuf][2 * jb + 1][0], (wv + 4 * jb + 2) & 7, 8, lane);
     }
-    eFn = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * wv + 4 * q);
-    eZn = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * wv + 4 * q);
+    const int sI = ls[half][i];
+    if (sI >= 0) {
+      const signed char *tI = store + (sI - s0) * sb_bytes;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) {
+        AnF[pl] = ld16(tI + OFF_F + pl * PLANE + (rw * 64 + lane) * 16);
+        AnZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (rw * 64 + lane) * 16);
+      }
+      eFn = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * rw + 4 * q);
+      eZn = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * rw + 4 * q);
+    }
   };
 
-  for (int64_t c0 = lo + split; c0 < hi; c0 += 256 * (int64_t)nsplit) {
+  for (int64_t c0 = lo + split; c0 < hi; c0 += 512 * (int64_t)nsplit) {
     const int64_t b = c0 + (int64_t)nsplit * threadIdx.x;
-    int sI = -1, sJ = -1;
+    int sl[4] = {-1, -1, -1, -1};
     if (b < hi) {
-      sI = tsI[b];
-      sJ = tsJ[b];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (tso[u] >= 0) sl[u] = ts0[tso[u] + b];
     }
-    const bool act = sI >= 0 && sJ >= 0;
+    const bool act = (sl[0] >= 0 || sl[1] >= 0) && (sl[2] >= 0 || sl[3] >= 0);
     const unsigned long long m = __ballot(act);
     __syncthreads();  // the previous window is done with the list and buffers
     if (lane == 0) wcnt[wv] = __popcll(m);
     __syncthreads();
-    int off = 0;
-    for (int w = 0; w < wv; ++w) off += wcnt[w];
-    const int n = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    int off = 0, n = 0;
+    for (int w = 0; w < 8; ++w) {
+      if (w < wv) off += wcnt[w];
+      n += wcnt[w];
+    }
     if (act) {
       const int i = off + __popcll(m & ((1ull << lane) - 1ull));
-      lsI[i] = sI;
-      lsJ[i] = sJ;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ls[u][i] = sl[u];
     }
     __syncthreads();
     if (n == 0) continue;
@@ -628,47 +950,63 @@ __global__ void __launch_bounds__(256) vgemm_i8(
         AZ[pl] = AnZ[pl];
       }
       const int eFc = eFn, eZc = eZn;
+      const bool mine = ls[half][i] >= 0;
+      const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
       if (i + 1 < n) issue(i + 1, buf ^ 1);
-      double sF[4], sZ[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        sF[r] = pow2_or_zero((int)(signed char)(eFc >> (8 * r)));
-        sZ[r] = pow2_or_zero((int)(signed char)(eZc >> (8 * r)));
-      }
-      const signed char *bz = &bbuf[buf][0][0];
-      const signed char *bf = &bbuf[buf][1][0];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v4i BZ[4], BF[4];
-#pragma unroll
-        for (int pl = 0; pl < 4; ++pl) {
-          BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
-          BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
-        }
-        const int eZJ = bz[4 * PLANE + 16 * j + l16];
-        const int eFJ = bf[4 * PLANE + 16 * j + l16];
-        const double sZJ = eZJ == -127 ? 0.0 : pow2(eZJ - 19);
-        const double sFJ = eFJ == -127 ? 0.0 : pow2(eFJ - 19);
-        int h1[4], o1[4], h2[4], o2[4];
-        tile_product(AF, BZ, h1, o1);
-        tile_product(AZ, BF, h2, o2);
+      if (mine) {
+        double sF[4], sZ[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          v64[j][r] = fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[j][r]);
-          v64[j][r] = fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[j][r]);
+          sF[r] = pow2_or_zero((int)(signed char)(eFc >> (8 * r)));
+          sZ[r] = pow2_or_zero((int)(signed char)(eZc >> (8 * r)));
+        }
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+          if (!jl[jb]) continue;
+          const signed char *bz = &bbuf[buf][2 * jb][0];
+          const signed char *bf = &bbuf[buf][2 * jb + 1][0];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            v4i BZ[4], BF[4];
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) {
+              BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
+              BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
+            }
+            const int eZJ = bz[4 * PLANE + 16 * j + l16];
+            const int eFJ = bf[4 * PLANE + 16 * j + l16];
+            const double sZJ = eZJ == -127 ? 0.0 : pow2(eZJ - 19);
+            const double sFJ = eFJ == -127 ? 0.0 : pow2(eFJ - 19);
+            int h1[4], o1[4], h2[4], o2[4];
+            tile_product(AF, BZ, h1, o1);
+            tile_product(AZ, BF, h2, o2);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              v64[jb][j][r] =
+                  fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[jb][j][r]);
+              v64[jb][j][r] =
+                  fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[jb][j][r]);
+            }
+          }
         }
       }
       buf ^= 1;
     }
   }
-  double *out = vpart + ((int64_t)split * npairs + p) * (FB * FB);
+  const int4 pix = sp_pidx[sp];
+  const int pp[2] = {half ? pix.z : pix.x, half ? pix.w : pix.y};
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int jb = 0; jb < 2; ++jb) {
+    if (pp[jb] < 0) continue;
+    double *out = vpart + ((int64_t)split * npairs + pp[jb]) * (FB * FB);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int mu = 16 * wv + 4 * q + r, nu = 16 * j + l16;
-      out[mu * FB + nu] += v64[j][r];
-    }
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = 16 * rw + 4 * q + r, nu = 16 * j + l16;
+        out[mu * FB + nu] += v64[jb][j][r];
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -700,14 +1038,52 @@ static void launch_density_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
                      coef, blockEN);
 }
 
-// shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1);
-// other values take 2
+template <int GBT, int MB>
+static void launch_density_x_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
+                                const Plan &pl, const signed char *Ps, int gga,
+                                const signed char *store, double *part) {
+  const int64_t ngroups = (b1 - b0 + GBT - 1) / GBT;
+  const int64_t ngq = (ngroups + 7) / 8;
+  const int umax = std::min(pl.nfb, GBT * pl.max_mol_nfb);
+  const int nmg = (umax + MB - 1) / MB;
+  const int64_t nwg = ngq * 8 * nmg;
+  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
+  hipLaunchKernelGGL((density_x_i8<GBT, MB>), dim3((unsigned)nwg), dim3(256 * MB), 0, s,
+                     pl.tab(), pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store,
+                     part);
+}
+
+// shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1),
+// + 32 for the split form (contraction, functional, Z as three kernels);
+// other values take 33
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
-                 double *coef, double *blockEN) {
+                 double *coef, double *blockEN, double *part) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
+  if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
+      shape != 49 && shape != 50)
+    shape = 33;
+  if (shape & 32) {
+    NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "split density form without a partial buffer");
+    switch (shape & 31) {
+      case 1: launch_density_x_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+      case 17: launch_density_x_i8<1, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+      case 18: launch_density_x_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+      default: launch_density_x_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+    }
+    NKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
+                       pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN);
+    NKX_HIP_CHECK(hipGetLastError());
+    const int64_t s1 = pl.b_off_h[b1];
+    if (s1 > s0)
+      hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+                         pl.slot_b.p, gga, coef, store);
+    NKX_HIP_CHECK(hipGetLastError());
+    return;
+  }
   switch (shape) {
     case 1: launch_density_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
     case 2: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
@@ -720,10 +1096,11 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
 
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
                int gga, const signed char *store, double *vpart) {
-  if (b1 <= b0 || pl.npairs == 0) return;
-  hipLaunchKernelGGL(vgemm_i8, dim3((unsigned)(pl.npairs * nsplit)), dim3(256), 0, s,
-                     pl.npairs, nsplit, pl.pair_I.p, pl.pair_J.p, pl.tile_slot.p, pl.tab(), b0,
-                     b1, s0, gga, store, vpart);
+  if (b1 <= b0 || pl.nspairs == 0) return;
+  hipLaunchKernelGGL(vgemm_i8, dim3((unsigned)(pl.nspairs * nsplit)), dim3(512), 0, s,
+                     pl.nspairs, nsplit, reinterpret_cast<const int4 *>(pl.sp_blk.p),
+                     reinterpret_cast<const int4 *>(pl.sp_pidx.p), pl.npairs, pl.tile_slot.p,
+                     pl.tab(), b0, b1, s0, gga, store, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -128,6 +128,12 @@ struct Plan {
   int npairs = 0;
   std::vector<int> pair_I_h, pair_J_h;
   DevBuf<int> pair_I, pair_J;
+  // 2 x 2 super-pairs of blocks (one molecule) for the int8 V contraction:
+  // 4 ints each, blocks I0 I1 J0 J1 (-1: none) and the pair indices of the
+  // tiles (I0,J0) (I0,J1) (I1,J0) (I1,J1) (-1: not a kept pair, or redundant)
+  int nspairs = 0;
+  int max_mol_nfb = 0;            // largest function-block count of one molecule
+  DevBuf<int> sp_blk, sp_pidx;
   DevBuf<int> tile_slot;          // per molecule [block][batch] -> slot in the tile store or -1
   int64_t nslots = 0;
   int64_t pairs_kept = 0;
@@ -149,7 +155,7 @@ struct Options {
   double screen_tol = 1e-12;
   int batch_points = 64;
   int vgemm_split = 0;          // V-contraction split count; 0 = automatic
-  int density_group = 0;        // batches per density workgroup (1, 2, 4); 0 = form default
+  int density_group = 0;        // density workgroup shape code; 0 = form default
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
 };
 
@@ -161,6 +167,7 @@ struct Workspace {
   DevBuf<double> coef;             // per padded point and spin: a, bx, by, bz
   DevBuf<double> blockEN;          // per batch: E, N_alpha, N_beta partial sums
   DevBuf<double> vpart;            // split-K partial V tiles
+  DevBuf<double> dpart;            // per slot: column sums of the density contraction (int8 split form)
   DevBuf<double> red;              // small reduction outputs
   DevBuf<double> scratch;
   DevBuf<double> gbuf;             // nuclear-gradient buffers

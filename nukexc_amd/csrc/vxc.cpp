@@ -331,6 +331,34 @@ void upload_plan(Plan &pl, const std::vector<PlanPart> &parts, hipStream_t s) {
     nbf_total += pt.nbf;
   }
   b_off[nb] = (int)b_fbl.size();
+  // super-pairs: blocks (2a, 2a + 1) x (2c, 2c + 1), a <= c, per molecule,
+  // kept when one of the four tiles is a kept pair
+  std::vector<int> spb, spp;
+  {
+    size_t pbase = 0;
+    for (int m = 0; m < nmol; ++m) {
+      const PlanPart &pt = parts[m];
+      const int blk0 = pl.mol_blk0_h[m], nf = pt.nfb;
+      std::vector<int> pid((size_t)nf * nf, -1);
+      for (size_t k = 0; k < pt.pair_I.size(); ++k)
+        pid[(size_t)pt.pair_I[k] * nf + pt.pair_J[k]] = (int)(pbase + k);
+      auto at = [&](int I, int J) {
+        return (I < 0 || J < 0 || I > J) ? -1 : pid[(size_t)I * nf + J];
+      };
+      const int ns = (nf + 1) / 2;
+      for (int a = 0; a < ns; ++a)
+        for (int c = a; c < ns; ++c) {
+          const int I0 = 2 * a, I1 = 2 * a + 1 < nf ? 2 * a + 1 : -1;
+          const int J0 = 2 * c, J1 = 2 * c + 1 < nf ? 2 * c + 1 : -1;
+          const int px[4] = {at(I0, J0), at(I0, J1), a == c ? -1 : at(I1, J0), at(I1, J1)};
+          if (px[0] < 0 && px[1] < 0 && px[2] < 0 && px[3] < 0) continue;
+          spb.insert(spb.end(), {blk0 + I0, I1 >= 0 ? blk0 + I1 : -1, blk0 + J0,
+                                 J1 >= 0 ? blk0 + J1 : -1});
+          spp.insert(spp.end(), px, px + 4);
+        }
+      pbase += pt.pair_I.size();
+    }
+  }
   if (terms.empty()) terms.push_back(HarmTerm{0, 0, 0, 0, 0});
   if (b_fbl.empty()) {  // keep device buffers non-empty
     b_fbl.push_back(0);
@@ -367,6 +395,13 @@ void upload_plan(Plan &pl, const std::vector<PlanPart> &parts, hipStream_t s) {
   if (pl.npairs) {
     pl.pair_I.upload(pl.pair_I_h.data(), pl.npairs, s);
     pl.pair_J.upload(pl.pair_J_h.data(), pl.npairs, s);
+  }
+  pl.nspairs = (int)(spb.size() / 4);
+  pl.max_mol_nfb = 0;
+  for (int m = 0; m < nmol; ++m) pl.max_mol_nfb = std::max(pl.max_mol_nfb, parts[m].nfb);
+  if (pl.nspairs) {
+    pl.sp_blk.upload(spb.data(), spb.size(), s);
+    pl.sp_pidx.upload(spp.data(), spp.size(), s);
   }
   pl.blk_mol.upload(blk_mol.data(), nfb, s);
   pl.blk_local.upload(blk_local.data(), nfb, s);
@@ -431,10 +466,13 @@ int64_t chunk_slot_budget(size_t slot_bytes) {
 }
 
 // Split count of the V contraction over batches: enough workgroups to fill
-// the chip (about 2048), or the autotuned value.
-int choose_nsplit(const Plan &pl, int tuned = 0) {
-  if (pl.npairs == 0) return 1;
-  int64_t ns = tuned > 0 ? tuned : (2048 + pl.npairs - 1) / pl.npairs;
+// the chip (about 2048 of the FP64 kernel's pair tiles, 4096 of the int8
+// kernel's super-pair tiles), or the autotuned value.
+int choose_nsplit(const Plan &pl, int tuned = 0, bool super = false) {
+  const int64_t units = super ? pl.nspairs : pl.npairs;
+  if (units == 0) return 1;
+  const int64_t target = super ? 4096 : 2048;
+  int64_t ns = tuned > 0 ? tuned : (target + units - 1) / units;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, pl.nb));
   return (int)ns;
 }
@@ -475,12 +513,13 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
                          int tuned_split, int tuned_group) {
   BuildSetup b;
   b.nspin = nspin;
-  // batches per density workgroup: tuned, else 4 (FP64) / 2 (int8 form)
-  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 2 : 4);
+  // density workgroup shape: tuned, else 4 batches per workgroup (FP64) /
+  // the split form with one batch and one function block (int8 form)
+  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 33 : 4);
   b.gga = gga;
   const int nmol = pl.nmol;
   b.narr = (gga ? 4 : 1) + nspin;
-  b.nsplit = choose_nsplit(pl, tuned_split);
+  b.nsplit = choose_nsplit(pl, tuned_split, precision == NKX_PREC_MIXED);
   b.bf16 = precision == NKX_PREC_MIXED_BF16;
   b.mixed = precision == NKX_PREC_MIXED || b.bf16;
   NKX_REQUIRE(!b.mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
@@ -497,6 +536,7 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
   b.slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   if (b.mixed) {
     ws.mstore.reserve((size_t)b.slots_alloc * slot_bytes);
+    if (!b.bf16) ws.dpart.reserve((size_t)b.slots_alloc * 4 * PT);
     int64_t psb = 0;
     for (int m = 0; m < nmol; ++m) psb += mixed_p_bytes(pl.mol_nfb_h[m]);
     ws.Ps.reserve(b.bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)psb);
@@ -558,7 +598,7 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
       vkb_density(s, 6, b0, b1, s0, pl, bPs, ig, bstore, xl, ws.coef.p, ws.blockEN.p);
     else if (mixed)
       vkm_density(s, b.group, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
-                  ws.blockEN.p);
+                  ws.blockEN.p, ws.dpart.p);
     else
       vk_density(s, nspin, b.group, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl,
                  ws.coef.p,
@@ -612,7 +652,7 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
 double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
                 int *best_split, int *best_group) {
   ensure_plan(c);
-  const int base = choose_nsplit(c.plan, 0);
+  const int base = choose_nsplit(c.plan, 0, precision == NKX_PREC_MIXED);
   std::vector<int> splits;
   for (int k : {base / 4, base / 2, base, 2 * base, 4 * base}) {
     const int v = std::max(1, std::min<int>(k, (int)std::max<int64_t>(c.plan.nb, 1)));
@@ -639,11 +679,12 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
   try {
     c.opt.vgemm_split = base;
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
-    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 2 : 4);
-    // density shapes: batches per workgroup (+ 16 for two function blocks
-    // per step in the int8 form)
+    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 33 : 4);
+    // density shapes: batches per workgroup (int8 form: + 16 for two
+    // function blocks per step, + 32 for the split form)
     const std::vector<int> shapes =
-        precision == NKX_PREC_MIXED ? std::vector<int>{1, 2, 17, 18} : std::vector<int>{1, 2, 4};
+        precision == NKX_PREC_MIXED ? std::vector<int>{1, 2, 17, 18, 33, 34, 49, 50}
+                                    : std::vector<int>{1, 2, 4};
     if (!bf16)
       for (int g : shapes) {
         c.opt.density_group = g;

@@ -360,10 +360,11 @@ def test_vxc_mixed_no_density_bias(ctx):
 
 def test_vxc_mixed_density_shapes(ctx):
     """The int8 density kernel's workgroup shapes on an 8-water cluster
-    (several function blocks per batch): one or two batches per workgroup
-    give bitwise the same build (same per-batch K order and sums); two
-    function blocks per step change only the order of the FP64 column sums.
-    Out-of-range shapes are rejected by nkx_set_launch."""
+    (several function blocks per batch): one or two batches per workgroup,
+    fused or split form (+32), give bitwise the same build (same per-batch K
+    order and sums); two function blocks per step in the fused form (17, 18)
+    change only the order of the FP64 column sums.  Out-of-range shapes are
+    rejected by nkx_set_launch."""
     rng = np.random.Generator(np.random.PCG64(8))
     xyz, Z, _ = configs.water_cluster(8, rng)
     basis = nk.make_manual_basis(configs.sto_functions(Z, xyz, rng))
@@ -373,7 +374,7 @@ def test_vxc_mixed_density_shapes(ctx):
     C, P = configs.loewdin_orbitals(S, 40, rng)
     res = {}
     try:
-        for shape in (2, 1, 17, 18):
+        for shape in (2, 1, 17, 18, 33, 34, 49, 50):
             ctx.set_launch(0, shape)
             assert ctx.get_launch() == (0, shape)
             r = ctx.vxc(dev(P), funcs=(101, 130), precision="mixed")
@@ -386,7 +387,9 @@ def test_vxc_mixed_density_shapes(ctx):
     finally:
         ctx.set_launch(0, 0)
     E2, N2, V2 = res[2]
-    assert res[1][0] == E2 and res[1][1] == N2 and np.array_equal(res[1][2], V2)
+    for shape in (1, 33, 34, 49, 50):
+        assert res[shape][0] == E2 and res[shape][1] == N2, shape
+        assert np.array_equal(res[shape][2], V2), shape
     for shape in (17, 18):
         E, N, V = res[shape]
         assert abs(E - E2) <= 1e-11 * abs(E2)
