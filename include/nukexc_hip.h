@@ -267,7 +267,8 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
    workgroup shape (FP64: 1, 2 or 4 point batches per workgroup; int8 form:
    1 or 2 batches, plus 16 for two function blocks per step, plus 32 for the
    split form, where contraction, functional and Z are three kernels and
-   the workgroups of one batch group share an XCD), then the split
+   the workgroups of one batch group share an XCD; 65 and 66 for the split
+   form with 32 rows per wave), then the split
    count of the V contraction over batches (the automatic choice x 1/4 .. 4)
    -- keeps the fastest for later builds and captures, and returns them
    (either pointer may be NULL) with the best build time.  V receives the
@@ -278,8 +279,8 @@ int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_i
 /* Launch parameters (no counterpart in the reference): set or read the
    values the autotuner picks, e.g. to reuse a tuned pair without timing
    again.  vgemm_split 0 = automatic; density_shape 0 = default, else one of
-   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50 (int8 form); the FP64 form
-   reads the int8 codes modulo 16, the int8 form reads 4 as 33. */
+   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66 (int8 form); the
+   FP64 form reads the int8 codes modulo 16, the int8 form reads 4 as 33. */
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
 int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)

@@ -767,6 +767,236 @@ __global__ void __launch_bounds__(256 * MB) density_x_i8(
   }
 }
 
+// density_x2_i8: the split form's contraction with two row fragments per
+// wave.  One workgroup of four waves per (batch group of GBT batches, pair
+// of union-list blocks m_0 < m_1); wave w holds rows 32 (w % 2) .. +31 of
+// block m_(w / 2), so each B fragment read from LDS feeds two tile products.
+// The active k of the union list and their masks are listed in LDS once;
+// XCD placement, staging and the per-slot output are those of density_x_i8.
+template <int GBT>
+__global__ void __launch_bounds__(256) density_x2_i8(
+    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
+    const signed char *__restrict__ store, double *__restrict__ part) {
+  const int xcd = blockIdx.x & 7;
+  const int rr = blockIdx.x >> 3;
+  const int mgi = rr % nmg;
+  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
+  if (bg >= b1) return;
+  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int mb = wv >> 1, rh = wv & 1;
+  const int l16 = lane & 15, q = lane >> 4;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
+  __shared__ double red[4][4][PT];
+  __shared__ int tab[GBT][MAXNFB];
+  __shared__ int ublk[MAXNFB];
+  __shared__ short sk[MAXNFB];            // active steps: union index of k
+  __shared__ unsigned char spm[MAXNFB];   // and mask, bit (m * GBT + sb)
+  __shared__ int nu_s, nst_s;
+
+  for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+  __syncthreads();
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
+    for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nfb; base += 64) {
+      const int f = base + lane;
+      bool live = false;
+      if (f < nfb)
+        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
+      const unsigned long long m = __ballot(live);
+      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nu_s = cnt;
+  }
+  __syncthreads();
+  const int nu = nu_s;
+  if (mgi * 2 >= nu) return;  // uniform: no block of this pair
+  const int fm0 = ublk[mgi * 2], fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nu; base += 64) {
+      const int ki = base + lane;
+      unsigned pm = 0;
+      if (ki < nu) {
+        const int fk = ublk[ki];
+#pragma unroll
+        for (int sb = 0; sb < GBT; ++sb) {
+          if (sb >= nsub || tab[sb][fk] < 0) continue;
+          if (tab[sb][fm0] >= 0) pm |= 1u << sb;
+          if (fm1 >= 0 && tab[sb][fm1] >= 0) pm |= 1u << (GBT + sb);
+        }
+      }
+      const unsigned long long m = __ballot(pm != 0);
+      if (pm) {
+        const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        sk[i] = (short)ki;
+        spm[i] = (unsigned char)pm;
+      }
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nst_s = cnt;
+  }
+  __syncthreads();
+  const int nst = nst_s;
+  const int nq = gga ? 4 : 1;
+  const int fm = mb ? fm1 : fm0;
+  constexpr unsigned ROW = (1u << GBT) - 1u;
+  // this wave's P row: sections (fm, k) of its molecule
+  const signed char *prow = nullptr;
+  if (fm >= 0) {
+    const int mol = mt.blk_mol[fm];
+    prow = Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
+  }
+
+  // P fragments and row exponents of a step (two register sets, used in turn)
+  struct AFrag {
+    v4i v[2][4];
+    int e[2];
+  };
+  auto issue = [&](int i, int dbuf, AFrag &an) {
+    const int fk = ublk[sk[i]];
+    const unsigned pm = spm[i];
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!(((pm >> sb) | (pm >> (GBT + sb))) & 1u)) continue;
+      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                  (wv + sb) & 3, 4, lane);
+    }
+    if ((pm >> (mb * GBT)) & ROW) {
+      const signed char *pa = prow + (int64_t)mt.blk_local[fk] * SEC;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          an.v[f][p] = ld16(pa + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
+        an.e[f] = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
+      }
+    }
+  };
+
+  double x64[GBT][2][4][4];
+#pragma unroll
+  for (int sb = 0; sb < GBT; ++sb)
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x64[sb][f][j][r] = 0.0;
+  // One step: wait for its operands, start the next step's copies into the
+  // other buffer and register set, then the tile products.  Rows or points
+  // stored as zero have zero slices, so their scale does not matter.
+  auto step = [&](int i, int buf, const AFrag &ac, AFrag &an) {
+    wait_vm0();
+    __syncthreads();  // buffer buf complete, buf ^ 1 free
+    const unsigned pmc = spm[i];
+    if (i + 1 < nst) issue(i + 1, buf ^ 1, an);
+    double sA[2][4];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ac.e[f] >> (8 * r)));
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
+      const signed char *bb = &bbuf[buf][sb][0];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4i B[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+        const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          int hi[4], lo[4];
+          tile_product(ac.v[f], B, hi, lo);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            x64[sb][f][j][r] = fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[f][r] * sB,
+                                   x64[sb][f][j][r]);
+        }
+      }
+    }
+  };
+  AFrag a0, a1;
+  if (nst > 0) issue(0, 0, a0);
+  for (int i = 0; i < nst; i += 2) {
+    step(i, 0, a0, a1);
+    if (i + 1 < nst) step(i + 1, 1, a1, a0);
+  }
+
+  // Epilogue: per live slot (m, sb), column sums over the block's functions.
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int slm = fm >= 0 ? tab[sb][fm] : -1;
+    double prt[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
+    if (slm >= 0) {
+      const signed char *tb = store + (slm - s0) * sb_bytes;
+      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int g = 16 * j + l16;
+        double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mu = 32 * rh + 16 * f + 4 * q + r;
+            double x = 0.0;
+#pragma unroll
+            for (int s2 = 0; s2 < GBT; ++s2)
+              if (s2 == sb) x = x64[s2][f][j][r];
+            sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+            if (gga) {
+              sm[1] += (double)G[mu * PT + g] * x;
+              sm[2] += (double)G[TILE + mu * PT + g] * x;
+              sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+            }
+          }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = sm[c];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          prt[j][c] = v;
+        }
+      }
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
+    }
+    __syncthreads();
+    {
+      const int m = threadIdx.x >> 7, c = (threadIdx.x >> 6) & 1, g = threadIdx.x & 63;
+      const int fmx = m ? fm1 : fm0;
+      const int sl = fmx >= 0 ? tab[sb][fmx] : -1;
+      if (sl >= 0) {
+#pragma unroll
+        for (int cc = 0; cc < 4; cc += 2) {
+          const int c2 = c + cc;
+          const double acc = c2 < nq ? red[2 * m][c2][g] + red[2 * m + 1][c2][g] : 0.0;
+          part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Per batch: column sums of the live slots in block order (the fused
 // kernel's order), functional, Z coefficients, (E, N) partial sums.
 __global__ void __launch_bounds__(256) density_f_i8(
@@ -1053,8 +1283,23 @@ static void launch_density_x_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s
                      part);
 }
 
+template <int GBT>
+static void launch_density_x2_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
+                                 const Plan &pl, const signed char *Ps, int gga,
+                                 const signed char *store, double *part) {
+  const int64_t ngroups = (b1 - b0 + GBT - 1) / GBT;
+  const int64_t ngq = (ngroups + 7) / 8;
+  const int umax = std::min(pl.nfb, GBT * pl.max_mol_nfb);
+  const int nmg = (umax + 1) / 2;
+  const int64_t nwg = ngq * 8 * nmg;
+  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
+  hipLaunchKernelGGL((density_x2_i8<GBT>), dim3((unsigned)nwg), dim3(256), 0, s, pl.tab(),
+                     pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store, part);
+}
+
 // shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1),
-// + 32 for the split form (contraction, functional, Z as three kernels);
+// + 32 for the split form (contraction, functional, Z as three kernels),
+// 65 / 66: the split form with two row fragments per wave (1 / 2 batches);
 // other values take 33
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
@@ -1063,15 +1308,22 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
   if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
-      shape != 49 && shape != 50)
+      shape != 49 && shape != 50 && shape != 65 && shape != 66)
     shape = 33;
-  if (shape & 32) {
+  if (shape & 96) {
     NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "split density form without a partial buffer");
+    if (shape & 64) {
+      if (shape == 65)
+        launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part);
+      else
+        launch_density_x2_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, part);
+    } else {
     switch (shape & 31) {
       case 1: launch_density_x_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
       case 17: launch_density_x_i8<1, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
       case 18: launch_density_x_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
       default: launch_density_x_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+    }
     }
     NKX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,

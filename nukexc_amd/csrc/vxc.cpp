@@ -466,12 +466,12 @@ int64_t chunk_slot_budget(size_t slot_bytes) {
 }
 
 // Split count of the V contraction over batches: enough workgroups to fill
-// the chip (about 2048 of the FP64 kernel's pair tiles, 4096 of the int8
+// the chip (about 2048 of the FP64 kernel's pair tiles, 8192 of the int8
 // kernel's super-pair tiles), or the autotuned value.
 int choose_nsplit(const Plan &pl, int tuned = 0, bool super = false) {
   const int64_t units = super ? pl.nspairs : pl.npairs;
   if (units == 0) return 1;
-  const int64_t target = super ? 4096 : 2048;
+  const int64_t target = super ? 8192 : 2048;
   int64_t ns = tuned > 0 ? tuned : (target + units - 1) / units;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, pl.nb));
   return (int)ns;
@@ -514,8 +514,8 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
   BuildSetup b;
   b.nspin = nspin;
   // density workgroup shape: tuned, else 4 batches per workgroup (FP64) /
-  // the split form with one batch and one function block (int8 form)
-  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 33 : 4);
+  // the split form with 32 rows per wave and one batch (int8 form)
+  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 65 : 4);
   b.gga = gga;
   const int nmol = pl.nmol;
   b.narr = (gga ? 4 : 1) + nspin;
@@ -679,11 +679,13 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
   try {
     c.opt.vgemm_split = base;
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
-    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 33 : 4);
+    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 65 : 4);
     // density shapes: batches per workgroup (int8 form: + 16 for two
-    // function blocks per step, + 32 for the split form)
+    // function blocks per step, + 32 for the split form, 65 / 66 the split
+    // form with two row fragments per wave; the fused shapes other than 18
+    // are never the fastest on large grids and are not timed)
     const std::vector<int> shapes =
-        precision == NKX_PREC_MIXED ? std::vector<int>{1, 2, 17, 18, 33, 34, 49, 50}
+        precision == NKX_PREC_MIXED ? std::vector<int>{33, 49, 50, 65, 66, 18}
                                     : std::vector<int>{1, 2, 4};
     if (!bf16)
       for (int g : shapes) {
