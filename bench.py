@@ -25,6 +25,50 @@ PEAK_TFLOPS = {"f64": 78.6, "i8": 5000.0, "bf16": 2500.0}
 # The mixed form runs each tile product as ten int8 slice products on MFMA.
 MIXED_OPS_PER_FLOP = 10
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs scaled by a plain copy kernel (written by hand from the
+# profiles in profiles/; see the .md files there for the commands)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def kernel_name(precision, stage, shape):
+    """Name of the kernel behind a timing stage for a density shape code."""
+    if precision == "fp64":
+        return {"density": "density_fused_f64", "vgemm": "vgemm_f64",
+                "collocation": "colloc_tiles_f64"}[stage]
+    if precision == "mixed_bf16":
+        return {"density": "density_fused_mixed", "vgemm": "vgemm_mixed",
+                "collocation": "colloc_tiles_mixed"}[stage]
+    if stage == "density":
+        shape = shape or 65
+        if shape == 129:
+            return "density_x4_i8"
+        if shape == 67:
+            return "density_x22_i8"
+        if shape in (65, 66):
+            return "density_x2_i8"
+        if shape & 32:
+            return "density_x_i8"
+        return "density_fused_i8"
+    return {"vgemm": "vgemm_i8", "collocation": "colloc_tiles_i8"}[stage]
+
+
+def measured_traffic(workload, precision, split, shape, kernel):
+    """(bytes per launch, source) of a profiled configuration, or (None, None).
+    A density kernel must match the density shape, a V kernel the split count."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        entries = json.load(f)
+    for e in entries:
+        if e["workload"] != workload or e["precision"] != precision or kernel not in e["kernels"]:
+            continue
+        if kernel.startswith("density") and e["shape"] != shape:
+            continue
+        if kernel.startswith("vgemm") and e["split"] != split:
+            continue
+        return e["kernels"][kernel]["bytes_per_launch"], e["source"]
+    return None, None
 
 
 def log(msg):
@@ -147,11 +191,16 @@ def main():
     funcs = w["funcs"]
     npts = grid.weights.shape[0]
 
-    ref, fp64_ms = None, None
+    ref, fp64_ms, fp64_tuned = None, None, None
     if args.precision != "fp64" and not args.no_reference:
         ref = ctx.vxc(P, funcs, precision="fp64")
         log(f"FP64 reference build: E_xc = {ref.energy:.10f}")
         if args.fp64_steps > 0:
+            fp64_tuned = None
+            if not args.no_autotune:  # the FP64 form at its own best launch parameters
+                k, g, ms = ctx.autotune(P, funcs, precision="fp64")
+                fp64_tuned = {"vgemm_split": k, "density_group": g}
+                log(f"FP64 autotune: V split {k}, density group {g}, {ms:.1f} ms/build")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.fp64_steps):
@@ -159,6 +208,7 @@ def main():
             torch.cuda.synchronize()
             fp64_ms = 1e3 * (time.perf_counter() - t0) / args.fp64_steps
             log(f"FP64 form: {fp64_ms:.2f} ms/build over {args.fp64_steps} builds")
+            ctx.set_launch(0, 0)
 
     tuned = None
     if args.launch:
@@ -196,6 +246,8 @@ def main():
     stage_flops = {"density": work["flops_density"], "vgemm": work["flops_v"]}
     dom = max(("density", "vgemm", "collocation"), key=lambda k: tm[k][0])
     dom_ms, dom_launches = tm[dom]
+    split_used, shape_used = ctx.get_launch()
+    dom_kernel = kernel_name(args.precision, dom, shape_used)
     if dom == "collocation":
         elem_bytes = (8 if args.precision == "fp64" else 4) * (4 if len(funcs) and max(funcs) > 100 else 1)
         traffic_alg = work["colloc_elems"] * elem_bytes
@@ -212,10 +264,16 @@ def main():
             # products (six bf16 products) per tile product
             ops = MIXED_OPS_PER_FLOP if args.precision == "mixed" else 6
             roof["mfma_issue_frac"] = achieved * ops / peak
-    roof.update({"kernel": {"density": "density_fused", "vgemm": "vgemm",
-                            "collocation": "colloc_tiles"}[dom],
+    traffic, source = measured_traffic(args.workload, args.precision, split_used, shape_used,
+                                       dom_kernel)
+    roof["traffic"] = traffic
+    roof.update({"kernel": dom_kernel,
                  "avg_launch_ms": dom_ms / max(dom_launches, 1),
-                 "launches": dom_launches})
+                 "launches": dom_launches,
+                 "stages_ms_per_build": {k: v[0] / args.steps for k, v in tm.items()
+                                         if k != "build" and v[1]}})
+    if source:
+        roof["traffic_source"] = source
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -257,6 +315,7 @@ def main():
         out["config"]["max_dV_vs_fp64"] = float((r.potential - ref.potential).abs().max())
         if fp64_ms is not None:
             out["config"]["fp64_ms_per_build"] = fp64_ms
+            out["config"]["fp64_launch"] = fp64_tuned
             out["config"]["speedup_vs_fp64"] = fp64_ms / ms_step
     print(json.dumps(out), flush=True)
     return 0
