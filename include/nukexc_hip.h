@@ -268,7 +268,9 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
    1 or 2 batches, plus 16 for two function blocks per step, plus 32 for the
    split form, where contraction, functional and Z are three kernels and
    the workgroups of one batch group share an XCD; 65 and 66 for the split
-   form with 32 rows per wave), then the split
+   form with 32 rows per wave; 67 for two batches x two blocks per
+   workgroup with both operands in LDS; 129 for the split form over quads of
+   function blocks, one batch and eight waves per workgroup), then the split
    count of the V contraction over batches (the automatic choice x 1/4 .. 4)
    -- keeps the fastest for later builds and captures, and returns them
    (either pointer may be NULL) with the best build time.  V receives the
@@ -279,8 +281,9 @@ int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_i
 /* Launch parameters (no counterpart in the reference): set or read the
    values the autotuner picks, e.g. to reuse a tuned pair without timing
    again.  vgemm_split 0 = automatic; density_shape 0 = default, else one of
-   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66 (int8 form); the
-   FP64 form reads the int8 codes modulo 16, the int8 form reads 4 as 33. */
+   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66, 67, 129 (int8
+   form); the FP64 form reads the int8 codes modulo 16, the int8 form reads
+   4 as 65. */
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
 int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
@@ -293,9 +296,10 @@ int nkx_plan_stats(nkx_ctx *ctx, int64_t *npts, int64_t *npts_padded, int64_t *n
 /* Kernel timing (the reference driver's TimingRegistry,
    tests/standalone_helpers.hpp:71-156, at kernel granularity): when enabled,
    nkx_vxc records HIP events on the ctx stream around each launch.  Stages:
-   0 collocation, 1 density contraction + functional, 2 V contraction,
-   3 gather / reductions, 4 whole build.  ms is the summed event time since
-   the last reset, launches the number of timed launches. */
+   0 collocation, 1 density contraction (+ functional and Z in the fused
+   forms), 2 V contraction, 3 gather / reductions, 4 whole build, 5 functional
+   and Z of the int8 split form.  ms is the summed event time since the last
+   reset, launches the number of timed launches. */
 int nkx_set_timing(nkx_ctx *ctx, int enable);
 int nkx_reset_timing(nkx_ctx *ctx);
 int nkx_get_timing(nkx_ctx *ctx, int stage, double *ms, int64_t *launches);

@@ -313,7 +313,7 @@ class Context:
     def set_launch(self, vgemm_split=0, density_shape=0):
         """Set the launch parameters the autotuner picks (0 = automatic /
         default); density shapes 1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50,
-        65, 66 (int8 form)."""
+        65, 66, 67, 129 (int8 form)."""
         self._chk(lib().nkx_set_launch(self.h, int(vgemm_split), int(density_shape)))
 
     def get_launch(self):
@@ -385,7 +385,7 @@ class Context:
         self._chk(lib().nkx_molset_vxc(self.h, Pp, nf, ids, prec, E, N, Vp))
         return [XC_result(E[k], Vs[k], N[k]) for k in range(nmol)]
 
-    STAGES = ("collocation", "density", "vgemm", "final", "build")
+    STAGES = ("collocation", "density", "vgemm", "final", "build", "functional_z")
 
     def set_timing(self, on=True):
         self._chk(lib().nkx_set_timing(self.h, int(bool(on))))

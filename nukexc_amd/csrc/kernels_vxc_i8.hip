@@ -123,6 +123,13 @@ __device__ __forceinline__ void tile_product(const v4i *A, const v4i *B, int hi[
   }
 }
 
+// Z = a phi + bx dphi/dx + by dphi/dy + bz dphi/dz, in one fixed order of
+// roundings (every kernel that forms Z uses this)
+__device__ __forceinline__ double z_value(double ph, double gx, double gy, double gz, double a,
+                                          double bx, double by, double bz) {
+  return fma(bz, gz, fma(by, gy, fma(bx, gx, a * ph)));
+}
+
 // Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65) with
 // per-row exponents ex[row] into a section at dst (fragment order); one
 // thread per (row fragment, lane) item, 256 threads.
@@ -523,11 +530,10 @@ __global__ void __launch_bounds__(256 * MB) density_fused_i8(
       const float *G = reinterpret_cast<const float *>(tb + OFF_G);
       for (int e = threadIdx.x; e < TILE; e += NT) {
         const int f = e / PT, g = e % PT;
-        double zz = cf[sb][0][g] * unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
-        if (gga)
-          zz += cf[sb][1][g] * (double)G[e] + cf[sb][2][g] * (double)G[TILE + e] +
-                cf[sb][3][g] * (double)G[2 * TILE + e];
-        zt[f][g] = zz;
+        const double ph = unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
+        zt[f][g] = gga ? z_value(ph, (double)G[e], (double)G[TILE + e], (double)G[2 * TILE + e],
+                                 cf[sb][0][g], cf[sb][1][g], cf[sb][2][g], cf[sb][3][g])
+                       : cf[sb][0][g] * ph;
       }
       __syncthreads();
       if (threadIdx.x < FB) {
@@ -997,6 +1003,396 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   }
 }
 
+// density_x22_i8: the split form's contraction for a group of two batches
+// and a pair of union-list blocks m_0 < m_1, eight waves: wave w computes
+// batch w / 4, rows 32 (w % 2) .. +31 of block m_((w / 2) % 2).  Both
+// operands are staged in LDS (two buffers of the P sections (m_0, k),
+// (m_1, k) and the Pm sections (k, b_0), (k, b_1)), so each P section read
+// from memory serves both batches and each Pm section both blocks.  Per
+// wave the work and the sums are those of density_x2_i8 for its batch, and
+// the per-slot output is the same.
+__global__ void __launch_bounds__(512) density_x22_i8(
+    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
+    const signed char *__restrict__ store, double *__restrict__ part) {
+  const int xcd = blockIdx.x & 7;
+  const int rr = blockIdx.x >> 3;
+  const int mgi = rr % nmg;
+  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * 2;
+  if (bg >= b1) return;
+  const int nsub = (int)(b1 - bg < 2 ? b1 - bg : 2);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int sb = wv >> 2, mb = (wv >> 1) & 1, rh = wv & 1;
+  const int l16 = lane & 15, q = lane >> 4;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+
+  // [buffer][P (m_0, k), P (m_1, k), Pm (k, b_0), Pm (k, b_1)]; the column-sum
+  // exchange of the epilogue reuses the first buffer
+  __shared__ __attribute__((aligned(16))) signed char buf4[2][4][SEC];
+  __shared__ short tab[2][MAXNFB];  // slot - b_off[batch], -1 if not live
+  __shared__ short ublk[MAXNFB];
+  __shared__ short sk[MAXNFB];
+  __shared__ unsigned char spm[MAXNFB];
+  __shared__ int nu_s, nst_s;
+  static_assert(sizeof(buf4[0]) >= sizeof(double) * 8 * 4 * PT, "red alias");
+  double(*red)[4][PT] = reinterpret_cast<double(*)[4][PT]>(&buf4[0][0][0]);
+
+  for (int e = threadIdx.x; e < 2 * nfb; e += 512) tab[e / nfb][e % nfb] = -1;
+  __syncthreads();
+  int boff[2] = {0, 0};
+  for (int s2 = 0; s2 < 2; ++s2) {
+    if (s2 >= nsub) break;
+    const int lo = b_off[bg + s2], hi = b_off[bg + s2 + 1];
+    boff[s2] = lo;
+    for (int k = lo + (int)threadIdx.x; k < hi; k += 512) tab[s2][b_fbl[k]] = (short)(k - lo);
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nfb; base += 64) {
+      const int f = base + lane;
+      const bool live = f < nfb && (tab[0][f] >= 0 || tab[1][f] >= 0);
+      const unsigned long long m = __ballot(live);
+      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (short)f;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nu_s = cnt;
+  }
+  __syncthreads();
+  const int nu = nu_s;
+  if (mgi * 2 >= nu) return;  // uniform: no block of this pair
+  const int fm0 = ublk[mgi * 2], fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
+  if (wv == 0) {
+    int cnt = 0;
+    for (int base = 0; base < nu; base += 64) {
+      const int ki = base + lane;
+      unsigned pm = 0;
+      if (ki < nu) {
+        const int fk = ublk[ki];
+        for (int s2 = 0; s2 < 2; ++s2) {
+          if (s2 >= nsub || tab[s2][fk] < 0) continue;
+          if (tab[s2][fm0] >= 0) pm |= 1u << s2;
+          if (fm1 >= 0 && tab[s2][fm1] >= 0) pm |= 1u << (2 + s2);
+        }
+      }
+      const unsigned long long m = __ballot(pm != 0);
+      if (pm) {
+        const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        sk[i] = (short)ki;
+        spm[i] = (unsigned char)pm;
+      }
+      cnt += __popcll(m);
+    }
+    if (lane == 0) nst_s = cnt;
+  }
+  __syncthreads();
+  const int nst = nst_s;
+  const int nq = gga ? 4 : 1;
+  const int fm = mb ? fm1 : fm0;
+  const signed char *prow[2] = {nullptr, nullptr};
+#pragma unroll
+  for (int m2 = 0; m2 < 2; ++m2) {
+    const int f = m2 ? fm1 : fm0;
+    if (f >= 0) {
+      const int mol = mt.blk_mol[f];
+      prow[m2] = Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[f] * mt.nfb[mol] * SEC;
+    }
+  }
+  // P sections of the live blocks and Pm sections of the live batches of step
+  // i -> buffer dbuf; 8 waves, pieces spread over them
+  auto issue = [&](int i, int dbuf) {
+    const int fk = ublk[sk[i]];
+    const unsigned pm = spm[i];
+#pragma unroll
+    for (int m2 = 0; m2 < 2; ++m2)
+      if ((pm >> (2 * m2)) & 3u)
+        dma_section(prow[m2] + (int64_t)mt.blk_local[fk] * SEC, &buf4[dbuf][m2][0],
+                    (wv + 3 * m2) & 7, 8, lane);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+      if (((pm >> s2) | (pm >> (2 + s2))) & 1u)
+        dma_section(store + (boff[s2] + tab[s2][fk] - s0) * sb_bytes + OFF_PM,
+                    &buf4[dbuf][2 + s2][0], (wv + 6 + 3 * s2) & 7, 8, lane);
+  };
+
+  double x64[2][4][4];
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x64[f][j][r] = 0.0;
+  if (nst > 0) issue(0, 0);
+  int bufi = 0;
+  for (int i = 0; i < nst; ++i) {
+    wait_vm0();
+    __syncthreads();  // buffer bufi complete, bufi ^ 1 free
+    const unsigned pmc = spm[i];
+    if (i + 1 < nst) issue(i + 1, bufi ^ 1);
+    if ((pmc >> (2 * mb + sb)) & 1u) {
+      const signed char *ab = &buf4[bufi][mb][0];
+      const signed char *bb = &buf4[bufi][2 + sb][0];
+      v4i A[2][4];
+      double sA[2][4];
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) A[f][p] = ld16(ab + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
+        const int ea = *reinterpret_cast<const int *>(ab + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ea >> (8 * r)));
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4i B[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+        const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          int hi4[4], lo4[4];
+          tile_product(A[f], B, hi4, lo4);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            x64[f][j][r] =
+                fma(fma((double)lo4[r], 0x1p-14, (double)hi4[r]), sA[f][r] * sB, x64[f][j][r]);
+        }
+      }
+    }
+    bufi ^= 1;
+  }
+  __syncthreads();  // every wave is done with the buffers (red aliases them)
+
+  // Epilogue: per live slot (m, batch), column sums over the block's functions.
+  const int slm = (sb < nsub && fm >= 0 && tab[sb][fm] >= 0) ? boff[sb] + tab[sb][fm] : -1;
+  double prt[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
+  if (slm >= 0) {
+    const signed char *tb = store + (slm - s0) * sb_bytes;
+    const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int g = 16 * j + l16;
+      double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = 32 * rh + 16 * f + 4 * q + r;
+          const double x = x64[f][j][r];
+          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+          if (gga) {
+            sm[1] += (double)G[mu * PT + g] * x;
+            sm[2] += (double)G[TILE + mu * PT + g] * x;
+            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+          }
+        }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double v = sm[c];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        prt[j][c] = v;
+      }
+    }
+  }
+  if (q == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
+  }
+  __syncthreads();
+  {
+    // thread -> (batch, block, column group c, point); waves 4 s + 2 m + {0, 1}
+    const int s2 = threadIdx.x >> 8, m2 = (threadIdx.x >> 7) & 1, c = (threadIdx.x >> 6) & 1,
+              g = threadIdx.x & 63;
+    const int fmx = m2 ? fm1 : fm0;
+    const int sl = (s2 < nsub && fmx >= 0 && tab[s2][fmx] >= 0) ? boff[s2] + tab[s2][fmx] : -1;
+    if (sl >= 0) {
+      const int w0 = 4 * s2 + 2 * m2;
+#pragma unroll
+      for (int cc = 0; cc < 4; cc += 2) {
+        const int c2 = c + cc;
+        const double acc = c2 < nq ? red[w0][c2][g] + red[w0 + 1][c2][g] : 0.0;
+        part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
+      }
+    }
+  }
+}
+
+// density_x4_i8: the split form's contraction for one batch and a quad of
+// function blocks 4i .. 4i + 3 of the batch's molecule, chosen by block
+// index rather than by position in the live list, so that workgroups of
+// different batches with the same quad read the same P sections.  Eight
+// waves; wave w holds rows 32 (w % 2) .. +31 of block 4i + w / 2 and idles
+// when that block is not live in the batch.  Workgroup w takes quad
+// (w / 8) / nbq and batch b0 + 8 ((w / 8) % nbq) + w % 8: the workgroups that
+// run together on one XCD share the quad, whose row of P sections stays in
+// that XCD's L2, and stream the Pm sections of their own batches.  The steps
+// are the batch's live blocks k in ascending order (its CSR range).
+__global__ void __launch_bounds__(512) density_x4_i8(
+    MolTab mt, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int64_t nbq, int gga,
+    const signed char *__restrict__ store, double *__restrict__ part) {
+  const int xcd = blockIdx.x & 7;
+  const int64_t rr = blockIdx.x >> 3;
+  const int quad = (int)(rr / nbq);
+  const int64_t b = b0 + (rr % nbq) * 8 + xcd;
+  if (b >= b1) return;
+  const int lo = b_off[b], hi = b_off[b + 1];
+  if (hi <= lo) return;
+  const int f0 = b_fbl[lo];
+  const int mol = mt.blk_mol[f0];
+  const int mbase = f0 - mt.blk_local[f0];  // the molecule's first (global) block
+  const int nbm = mt.nfb[mol];
+  if (4 * quad >= nbm) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int u = wv >> 1, rh = wv & 1;
+  const int l16 = lane & 15, q = lane >> 4;
+  const int64_t sb_bytes = slot_bytes_i8(gga);
+
+  __shared__ __attribute__((aligned(16))) signed char bbuf[2][SEC];
+  __shared__ double red[8][4][PT];
+  __shared__ int qslot[4];
+  if (threadIdx.x < 4) {  // slot of each quad block in this batch, -1 if not live
+    const int fb = mbase + 4 * quad + (int)threadIdx.x;
+    int sl = -1;
+    if (4 * quad + (int)threadIdx.x < nbm) {
+      int a = lo, c = hi;  // b_fbl ascending in [lo, hi)
+      while (a < c) {
+        const int mid = (a + c) >> 1;
+        if (b_fbl[mid] < fb) a = mid + 1;
+        else c = mid;
+      }
+      if (a < hi && b_fbl[a] == fb) sl = a;
+    }
+    qslot[threadIdx.x] = sl;
+  }
+  __syncthreads();
+  if (qslot[0] < 0 && qslot[1] < 0 && qslot[2] < 0 && qslot[3] < 0) return;  // uniform
+  const int myslot = qslot[u];
+  const int nq = gga ? 4 : 1;
+  const signed char *prow =
+      Ps + mt.Ps_off[mol] + (int64_t)(4 * quad + u) * nbm * SEC;  // sections (m, k), k local
+
+  struct AFrag {
+    v4i v[2][4];
+    int e[2];
+  };
+  auto issue = [&](int i, int dbuf, AFrag &an) {
+    dma_section(store + (i - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][0], wv, 8, lane);
+    if (myslot >= 0) {
+      const signed char *pa = prow + (int64_t)(b_fbl[i] - mbase) * SEC;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          an.v[f][p] = ld16(pa + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
+        an.e[f] = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
+      }
+    }
+  };
+  double x64[2][4][4];
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x64[f][j][r] = 0.0;
+  // rows or points stored as zero have zero slices: their scale does not matter
+  auto step = [&](int i, int buf, const AFrag &ac, AFrag &an) {
+    wait_vm0();
+    __syncthreads();  // buffer buf complete, buf ^ 1 free
+    if (i + 1 < hi) issue(i + 1, buf ^ 1, an);
+    if (myslot < 0) return;
+    double sA[2][4];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ac.e[f] >> (8 * r)));
+    const signed char *bb = &bbuf[buf][0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v4i B[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+      const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        int hi4[4], lo4[4];
+        tile_product(ac.v[f], B, hi4, lo4);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          x64[f][j][r] =
+              fma(fma((double)lo4[r], 0x1p-14, (double)hi4[r]), sA[f][r] * sB, x64[f][j][r]);
+      }
+    }
+  };
+  AFrag a0, a1;
+  issue(lo, 0, a0);
+  for (int i = lo; i < hi; i += 2) {
+    step(i, 0, a0, a1);
+    if (i + 1 < hi) step(i + 1, 1, a1, a0);
+  }
+
+  // Epilogue: column sums over each live quad block's functions.
+  double prt[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
+  if (myslot >= 0) {
+    const signed char *tb = store + (myslot - s0) * sb_bytes;
+    const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int g = 16 * j + l16;
+      double sm[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = 32 * rh + 16 * f + 4 * q + r;
+          const double x = x64[f][j][r];
+          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+          if (gga) {
+            sm[1] += (double)G[mu * PT + g] * x;
+            sm[2] += (double)G[TILE + mu * PT + g] * x;
+            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
+          }
+        }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double v = sm[c];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        prt[j][c] = v;
+      }
+    }
+  }
+  if (q == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
+  }
+  __syncthreads();
+  {
+    const int uu = threadIdx.x >> 7, c = (threadIdx.x >> 6) & 1, g = threadIdx.x & 63;
+    const int sl = qslot[uu];
+    if (sl >= 0) {
+#pragma unroll
+      for (int cc = 0; cc < 4; cc += 2) {
+        const int c2 = c + cc;
+        const double acc = c2 < nq ? red[2 * uu][c2][g] + red[2 * uu + 1][c2][g] : 0.0;
+        part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
+      }
+    }
+  }
+}
+
 // Per batch: column sums of the live slots in block order (the fused
 // kernel's order), functional, Z coefficients, (E, N) partial sums.
 __global__ void __launch_bounds__(256) density_f_i8(
@@ -1038,38 +1434,75 @@ __global__ void __launch_bounds__(256) density_f_i8(
 }
 
 // Z = a Phi + b . grad Phi of one live slot in FP64, per-function exponents,
-// slices (F order); one workgroup per slot.
+// slices (F order); one workgroup per slot.  Thread t owns the 16 entries of
+// fragment item t (row 16 (t / 64) + t % 16, points 16 ((t % 64) / 16) ..
+// +15): 16-byte loads of the four F planes, 64-byte loads of each gradient
+// row piece, the row maximum over the four threads of a row by shuffles,
+// 16-byte stores of the Z planes.
 __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restrict__ slot_b,
                                                   int gga, const double *__restrict__ coef,
                                                   signed char *__restrict__ store) {
   const int64_t sl = s0 + blockIdx.x;
   const int64_t b = slot_b[sl];
-  __shared__ double cf[4][PT];
-  __shared__ double zt[FB][65];
-  __shared__ int ez[FB];
-  {
-    const int c = threadIdx.x / PT, g = threadIdx.x % PT;
-    cf[c][g] = coef[4 * (b * PT + g) + c];
-  }
-  __syncthreads();
+  __shared__ double cf[PT][4];
+  cf[threadIdx.x >> 2][threadIdx.x & 3] = coef[4 * b * PT + threadIdx.x];
   signed char *tb = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
-  const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-  for (int e = threadIdx.x; e < TILE; e += 256) {
-    const int f = e / PT, g = e % PT;
-    double zz = cf[0][g] * unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
-    if (gga)
-      zz += cf[1][g] * (double)G[e] + cf[2][g] * (double)G[TILE + e] +
-            cf[3][g] * (double)G[2 * TILE + e];
-    zt[f][g] = zz;
-  }
+  const int t = threadIdx.x, l = t & 63;
+  const int row = 16 * (t >> 6) + (l & 15), k0 = 16 * (l >> 4);
+  v4i fp[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) fp[p] = ld16(tb + OFF_F + p * PLANE + t * 16);
+  const int eF = tb[OFF_F + 4 * PLANE + row];
+  const float *G = reinterpret_cast<const float *>(tb + OFF_G) + row * PT + k0;
   __syncthreads();
-  if (threadIdx.x < FB) {
-    double m = 0.0;
-    for (int k = 0; k < PT; ++k) m = fmax(m, fabs(zt[threadIdx.x][k]));
-    ez[threadIdx.x] = row_exp(m);
+  const signed char *c0 = reinterpret_cast<const signed char *>(&fp[0]);
+  const signed char *c1 = reinterpret_cast<const signed char *>(&fp[1]);
+  const signed char *c2 = reinterpret_cast<const signed char *>(&fp[2]);
+  const signed char *c3 = reinterpret_cast<const signed char *>(&fp[3]);
+  double z[16];
+  double m = 0.0;
+#pragma unroll
+  for (int q4 = 0; q4 < 4; ++q4) {
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx, gz = gx;
+    if (gga) {
+      gx = *reinterpret_cast<const float4 *>(G + 4 * q4);
+      gy = *reinterpret_cast<const float4 *>(G + TILE + 4 * q4);
+      gz = *reinterpret_cast<const float4 *>(G + 2 * TILE + 4 * q4);
+    }
+    const float gxa[4] = {gx.x, gx.y, gx.z, gx.w}, gya[4] = {gy.x, gy.y, gy.z, gy.w},
+                gza[4] = {gz.x, gz.y, gz.z, gz.w};
+#pragma unroll
+    for (int e4 = 0; e4 < 4; ++e4) {
+      const int e = 4 * q4 + e4, g = k0 + e;
+      double ph = 0.0;
+      if (eF != -127) {
+        const double v = (double)c0[e] + ldexp((double)c1[e], -7) + ldexp((double)c2[e], -14) +
+                         ldexp((double)c3[e], -21);
+        ph = ldexp(v, eF - 6);
+      }
+      const double zz = gga ? z_value(ph, (double)gxa[e4], (double)gya[e4], (double)gza[e4],
+                                      cf[g][0], cf[g][1], cf[g][2], cf[g][3])
+                            : cf[g][0] * ph;
+      z[e] = zz;
+      m = fmax(m, fabs(zz));
+    }
   }
-  __syncthreads();
-  store_section(zt, ez, tb + OFF_Z, false);
+  m = fmax(m, __shfl_xor(m, 16));
+  m = fmax(m, __shfl_xor(m, 32));
+  const int ez = row_exp(m);
+  __attribute__((aligned(16))) signed char out[4][16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    signed char sc[4];
+    slice4(z[e], ez, sc);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[p][e] = sc[p];
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    *reinterpret_cast<v4i *>(tb + OFF_Z + p * PLANE + t * 16) =
+        *reinterpret_cast<const v4i *>(out[p]);
+  if (l < 16) tb[OFF_Z + 4 * PLANE + row] = (signed char)ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,6 +1522,8 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     int nsp, int nsplit, const int4 *__restrict__ sp_blk, const int4 *__restrict__ sp_pidx,
     int npairs, const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1,
     int64_t s0, int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
+  // split-major: the super-pairs of one split run together and walk the same
+  // batches, sharing each batch's slots in the caches
   const int sp = blockIdx.x % nsp;
   const int split = blockIdx.x / nsp;
   const int4 blk = sp_blk[sp];
@@ -1184,11 +1619,13 @@ __global__ void __launch_bounds__(512) vgemm_i8(
       const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
       if (i + 1 < n) issue(i + 1, buf ^ 1);
       if (mine) {
+        // rows and columns stored as zero have zero slices: no zero test on
+        // the scales
         double sF[4], sZ[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          sF[r] = pow2_or_zero((int)(signed char)(eFc >> (8 * r)));
-          sZ[r] = pow2_or_zero((int)(signed char)(eZc >> (8 * r)));
+          sF[r] = pow2((int)(signed char)(eFc >> (8 * r)));
+          sZ[r] = pow2((int)(signed char)(eZc >> (8 * r)));
         }
 #pragma unroll
         for (int jb = 0; jb < 2; ++jb) {
@@ -1203,10 +1640,8 @@ __global__ void __launch_bounds__(512) vgemm_i8(
               BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
               BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
             }
-            const int eZJ = bz[4 * PLANE + 16 * j + l16];
-            const int eFJ = bf[4 * PLANE + 16 * j + l16];
-            const double sZJ = eZJ == -127 ? 0.0 : pow2(eZJ - 19);
-            const double sFJ = eFJ == -127 ? 0.0 : pow2(eFJ - 19);
+            const double sZJ = pow2((int)bz[4 * PLANE + 16 * j + l16] - 19);
+            const double sFJ = pow2((int)bf[4 * PLANE + 16 * j + l16] - 19);
             int h1[4], o1[4], h2[4], o2[4];
             tile_product(AF, BZ, h1, o1);
             tile_product(AZ, BF, h2, o2);
@@ -1283,6 +1718,30 @@ static void launch_density_x_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s
                      part);
 }
 
+static void launch_density_x22_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
+                                  const Plan &pl, const signed char *Ps, int gga,
+                                  const signed char *store, double *part) {
+  const int64_t ngroups = (b1 - b0 + 1) / 2;
+  const int64_t ngq = (ngroups + 7) / 8;
+  const int umax = std::min(pl.nfb, 2 * pl.max_mol_nfb);
+  const int nmg = (umax + 1) / 2;
+  const int64_t nwg = ngq * 8 * nmg;
+  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
+  hipLaunchKernelGGL(density_x22_i8, dim3((unsigned)nwg), dim3(512), 0, s, pl.tab(), pl.nfb,
+                     Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store, part);
+}
+
+static void launch_density_x4_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
+                                 const Plan &pl, const signed char *Ps, int gga,
+                                 const signed char *store, double *part) {
+  const int64_t nbq = (b1 - b0 + 7) / 8;
+  const int nquad = (pl.max_mol_nfb + 3) / 4;
+  const int64_t nwg = nbq * 8 * nquad;
+  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
+  hipLaunchKernelGGL(density_x4_i8, dim3((unsigned)nwg), dim3(512), 0, s, pl.tab(), Ps,
+                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nbq, gga, store, part);
+}
+
 template <int GBT>
 static void launch_density_x2_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
                                  const Plan &pl, const signed char *Ps, int gga,
@@ -1299,8 +1758,10 @@ static void launch_density_x2_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t 
 
 // shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1),
 // + 32 for the split form (contraction, functional, Z as three kernels),
-// 65 / 66: the split form with two row fragments per wave (1 / 2 batches);
-// other values take 33
+// 65 / 66: the split form with two row fragments per wave (1 / 2 batches),
+// 67: the split form for two batches x two blocks with both operands in LDS,
+// 129: the split form over block quads (one batch, eight waves);
+// other values take 65
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN, double *part) {
@@ -1308,11 +1769,15 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
   if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
-      shape != 49 && shape != 50 && shape != 65 && shape != 66)
-    shape = 33;
-  if (shape & 96) {
+      shape != 49 && shape != 50 && shape != 65 && shape != 66 && shape != 67 && shape != 129)
+    shape = 65;
+  if (shape & 224) {
     NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "split density form without a partial buffer");
-    if (shape & 64) {
+    if (shape == 67) {
+      launch_density_x22_i8(s, b0, b1, s0, pl, Ps, gga, store, part);
+    } else if (shape == 129) {
+      launch_density_x4_i8(s, b0, b1, s0, pl, Ps, gga, store, part);
+    } else if (shape & 64) {
       if (shape == 65)
         launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part);
       else
@@ -1326,15 +1791,7 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
     }
     }
     NKX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
-                       pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN);
-    NKX_HIP_CHECK(hipGetLastError());
-    const int64_t s1 = pl.b_off_h[b1];
-    if (s1 > s0)
-      hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
-                         pl.slot_b.p, gga, coef, store);
-    NKX_HIP_CHECK(hipGetLastError());
-    return;
+    return;  // functional and Z: vkm_density_finish
   }
   switch (shape) {
     case 1: launch_density_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
@@ -1343,6 +1800,27 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
     case 18: launch_density_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
     default: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
   }
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+// Second half of the split form (shape with bit 32, 64 or 128): per-batch
+// functional from the slot partial sums, then the Z slices of every slot.
+// Nothing for the fused shapes.
+void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0,
+                        const Plan &pl, int gga, signed char *store, const XcList &xl,
+                        double *coef, double *blockEN, const double *part) {
+  if (b1 <= b0) return;
+  if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
+      shape != 49 && shape != 50 && shape != 65 && shape != 66 && shape != 67 && shape != 129)
+    shape = 65;
+  if (!(shape & 224)) return;
+  hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
+                     pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN);
+  NKX_HIP_CHECK(hipGetLastError());
+  const int64_t s1 = pl.b_off_h[b1];
+  if (s1 > s0)
+    hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+                       pl.slot_b.p, gga, coef, store);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

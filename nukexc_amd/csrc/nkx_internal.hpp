@@ -180,7 +180,7 @@ struct Workspace {
 // (the reference's driver-level TimingRegistry, standalone_helpers.hpp:71-156,
 // at kernel granularity).
 enum Stage { kStageColloc = 0, kStageDensity = 1, kStageVgemm = 2, kStageFinal = 3,
-             kStageBuild = 4, kNumStages = 5 };
+             kStageBuild = 4, kStageZ = 5, kNumStages = 6 };
 struct Timing {
   bool on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;

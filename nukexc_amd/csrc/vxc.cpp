@@ -465,13 +465,14 @@ int64_t chunk_slot_budget(size_t slot_bytes) {
   return (int64_t)(budget / slot_bytes);
 }
 
-// Split count of the V contraction over batches: enough workgroups to fill
-// the chip (about 2048 of the FP64 kernel's pair tiles, 8192 of the int8
-// kernel's super-pair tiles), or the autotuned value.
+// Split count of the V contraction over batches: about 8192 workgroups of
+// the FP64 kernel's pair tiles or of the int8 kernel's super-pair tiles
+// (a few per CU at a time, the rest balancing the tail), or the autotuned
+// value.
 int choose_nsplit(const Plan &pl, int tuned = 0, bool super = false) {
   const int64_t units = super ? pl.nspairs : pl.npairs;
   if (units == 0) return 1;
-  const int64_t target = super ? 8192 : 2048;
+  const int64_t target = 8192;
   int64_t ns = tuned > 0 ? tuned : (target + units - 1) / units;
   ns = std::max<int64_t>(1, std::min<int64_t>(ns, pl.nb));
   return (int)ns;
@@ -513,9 +514,9 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
                          int tuned_split, int tuned_group) {
   BuildSetup b;
   b.nspin = nspin;
-  // density workgroup shape: tuned, else 4 batches per workgroup (FP64) /
+  // density workgroup shape: tuned, else 2 batches per workgroup (FP64) /
   // the split form with 32 rows per wave and one batch (int8 form)
-  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 65 : 4);
+  b.group = tuned_group > 0 ? tuned_group : (precision == NKX_PREC_MIXED ? 65 : 2);
   b.gga = gga;
   const int nmol = pl.nmol;
   b.narr = (gga ? 4 : 1) + nspin;
@@ -601,9 +602,14 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
                   ws.blockEN.p, ws.dpart.p);
     else
       vk_density(s, nspin, b.group, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl,
-                 ws.coef.p,
-                 ws.blockEN.p, nullptr);
+                 ws.coef.p, ws.blockEN.p, nullptr);
     tm.end(tt, s);
+    if (mixed && !bf16) {
+      tt = tm.begin(kStageZ, s);
+      vkm_density_finish(s, b.group, b0, b1, s0, pl, ig, ws.mstore.p, xl, ws.coef.p,
+                         ws.blockEN.p, ws.dpart.p);
+      tm.end(tt, s);
+    }
     tt = tm.begin(kStageVgemm, s);
     if (bf16)
       vkb_vgemm(s, 6, b0, b1, s0, pl, nsplit, ig, bstore, ws.vpart.p);
@@ -679,13 +685,13 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
   try {
     c.opt.vgemm_split = base;
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
-    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 65 : 4);
-    // density shapes: batches per workgroup (int8 form: + 16 for two
-    // function blocks per step, + 32 for the split form, 65 / 66 the split
-    // form with two row fragments per wave; the fused shapes other than 18
-    // are never the fastest on large grids and are not timed)
+    int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 65 : 2);
+    // density shapes: batches per workgroup (int8 form: the split-form
+    // shapes 65 (32 rows per wave), 67 (two batches x two blocks, both
+    // operands in LDS), 129 (block quads) and 33; the fused shapes and 34,
+    // 49, 50, 66 are never the fastest on large grids and are not timed)
     const std::vector<int> shapes =
-        precision == NKX_PREC_MIXED ? std::vector<int>{33, 49, 50, 65, 66, 18}
+        precision == NKX_PREC_MIXED ? std::vector<int>{65, 67, 129, 33}
                                     : std::vector<int>{1, 2, 4};
     if (!bf16)
       for (int g : shapes) {

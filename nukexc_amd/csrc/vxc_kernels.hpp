@@ -40,6 +40,9 @@ void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN, double *part);
+void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0,
+                        const Plan &pl, int gga, signed char *store, const XcList &xl,
+                        double *coef, double *blockEN, const double *part);
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
                int gga, const signed char *store, double *vpart);
 

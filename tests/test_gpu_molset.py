@@ -36,7 +36,8 @@ def setup():
 
 @pytest.mark.parametrize("precision,shape", [("fp64", 0), ("mixed", 0), ("mixed", 1),
                                              ("mixed", 17), ("mixed", 18), ("mixed", 33),
-                                             ("mixed", 50), ("mixed", 65), ("mixed", 66)])
+                                             ("mixed", 50), ("mixed", 65), ("mixed", 66),
+                                             ("mixed", 67), ("mixed", 129)])
 def test_molset_matches_single_builds(setup, precision, shape):
     """shape: the density workgroup shape of the set build (the single builds
     use the default); 17, 18 and 50 put blocks of two molecules in one step,
