@@ -158,7 +158,12 @@ int64_t mixed_slot_bytes(int gga) { return slot_bytes_i8(gga); }
 
 // ---------------------------------------------------------------------------
 // Collocation: one workgroup per live slot; FP64 values in LDS, row maxima
-// along functions (F) and along points (Pm), slices in both orders.
+// along functions (F) and along points (Pm), slices in both orders.  Thread
+// t evaluates the 16 consecutive functions 16 (t / 64) .. +15 of the block
+// at point t % 64 (one function at a time for the whole wave, a point per
+// lane).  Functions are sorted by centre, so consecutive functions that
+// share the centre share dx, dy, dz, r and 1/r, and those that also share
+// zeta, k and the normalisation share the radial factor.
 __global__ void __launch_bounds__(256) colloc_tiles_i8(
     int64_t s0, const int *__restrict__ slot_b, const int *__restrict__ b_fbl,
     const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
@@ -173,25 +178,52 @@ __global__ void __launch_bounds__(256) colloc_tiles_i8(
   if (threadIdx.x < FB) fr[threadIdx.x] = F[fb * FB + threadIdx.x];
   __syncthreads();
   const int g = threadIdx.x & (PT - 1);
-  const int row = threadIdx.x / PT;
+  const int f0 = 16 * (threadIdx.x / PT);
   const int64_t gp = (int64_t)b * PT + g;
   const double x = px[gp], y = py[gp], z = pz[gp];
   signed char *base = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
   float *G = reinterpret_cast<float *>(base + OFF_G);
+  double dx = 0.0, dy = 0.0, dz = 0.0, r = 1.0, rinv = 1.0, R = 0.0, common = 0.0;
 #pragma unroll 1
-  for (int f = row; f < FB; f += 256 / PT) {
+  for (int i = 0; i < 16; ++i) {
+    const int f = f0 + i;
     const FuncRec rec = fr[f];
-    double val;
-    if (gga) {
-      float a, bb, c;
-      sto_value_grad_mixed(rec, T, x, y, z, val, a, bb, c);
-      G[f * PT + g] = a;
-      G[TILE + f * PT + g] = bb;
-      G[2 * TILE + f * PT + g] = c;
-    } else {
-      val = sto_value_mixed(rec, T, x, y, z);
+    const bool newc =
+        i == 0 || rec.cx != fr[f - 1].cx || rec.cy != fr[f - 1].cy || rec.cz != fr[f - 1].cz;
+    if (newc) {
+      dx = x - rec.cx;
+      dy = y - rec.cy;
+      dz = z - rec.cz;
+      r = sqrt(dx * dx + dy * dy + dz * dz);
+      rinv = 1.0 / r;
     }
-    v[f][g] = val;
+    if (newc || rec.zeta != fr[f - 1].zeta || rec.k != fr[f - 1].k ||
+        rec.norm != fr[f - 1].norm) {
+      R = rec.norm * ipow(r, rec.k) * exp_neg_mixed(rec.zeta * r);
+      common = ((double)rec.k * rinv - rec.zeta) * R * rinv;
+    }
+    double S = 0.0, Sx = 0.0, Sy = 0.0, Sz = 0.0;
+    for (int t = 0; t < rec.term_cnt; ++t) {
+      const HarmTerm h = T[rec.term_off + t];
+      const double xa1 = ipow(dx, h.a > 0 ? h.a - 1 : 0);
+      const double yb1 = ipow(dy, h.b > 0 ? h.b - 1 : 0);
+      const double zc1 = ipow(dz, h.c > 0 ? h.c - 1 : 0);
+      const double xa = h.a > 0 ? xa1 * dx : 1.0;
+      const double yb = h.b > 0 ? yb1 * dy : 1.0;
+      const double zc = h.c > 0 ? zc1 * dz : 1.0;
+      S += h.coef * xa * yb * zc;
+      if (gga) {
+        if (h.a > 0) Sx += h.coef * h.a * xa1 * yb * zc;
+        if (h.b > 0) Sy += h.coef * h.b * xa * yb1 * zc;
+        if (h.c > 0) Sz += h.coef * h.c * xa * yb * zc1;
+      }
+    }
+    v[f][g] = R * S;
+    if (gga) {
+      G[f * PT + g] = (float)(R * Sx + S * (dx * common));
+      G[TILE + f * PT + g] = (float)(R * Sy + S * (dy * common));
+      G[2 * TILE + f * PT + g] = (float)(R * Sz + S * (dz * common));
+    }
   }
   __syncthreads();
   if (threadIdx.x < FB) {
@@ -1552,11 +1584,14 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   for (int u = 0; u < 4; ++u) tso[u] = bl[u] >= 0 ? (int64_t)mt.blk_local[bl[u]] * mnb : -1;
   const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
 
-  v4i AnF[4], AnZ[4];
-  int eFn = 0, eZn = 0;
-  // J sections -> LDS buffer dbuf (direct copies); this wave's F and Z
-  // fragments of its I slot -> registers
-  auto issue = [&](int i, int dbuf) {
+  // this wave's F and Z fragments of its I slot and their row exponents
+  // (two register sets, used in turn)
+  struct AFrag {
+    v4i F[4], Z[4];
+    int eF, eZ;
+  };
+  // J sections -> LDS buffer dbuf (direct copies); the I fragments -> an
+  auto issue = [&](int i, int dbuf, AFrag &an) {
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb) {
       const int sJ = ls[2 + jb][i];
@@ -1570,14 +1605,60 @@ __global__ void __launch_bounds__(512) vgemm_i8(
       const signed char *tI = store + (sI - s0) * sb_bytes;
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) {
-        AnF[pl] = ld16(tI + OFF_F + pl * PLANE + (rw * 64 + lane) * 16);
-        AnZ[pl] = ld16(tI + OFF_Z + pl * PLANE + (rw * 64 + lane) * 16);
+        an.F[pl] = ld16(tI + OFF_F + pl * PLANE + (rw * 64 + lane) * 16);
+        an.Z[pl] = ld16(tI + OFF_Z + pl * PLANE + (rw * 64 + lane) * 16);
       }
-      eFn = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * rw + 4 * q);
-      eZn = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * rw + 4 * q);
+      an.eF = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * rw + 4 * q);
+      an.eZ = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * rw + 4 * q);
+    }
+  };
+  // one batch of the list: wait for its operands, start the next batch's
+  // copies into the other buffer and register set, then the tile products
+  auto step = [&](int i, int n, int buf, const AFrag &ac, AFrag &an) {
+    wait_vm0();
+    __syncthreads();  // buffer buf complete, buf ^ 1 free
+    const bool mine = ls[half][i] >= 0;
+    const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
+    if (i + 1 < n) issue(i + 1, buf ^ 1, an);
+    if (!mine) return;
+    // rows and columns stored as zero have zero slices: no zero test on the
+    // scales
+    double sF[4], sZ[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sF[r] = pow2((int)(signed char)(ac.eF >> (8 * r)));
+      sZ[r] = pow2((int)(signed char)(ac.eZ >> (8 * r)));
+    }
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+      if (!jl[jb]) continue;
+      const signed char *bz = &bbuf[buf][2 * jb][0];
+      const signed char *bf = &bbuf[buf][2 * jb + 1][0];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4i BZ[4], BF[4];
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+          BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
+          BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
+        }
+        const double sZJ = pow2((int)bz[4 * PLANE + 16 * j + l16] - 19);
+        const double sFJ = pow2((int)bf[4 * PLANE + 16 * j + l16] - 19);
+        int h1[4], o1[4], h2[4], o2[4];
+        tile_product(ac.F, BZ, h1, o1);
+        tile_product(ac.Z, BF, h2, o2);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v64[jb][j][r] =
+              fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[jb][j][r]);
+          v64[jb][j][r] =
+              fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[jb][j][r]);
+        }
+      }
     }
   };
 
+  AFrag a0, a1;
   for (int64_t c0 = lo + split; c0 < hi; c0 += 512 * (int64_t)nsplit) {
     const int64_t b = c0 + (int64_t)nsplit * threadIdx.x;
     int sl[4] = {-1, -1, -1, -1};
@@ -1603,59 +1684,10 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     }
     __syncthreads();
     if (n == 0) continue;
-    issue(0, 0);
-    int buf = 0;
-    for (int i = 0; i < n; ++i) {
-      wait_vm0();
-      __syncthreads();  // buffer buf complete, buf ^ 1 free
-      v4i AF[4], AZ[4];
-#pragma unroll
-      for (int pl = 0; pl < 4; ++pl) {
-        AF[pl] = AnF[pl];
-        AZ[pl] = AnZ[pl];
-      }
-      const int eFc = eFn, eZc = eZn;
-      const bool mine = ls[half][i] >= 0;
-      const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
-      if (i + 1 < n) issue(i + 1, buf ^ 1);
-      if (mine) {
-        // rows and columns stored as zero have zero slices: no zero test on
-        // the scales
-        double sF[4], sZ[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          sF[r] = pow2((int)(signed char)(eFc >> (8 * r)));
-          sZ[r] = pow2((int)(signed char)(eZc >> (8 * r)));
-        }
-#pragma unroll
-        for (int jb = 0; jb < 2; ++jb) {
-          if (!jl[jb]) continue;
-          const signed char *bz = &bbuf[buf][2 * jb][0];
-          const signed char *bf = &bbuf[buf][2 * jb + 1][0];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            v4i BZ[4], BF[4];
-#pragma unroll
-            for (int pl = 0; pl < 4; ++pl) {
-              BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
-              BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
-            }
-            const double sZJ = pow2((int)bz[4 * PLANE + 16 * j + l16] - 19);
-            const double sFJ = pow2((int)bf[4 * PLANE + 16 * j + l16] - 19);
-            int h1[4], o1[4], h2[4], o2[4];
-            tile_product(AF, BZ, h1, o1);
-            tile_product(AZ, BF, h2, o2);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              v64[jb][j][r] =
-                  fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[jb][j][r]);
-              v64[jb][j][r] =
-                  fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[jb][j][r]);
-            }
-          }
-        }
-      }
-      buf ^= 1;
+    issue(0, 0, a0);
+    for (int i = 0; i < n; i += 2) {
+      step(i, n, 0, a0, a1);
+      if (i + 1 < n) step(i + 1, n, 1, a1, a0);
     }
   }
   const int4 pix = sp_pidx[sp];
