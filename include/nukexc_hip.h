@@ -261,6 +261,15 @@ int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P_dev, int nfunc, const in
    NKX_ERR_STATE until the next capture. */
 int nkx_graph_capture(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                       int precision, double *V_dev);
+/* The same for one nkx_molset_vxc build of the ctx molecule set (no
+   counterpart in the reference): P_dev[m], V_dev[m] per molecule; replays
+   with nkx_graph_launch, which then fills E_xc[m] and N_el[m] for every
+   molecule.  A later nkx_molset_create makes the replay fail with
+   NKX_ERR_STATE. */
+int nkx_molset_graph_capture(nkx_ctx *ctx, const double *const *P_dev, int nfunc,
+                             const int *func_ids, int precision, double *const *V_dev);
+/* Replays the last capture; E_xc and N_el hold one value per captured
+   molecule (one for nkx_graph_capture). */
 int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
 /* Autotuner (no counterpart in the reference): on the ctx grid and basis for
    this P, times one nkx_vxc build per candidate -- first the density
@@ -285,6 +294,11 @@ int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_i
    form); the FP64 form reads the int8 codes modulo 16, the int8 form reads
    4 as 65. */
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
+/* Device-memory budget of a build's tile store in bytes (no counterpart in
+   the reference): a build whose tiles do not fit runs in chunks of batches.
+   0 (default) = min(48 GB, 40 % of the free memory at build time); values
+   below 64 MB count as 64 MB.  For many contexts on one GPU. */
+int nkx_set_memory_budget(nkx_ctx *ctx, int64_t bytes);
 int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size (64 or 128; 0 keeps the current value). */

@@ -73,9 +73,12 @@ SIGNATURES = {
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_autotune": [_vp, _vp, _c_int, _pi, _c_int, _vp, _pi, _pi, _pd],
     "nkx_set_launch": [_vp, _c_int, _c_int],
+    "nkx_set_memory_budget": [_vp, _c_i64],
     "nkx_get_launch": [_vp, _pi, _pi],
     "nkx_graph_capture": [_vp, _vp, _c_int, _pi, _c_int, _vp],
     "nkx_graph_launch": [_vp, _pd, _pd],
+    "nkx_molset_graph_capture": [_vp, ctypes.POINTER(_vp), _c_int, _pi, _c_int,
+                                 ctypes.POINTER(_vp)],
     "nkx_molset_create": [_vp, _c_int, _pi, _pd, _pi, _c_int, _c_int, _c_int, _c_double,
                           _c_double, _c_int, _pi, _pi, _pi, _pi, _pd, _pd, _pd, _pi64],
     "nkx_molset_vxc": [_vp, ctypes.POINTER(_vp), _c_int, _pi, _c_int, _pd, _pd,

@@ -316,6 +316,11 @@ class Context:
         65, 66, 67, 129 (int8 form)."""
         self._chk(lib().nkx_set_launch(self.h, int(vgemm_split), int(density_shape)))
 
+    def set_memory_budget(self, nbytes=0):
+        """Tile-store bytes per build (0: automatic); larger builds run in
+        chunks of batches."""
+        self._chk(lib().nkx_set_memory_budget(self.h, int(nbytes)))
+
     def get_launch(self):
         """Current (vgemm_split, density_shape)."""
         k, g = ctypes.c_int(), ctypes.c_int()
@@ -332,9 +337,12 @@ class Context:
         ids, nf = _ids(funcs)
         self._graph_keep = (P, V)
         self._chk(lib().nkx_graph_capture(self.h, _ptr(P), nf, ids, prec, _ptr(V)))
+        self._graph_nmol = 1
 
     def graph_launch(self):
         """Replay the captured build; returns (E_xc, N_el)."""
+        if getattr(self, "_graph_nmol", 1) != 1:
+            raise RuntimeError("the captured build is a molecule set: use molset_graph_launch")
         E, N = ctypes.c_double(), ctypes.c_double()
         self._chk(lib().nkx_graph_launch(self.h, ctypes.byref(E), ctypes.byref(N)))
         return E.value, N.value
@@ -384,6 +392,29 @@ class Context:
         ids, nf = _ids(funcs)
         self._chk(lib().nkx_molset_vxc(self.h, Pp, nf, ids, prec, E, N, Vp))
         return [XC_result(E[k], Vs[k], N[k]) for k in range(nmol)]
+
+    def molset_graph_capture(self, Ps, Vs, funcs=PBE, precision="mixed"):
+        """Capture one build of the molecule set for the device tensors Ps
+        (inputs) and Vs (outputs); replay with molset_graph_launch()."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        nmol = len(self._molset_nbf)
+        assert len(Ps) == nmol and len(Vs) == nmol
+        for P, V in zip(Ps, Vs):
+            assert P.is_contiguous() and V.is_contiguous() and P.dtype == torch.float64
+        self._graph_keep = (list(Ps), list(Vs))
+        Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Ps])
+        Vp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Vs])
+        ids, nf = _ids(funcs)
+        self._chk(lib().nkx_molset_graph_capture(self.h, Pp, nf, ids, prec, Vp))
+        self._graph_nmol = nmol
+
+    def molset_graph_launch(self):
+        """Replay the captured molecule-set build; returns (E list, N list)."""
+        n = self._graph_nmol
+        E = (ctypes.c_double * n)()
+        N = (ctypes.c_double * n)()
+        self._chk(lib().nkx_graph_launch(self.h, E, N))
+        return list(E), list(N)
 
     STAGES = ("collocation", "density", "vgemm", "final", "build", "functional_z")
 

@@ -725,6 +725,25 @@ int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, 
   });
 }
 
+int nkx_molset_graph_capture(nkx_ctx *ctx, const double *const *P, int nfunc, const int *ids,
+                             int precision, double *const *V) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(c.mplan.nmol > 0, NKX_ERR_STATE, "no molecule set: call nkx_molset_create");
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_UNSUPPORTED, "molecule sets build in the FP64 or the int8 mixed form");
+    NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "molset_graph_capture: bad arguments");
+    for (int k = 0; k < c.mplan.nmol; ++k)
+      NKX_REQUIRE(P[k] && V[k], NKX_ERR_INVALID_ARGUMENT, "molset_graph_capture: null P or V");
+    nkx::molset_graph_capture(c, P, xl, gga, precision, V);
+  });
+}
+
 int nkx_graph_launch(nkx_ctx *ctx, double *E, double *N) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
@@ -762,6 +781,14 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
                 "density shape must be 0, 1, 2, 4, 17, 18, 33, 34, 49, 50, 65, 66, 67 or 129");
     ctx->c.opt.vgemm_split = vgemm_split;
     ctx->c.opt.density_group = density_shape;
+  });
+}
+
+int nkx_set_memory_budget(nkx_ctx *ctx, int64_t bytes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(bytes >= 0, NKX_ERR_INVALID_ARGUMENT, "memory budget must be >= 0");
+    ctx->c.opt.mem_budget = (size_t)bytes;
   });
 }
 

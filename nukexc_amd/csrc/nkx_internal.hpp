@@ -157,6 +157,7 @@ struct Options {
   int vgemm_split = 0;          // V-contraction split count; 0 = automatic
   int density_group = 0;        // density workgroup shape code; 0 = form default
   double dens_threshold = 0.0;  // <= 0: per-functional libxc default
+  size_t mem_budget = 0;        // tile-store bytes per build; 0: min(48 GB, 40 % of free)
 };
 
 struct Workspace {
@@ -207,7 +208,10 @@ struct Work {
 struct GraphState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  double *en_pinned = nullptr;  // (E, N_alpha, N_beta), written by the graph
+  double *en_pinned = nullptr;  // (E, N_alpha, N_beta) per molecule, written by the graph
+  int nmol = 1;
+  bool molset = false;          // captured on the molecule-set plan
+  uint64_t mplan_version = 0;
   std::vector<double *> vtab_h;
   DevBuf<double *> vtab;
   Workspace ws;
@@ -238,6 +242,7 @@ struct Ctx {
   uint64_t basis_version = 0;
   Plan plan;
   Plan mplan;                    // plan of the molecule set (nkx_molset_create)
+  uint64_t mplan_version = 0;    // bumped by every nkx_molset_create
   Workspace ws;
   GraphState graph;
 };
@@ -307,6 +312,8 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
 void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
                    double *V);
 void graph_launch(Ctx &c, double *E, double *N);
+void molset_graph_capture(Ctx &c, const double *const *P, const XcList &xl, bool gga,
+                          int precision, double *const *V);
 double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
                 int *best_split, int *best_group);
 void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,

@@ -452,15 +452,20 @@ void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
     parts.push_back(make_part(mol_pts[m], mol_w[m], bases[m], c.opt.screen_tol));
   upload_plan(c.mplan, parts, c.stream);
   c.mplan.screen_tol = c.opt.screen_tol;
+  ++c.mplan_version;
 }
 
 namespace {
 
 // Number of slots per chunk so that the chunk's tiles fit the budget.
-int64_t chunk_slot_budget(size_t slot_bytes) {
-  size_t freeb = 0, total = 0;
-  NKX_HIP_CHECK(hipMemGetInfo(&freeb, &total));
-  size_t budget = std::min<size_t>((size_t)48 << 30, freeb / 5 * 2);
+// cap: the ctx's tile-store budget in bytes, 0 for min(48 GB, 40 % of free).
+int64_t chunk_slot_budget(size_t slot_bytes, size_t cap) {
+  size_t budget = cap;
+  if (budget == 0) {
+    size_t freeb = 0, total = 0;
+    NKX_HIP_CHECK(hipMemGetInfo(&freeb, &total));
+    budget = std::min<size_t>((size_t)48 << 30, freeb / 5 * 2);
+  }
   budget = std::max<size_t>(budget, (size_t)64 << 20);
   return (int64_t)(budget / slot_bytes);
 }
@@ -511,7 +516,7 @@ double *const *upload_ptrs(Ctx &c, const std::vector<double *> &ptrs) {
 // Sizes and workspace of one build on plan pl (allocations happen here, so a
 // captured build performs none).
 BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int precision,
-                         int tuned_split, int tuned_group) {
+                         int tuned_split, int tuned_group, size_t mem_cap) {
   BuildSetup b;
   b.nspin = nspin;
   // density workgroup shape: tuned, else 2 batches per workgroup (FP64) /
@@ -530,7 +535,7 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
   const size_t slot_bytes = b.bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
                             : b.mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
                                       : (size_t)b.narr * FB * PT * sizeof(double);
-  int64_t max_slots = chunk_slot_budget(slot_bytes);
+  int64_t max_slots = chunk_slot_budget(slot_bytes, mem_cap);
   // at least the largest single batch
   for (int64_t bb = 0; bb < pl.nb; ++bb)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[bb + 1] - pl.b_off_h[bb]);
@@ -636,7 +641,8 @@ void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const Xc
                    bool gga, int precision, double *E, double *N, double *const *V) {
   const int nmol = pl.nmol;
   const BuildSetup b =
-      prepare_build(c.ws, pl, nspin, gga, precision, c.opt.vgemm_split, c.opt.density_group);
+      prepare_build(c.ws, pl, nspin, gga, precision, c.opt.vgemm_split, c.opt.density_group,
+                    c.opt.mem_budget);
   double *const *Vtab = upload_ptrs(c, std::vector<double *>(V, V + (size_t)nspin * nmol));
   std::vector<double> en((size_t)3 * nmol);
   enqueue_build(c, c.ws, pl, b, P, xl, V, Vtab, en.data());
@@ -731,24 +737,27 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
 }
 
 // ---- HIP graph of one build on the ctx grid and basis ----------------------
-void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
-                   double *V) {
-  ensure_plan(c);
+namespace {
+// Capture one build on plan pl (nmol molecules: P[m], V[m]) into c.graph.
+void capture_on_plan(Ctx &c, Plan &pl, const double *const *P, const XcList &xl, bool gga,
+                     int precision, double *const *V) {
   GraphState &gs = c.graph;
   gs.release();
-  Plan &pl = c.plan;
+  const int nmol = pl.nmol;
   const BuildSetup b =
-      prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split, c.opt.density_group);
-  NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned), 3 * sizeof(double),
-                              hipHostMallocDefault));
-  gs.vtab_h.assign(1, V);
-  gs.vtab.upload(gs.vtab_h.data(), 1, c.stream);
+      prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split, c.opt.density_group,
+                    c.opt.mem_budget);
+  NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned),
+                              3 * (size_t)nmol * sizeof(double), hipHostMallocDefault));
+  gs.nmol = nmol;
+  gs.vtab_h.assign(V, V + nmol);
+  gs.vtab.upload(gs.vtab_h.data(), nmol, c.stream);
   NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
   const bool timing = c.timing.on;
   c.timing.on = false;  // no event records inside the graph
   NKX_HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
   try {
-    enqueue_build(c, gs.ws, pl, b, &P, xl, &V, gs.vtab.p, gs.en_pinned);
+    enqueue_build(c, gs.ws, pl, b, P, xl, V, gs.vtab.p, gs.en_pinned);
   } catch (...) {
     hipGraph_t broken = nullptr;
     (void)hipStreamEndCapture(c.stream, &broken);
@@ -759,21 +768,47 @@ void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int prec
   c.timing.on = timing;
   NKX_HIP_CHECK(hipStreamEndCapture(c.stream, &gs.graph));
   NKX_HIP_CHECK(hipGraphInstantiate(&gs.exec, gs.graph, nullptr, nullptr, 0));
+}
+}  // namespace
+
+void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
+                   double *V) {
+  ensure_plan(c);
+  capture_on_plan(c, c.plan, &P, xl, gga, precision, &V);
+  GraphState &gs = c.graph;
+  gs.molset = false;
   gs.grid_version = c.grid.version;
   gs.basis_version = c.basis_version;
   gs.screen_tol = c.opt.screen_tol;
 }
 
+void molset_graph_capture(Ctx &c, const double *const *P, const XcList &xl, bool gga,
+                          int precision, double *const *V) {
+  NKX_REQUIRE(c.mplan.nmol > 0, NKX_ERR_STATE, "no molecule set: call nkx_molset_create");
+  capture_on_plan(c, c.mplan, P, xl, gga, precision, V);
+  GraphState &gs = c.graph;
+  gs.molset = true;
+  gs.mplan_version = c.mplan_version;
+  gs.screen_tol = c.mplan.screen_tol;
+}
+
+// Replay the captured build; E[m], N[m] for each molecule of the capture.
 void graph_launch(Ctx &c, double *E, double *N) {
   GraphState &gs = c.graph;
   NKX_REQUIRE(gs.exec, NKX_ERR_STATE, "no captured build: call nkx_graph_capture");
-  NKX_REQUIRE(gs.grid_version == c.grid.version && gs.basis_version == c.basis_version &&
-                  gs.screen_tol == c.opt.screen_tol,
-              NKX_ERR_STATE, "grid, basis or screening changed since the capture");
+  if (gs.molset)
+    NKX_REQUIRE(gs.mplan_version == c.mplan_version, NKX_ERR_STATE,
+                "the molecule set changed since the capture");
+  else
+    NKX_REQUIRE(gs.grid_version == c.grid.version && gs.basis_version == c.basis_version &&
+                    gs.screen_tol == c.opt.screen_tol,
+                NKX_ERR_STATE, "grid, basis or screening changed since the capture");
   NKX_HIP_CHECK(hipGraphLaunch(gs.exec, c.stream));
   NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
-  *E = gs.en_pinned[0];
-  if (N) *N = gs.en_pinned[1];
+  for (int m = 0; m < gs.nmol; ++m) {
+    E[m] = gs.en_pinned[3 * m];
+    if (N) N[m] = gs.en_pinned[3 * m + 1];
+  }
 }
 
 void GraphState::release() {
@@ -827,7 +862,8 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
   const int narr = 5;  // Phi, grad Phi (3), Z
   const size_t slot_bytes = (size_t)narr * FB * PT * sizeof(double);
   // tiles plus the per-slot gradient partials
-  int64_t max_slots = chunk_slot_budget(slot_bytes + 3 * FB * sizeof(double));
+  int64_t max_slots =
+      chunk_slot_budget(slot_bytes + 3 * FB * sizeof(double), c.opt.mem_budget);
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
@@ -919,7 +955,8 @@ void overlap_build(Ctx &c, double *S) {
   hipStream_t s = c.stream;
   const int narr = 2;
   const int nsplit = choose_nsplit(pl);
-  int64_t max_slots = chunk_slot_budget((size_t)narr * FB * PT * sizeof(double));
+  int64_t max_slots =
+      chunk_slot_budget((size_t)narr * FB * PT * sizeof(double), c.opt.mem_budget);
   for (int64_t b = 0; b < pl.nb; ++b)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));

@@ -112,3 +112,43 @@ def test_xc_grad_rejects_off_atom_functions(ctx):
     with pytest.raises(nk.NukexcError) as ei:
         ctx.xc_grad(torch.tensor(P, device="cuda"), funcs=(1,))
     assert ei.value.status == "NKX_ERR_INVALID_ARGUMENT"
+
+
+def test_xc_grad_mol30_vs_fd():
+    """The 30-atom differentiation workload (TA 75 x Lebedev 302, PBE, 90
+    components): the analytic gradient against central differences with
+    h = 1e-4 bohr of the FP64 E_xc of the same build path (which the parity
+    tests pin to the oracle), for six coordinates, and sum_A dE/dR_A = 0."""
+    import time
+    c = nk.Context(0)
+    try:
+        w = configs.workload("mol30")
+        grid = c.make_flat_grid(w["xyz"], w["Z"], **w["grid"])
+        c.set_basis(w["basis"])
+        S = c.overlap().cpu().numpy()
+        C, P = configs.loewdin_orbitals(S, w["nocc"], w["rng"])
+        fa = _fn_atom(w["basis"], w["xyz"])
+        Pt = torch.tensor(P, device="cuda")
+        funcs = (101, 130)
+        c.xc_grad(Pt, funcs=funcs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E, g = c.xc_grad(Pt, funcs=funcs)
+        t1 = time.perf_counter()
+        print(f"mol30: {len(w['Z'])} atoms, {grid.weights.shape[0]} points, "
+              f"{w['basis'].nbf()} functions; gradient {1e3 * (t1 - t0):.1f} ms")
+        h = 1e-4
+        for a, k in [(0, 0), (7, 1), (13, 2), (21, 0), (25, 1), (29, 2)]:
+            e = []
+            for t in (1, -1):
+                xyz = w["xyz"].copy()
+                xyz[a, k] += t * h
+                c.make_flat_grid(xyz, w["Z"], **w["grid"])
+                c.set_basis(_moved(w["basis"], fa, xyz))
+                e.append(c.vxc(Pt, funcs=funcs, precision="fp64").energy)
+            fd = (e[0] - e[1]) / (2 * h)
+            print(f"atom {a} comp {k}: g={g[a, k]:.10f} fd={fd:.10f} diff={g[a, k] - fd:.1e}")
+            assert abs(g[a, k] - fd) <= 1e-6
+        assert np.abs(g.sum(0)).max() <= 1e-9
+    finally:
+        c.close()

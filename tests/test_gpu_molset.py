@@ -76,3 +76,31 @@ def test_molset_rejects_split_bf16(setup):
     ids = (ctypes.c_int * 1)(101)
     rc = nk.lib().nkx_molset_vxc(c.h, Pp, 1, ids, nk._capi.PREC_MIXED_BF16, E, None, Vp)
     assert nk._capi.STATUS_NAMES[rc] == "NKX_ERR_UNSUPPORTED"
+
+
+def test_molset_graph_replays_match_eager(setup):
+    """One int8 build of the whole set captured as a HIP graph: replays after
+    P changes in place give the eager set build's E, N and V for every
+    molecule, and a later molset_create invalidates the graph."""
+    c, mols, Ps, singles = setup
+    c.molset_create(mols, **GRID)
+    c.set_launch(0, 0)
+    Pd = [torch.tensor(P, device="cuda") for P in Ps]
+    Vs = [torch.empty_like(P) for P in Pd]
+    c.molset_graph_capture(Pd, Vs, funcs=(101, 130), precision="mixed")
+    for rep in range(3):
+        scale = 1.0 + 0.01 * rep
+        for P, P0 in zip(Pd, Ps):
+            P.copy_(torch.tensor(P0 * scale, device="cuda"))
+        torch.cuda.synchronize()
+        E, N = c.molset_graph_launch()
+        eager = c.molset_vxc([P.clone() for P in Pd], funcs=(101, 130), precision="mixed")
+        for k in range(len(mols)):
+            assert E[k] == eager[k].energy and N[k] == eager[k].n_el, (rep, k)
+            assert torch.equal(Vs[k], eager[k].potential), (rep, k)
+        print(f"replay {rep}: {len(mols)} molecules, sum E = {sum(E):.10f}")
+    with pytest.raises(RuntimeError):
+        c.graph_launch()
+    c.molset_create(mols[:2], **GRID)
+    with pytest.raises(RuntimeError):
+        c.molset_graph_launch()
