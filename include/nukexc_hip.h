@@ -235,6 +235,13 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, 
    x, y, z) are written on return. */
 int nkx_xc_grad(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                 int partition, double *E_xc, double *grad_host);
+/* The same gradient in the mixed form (no counterpart in the reference): the
+   density and the basis-centre contractions X = P Phi, W = P Y run on int8
+   slices with exact integer MFMA accumulation (as NKX_PREC_MIXED builds V);
+   the functional, the STO derivatives and every sum stay FP64.  Arguments
+   and results as nkx_xc_grad; E_xc is the mixed-form energy. */
+int nkx_xc_grad_mixed(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
+                      int partition, double *E_xc, double *grad_host);
 /* Molecule set (no counterpart in the reference, whose driver handles one
    molecule): nmol molecules of different sizes, each with its own grid built
    as nkx_make_flat_grid does (shared grid parameters) and its own basis.

@@ -71,6 +71,7 @@ SIGNATURES = {
                                           _vp, _vp],
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
+    "nkx_xc_grad_mixed": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_autotune": [_vp, _vp, _c_int, _pi, _c_int, _vp, _pi, _pi, _pd],
     "nkx_set_launch": [_vp, _c_int, _c_int],
     "nkx_set_memory_budget": [_vp, _c_i64],

@@ -284,17 +284,20 @@ class Context:
 
     PARTITION = {"fixed": 0, "becke_capped": 1, "becke": 2}
 
-    def xc_grad(self, P, funcs=PBE, partition="becke_capped"):
+    def xc_grad(self, P, funcs=PBE, partition="becke_capped", precision="fp64"):
         """Nuclear gradient of E_xc at fixed P on the ctx grid (points move
-        with their owner atom, weights from the named partition).
+        with their owner atom, weights from the named partition), in FP64 or
+        in the mixed form (int8-slice contractions, precision="mixed").
         Returns (E_xc, grad ndarray (natoms, 3))."""
+        fn = {"fp64": "nkx_xc_grad", "mixed": "nkx_xc_grad_mixed"}[precision]
         P = _dev_f64(P, self.device).contiguous()
         natoms = self.grid.atom_centers.shape[0]
         g = np.zeros(3 * natoms)
         ids, nf = _ids(funcs)
         E = ctypes.c_double()
-        self._chk(lib().nkx_xc_grad(self.h, _ptr(P), nf, ids, self.PARTITION[partition],
-                                    ctypes.byref(E), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self._chk(getattr(lib(), fn)(self.h, _ptr(P), nf, ids, self.PARTITION[partition],
+                                     ctypes.byref(E),
+                                     g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return E.value, g.reshape(natoms, 3)
 
     def autotune(self, P, funcs=PBE, precision="fp64"):

@@ -694,8 +694,8 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfun
   });
 }
 
-int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
-                double *E, double *grad) {
+static int xc_grad_entry(nkx_ctx *ctx, const double *P, int nfunc, const int *ids,
+                         int partition, bool mixed, double *E, double *grad) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
     nkx::XcList xl;
@@ -704,9 +704,19 @@ int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int pa
                 NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
     NKX_REQUIRE(P && grad, NKX_ERR_INVALID_ARGUMENT, "xc_grad: bad arguments");
     double e = 0.0;
-    nkx::xc_grad(ctx->c, P, xl, gga, partition, &e, grad);
+    nkx::xc_grad(ctx->c, P, xl, gga, partition, mixed, &e, grad);
     if (E) *E = e;
   });
+}
+
+int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
+                double *E, double *grad) {
+  return xc_grad_entry(ctx, P, nfunc, ids, partition, false, E, grad);
+}
+
+int nkx_xc_grad_mixed(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
+                      double *E, double *grad) {
+  return xc_grad_entry(ctx, P, nfunc, ids, partition, true, E, grad);
 }
 
 int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,

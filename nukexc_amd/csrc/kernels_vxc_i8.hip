@@ -1261,7 +1261,7 @@ __global__ void __launch_bounds__(512) density_x4_i8(
 __global__ void __launch_bounds__(256) density_f_i8(
     int64_t s0, int64_t b0, const int *__restrict__ b_off, int gga,
     const double *__restrict__ part, const double *__restrict__ pw, XcList xl,
-    double *__restrict__ coef, double *__restrict__ blockEN) {
+    double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
   const int64_t b = b0 + blockIdx.x;
   __shared__ double cs[4][PT];
   const int c = threadIdx.x / PT, g = threadIdx.x % PT;
@@ -1284,6 +1284,7 @@ __global__ void __launch_bounds__(256) density_f_i8(
     coef[4 * gp + 2] = bs * gyr;
     coef[4 * gp + 3] = bs * gzr;
     double e = w * o.e, n = w * rho;
+    if (epoint) epoint[gp] = e;  // w e_xc of this point
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       e += __shfl_down(e, o2);
       n += __shfl_down(n, o2);
@@ -1678,12 +1679,21 @@ void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_
     shape = 65;
   if (!(shape & 224)) return;
   hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
-                     pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN);
+                     pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN, nullptr);
   NKX_HIP_CHECK(hipGetLastError());
   const int64_t s1 = pl.b_off_h[b1];
   if (s1 > s0)
     hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
                        pl.slot_b.p, gga, coef, store);
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+void vkm_density_functional(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                            int gga, const XcList &xl, double *coef, double *blockEN,
+                            const double *part, double *epoint) {
+  if (b1 <= b0) return;
+  hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
+                     pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN, epoint);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -134,7 +134,7 @@ __device__ __forceinline__ double z_value(double ph, double gx, double gy, doubl
 // Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65) with
 // per-row exponents ex[row] into a section at dst (fragment order); one
 // thread per (row fragment, lane) item, 256 threads.
-__device__ void store_section(const double (*v)[65], const int *ex, signed char *dst,
+__device__ inline void store_section(const double (*v)[65], const int *ex, signed char *dst,
                               bool transpose) {
   const int t = threadIdx.x;
   const int rt = t >> 6, l = t & 63;

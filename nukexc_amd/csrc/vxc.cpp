@@ -835,8 +835,8 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
 
 // Nuclear gradient of E_xc at fixed P (kernels_grad.hip).  Every basis
 // function must sit on an atom of the grid; grad is host memory, 3 natoms.
-void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, double *E,
-             double *grad) {
+void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, bool mixed,
+             double *E, double *grad) {
   ensure_plan(c);
   Plan &pl = c.plan;
   Workspace &ws = c.ws;
@@ -859,9 +859,12 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
     NKX_REQUIRE(fatom[sp] >= 0, NKX_ERR_INVALID_ARGUMENT,
                 "basis function " + std::to_string(i) + " is not centred on an atom of the grid");
   }
+  NKX_REQUIRE(pl.nmol == 1, NKX_ERR_STATE, "the gradient runs on the ctx grid and basis");
   const int narr = 5;  // Phi, grad Phi (3), Z
-  const size_t slot_bytes = (size_t)narr * FB * PT * sizeof(double);
-  // tiles plus the per-slot gradient partials
+  // FP64: the tile arrays; mixed: the int8 slot (sections and FP32 grad Phi)
+  // and the density partials; both plus the per-slot gradient partials
+  const size_t slot_bytes = mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0) + 4 * PT * sizeof(double)
+                                  : (size_t)narr * FB * PT * sizeof(double);
   int64_t max_slots =
       chunk_slot_budget(slot_bytes + 3 * FB * sizeof(double), c.opt.mem_budget);
   for (int64_t b = 0; b < pl.nb; ++b)
@@ -869,8 +872,14 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   const int nseg = 32;
   const int nblk = 1024;
-  ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
-  ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+  if (mixed) {
+    ws.mstore.reserve((size_t)slots_alloc * mixed_slot_bytes(gga ? 1 : 0));
+    ws.dpart.reserve((size_t)slots_alloc * 4 * PT);
+    ws.Ps.reserve((size_t)mixed_p_bytes(pl.mol_nfb_h[0]));
+  } else {
+    ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
+    ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+  }
   ws.coef.reserve((size_t)4 * pl.npts_pad);
   ws.blockEN.reserve((size_t)3 * pl.nb);
   ws.red.reserve(3);
@@ -883,14 +892,29 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
          *gfun = gacc + n_ga, *wpart = gfun + n_gf, *wsum = wpart + n_wp;
   ws.gint.reserve(pl.npts_pad);
 
-  vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
+  if (mixed)
+    vkm_gather_p(s, pl.nbf, pl.mol_nfb_h[0], pl.fperm_d.p, P, ws.Ps.p);
+  else
+    vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
   vk_fill0(s, gacc, (int64_t)n_ga);
   const int ig = gga ? 1 : 0;
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
-    vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
-    vk_density(s, 1, 4, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
-               ws.blockEN.p, epoint);
-    vkg_basis(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, ws.coef.p, cpart, tpt);
+    if (mixed) {
+      // density by the split form (contraction, then the functional with w e
+      // per point), Y slices over the Z sections, then the basis-centre term
+      vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
+      vkm_density(s, 65, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p, ws.blockEN.p,
+                  ws.dpart.p);
+      vkm_density_functional(s, b0, b1, s0, pl, ig, xl, ws.coef.p, ws.blockEN.p, ws.dpart.p,
+                             epoint);
+      vkgm_y_slots(s, s0, s1, pl, ig, ws.coef.p, ws.mstore.p);
+      vkgm_basis(s, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, ws.coef.p, cpart, tpt);
+    } else {
+      vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
+      vk_density(s, 1, 4, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+                 ws.blockEN.p, epoint);
+      vkg_basis(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, ws.coef.p, cpart, tpt);
+    }
     vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
   });
   vkg_colsum(s, nseg, pl.nbf_pad * 3, gacc, gfun);

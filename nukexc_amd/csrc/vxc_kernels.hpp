@@ -43,6 +43,11 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
 void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0,
                         const Plan &pl, int gga, signed char *store, const XcList &xl,
                         double *coef, double *blockEN, const double *part);
+// per-batch reduction of part and the functional only (no Z slices);
+// epoint (may be NULL): w e_xc per sorted point
+void vkm_density_functional(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                            int gga, const XcList &xl, double *coef, double *blockEN,
+                            const double *part, double *epoint);
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
                int gga, const signed char *store, double *vpart);
 
@@ -65,6 +70,15 @@ void vkb_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, con
 void vkg_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                const double *Pd, int narr, int gga, const double *tiles, const double *coef,
                double *cpart, double *tpt);
+// mixed form (kernels_grad_i8.hip), on the int8 slot store after vkm_colloc,
+// vkm_density (split shape) and vkm_density_functional: Y = 2 a Phi + b . grad
+// Phi slices per slot (rows = points), then X = P Phi and W = P Y on integer
+// MFMA with the same per-function and per-point sums as vkg_basis
+void vkgm_y_slots(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
+                  const double *coef, signed char *store);
+void vkgm_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                const signed char *Ps, int gga, const signed char *store, const double *coef,
+                double *cpart, double *tpt);
 // fold the per-slot partials of batches [b0, b1) into gacc[seg][nbf_pad * 3]
 void vkg_fold(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nseg,
               const double *cpart, double *gacc);

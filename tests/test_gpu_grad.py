@@ -1,7 +1,7 @@
-"""Nuclear gradient of E_xc at fixed P (nkx_xc_grad) against finite
-differences of the FP64 oracle energy, the grid rebuilt at each displaced
-geometry (points move with their atoms, capped Becke weights recomputed) and
-the basis centres moved with their atoms.  Fourth-order central differences,
+"""Nuclear gradient of E_xc at fixed P (nkx_xc_grad, and the mixed form
+nkx_xc_grad_mixed) against finite differences of the FP64 oracle energy, the
+grid rebuilt at each displaced geometry (points move with their atoms, capped
+Becke weights recomputed) and the basis centres moved with their atoms.  Fourth-order central differences,
 step 5e-4 bohr; bar |dE/dR - FD| <= 1e-6 Eh/bohr."""
 import dataclasses
 
@@ -73,6 +73,13 @@ def test_xc_grad_water_vs_oracle_fd(ctx, funcs):
     print(f"{funcs}: max|g - fd| = {np.abs(g - fd).max():.2e}  sum_A g = {g.sum(0)}")
     assert np.abs(g - fd).max() <= 1e-6
     assert np.abs(g.sum(0)).max() <= 1e-10
+    # mixed form (int8-slice contractions) against the same differences
+    Em, gm = ctx.xc_grad(torch.tensor(P, device="cuda"), funcs=funcs, precision="mixed")
+    print(f"{funcs} mixed: |E - E0| = {abs(Em - E0):.1e}  max|g - fd| = "
+          f"{np.abs(gm - fd).max():.2e}  max|g - g_fp64| = {np.abs(gm - g).max():.2e}")
+    assert abs(Em - E0) <= 1e-6
+    assert np.abs(gm - fd).max() <= 1e-6
+    assert np.abs(gm.sum(0)).max() <= 1e-10
 
 
 def test_xc_grad_capped_vs_uncapped_small_molecule(ctx):
@@ -92,6 +99,10 @@ def test_xc_grad_benzene_capped_fd(ctx):
     w, P, fa = _setup(ctx, "benzene", nrad, order)
     funcs = (101, 130)
     E, g = ctx.xc_grad(torch.tensor(P, device="cuda"), funcs=funcs)
+    Em, gm = ctx.xc_grad(torch.tensor(P, device="cuda"), funcs=funcs, precision="mixed")
+    print(f"benzene mixed: |E - E_fp64| = {abs(Em - E):.1e}  max|g - g_fp64| = "
+          f"{np.abs(gm - g).max():.2e}")
+    assert abs(Em - E) <= 1e-6
     h = 5e-4
     for a, k in [(0, 0), (0, 2), (6, 0), (6, 1), (9, 2)]:
         e = []
@@ -102,7 +113,9 @@ def test_xc_grad_benzene_capped_fd(ctx):
         fd = (-e[0] + 8 * e[1] - 8 * e[2] + e[3]) / (12 * h)
         print(f"atom {a} comp {k}: g={g[a, k]:.10f} fd={fd:.10f} diff={g[a, k] - fd:.1e}")
         assert abs(g[a, k] - fd) <= 1e-6
+        assert abs(gm[a, k] - fd) <= 1e-6
     assert np.abs(g.sum(0)).max() <= 1e-10
+    assert np.abs(gm.sum(0)).max() <= 1e-10
 
 
 def test_xc_grad_rejects_off_atom_functions(ctx):
@@ -135,8 +148,16 @@ def test_xc_grad_mol30_vs_fd():
         t0 = time.perf_counter()
         E, g = c.xc_grad(Pt, funcs=funcs)
         t1 = time.perf_counter()
+        c.xc_grad(Pt, funcs=funcs, precision="mixed")
+        t2 = time.perf_counter()
+        Em, gm = c.xc_grad(Pt, funcs=funcs, precision="mixed")
+        t3 = time.perf_counter()
         print(f"mol30: {len(w['Z'])} atoms, {grid.weights.shape[0]} points, "
-              f"{w['basis'].nbf()} functions; gradient {1e3 * (t1 - t0):.1f} ms")
+              f"{w['basis'].nbf()} functions; gradient FP64 {1e3 * (t1 - t0):.1f} ms, "
+              f"mixed {1e3 * (t3 - t2):.1f} ms; mixed - FP64: |dE| {abs(Em - E):.1e}, "
+              f"max|dg| {np.abs(gm - g).max():.1e}")
+        assert abs(Em - E) <= 1e-6
+        assert np.abs(gm.sum(0)).max() <= 1e-9
         h = 1e-4
         for a, k in [(0, 0), (7, 1), (13, 2), (21, 0), (25, 1), (29, 2)]:
             e = []
@@ -147,8 +168,10 @@ def test_xc_grad_mol30_vs_fd():
                 c.set_basis(_moved(w["basis"], fa, xyz))
                 e.append(c.vxc(Pt, funcs=funcs, precision="fp64").energy)
             fd = (e[0] - e[1]) / (2 * h)
-            print(f"atom {a} comp {k}: g={g[a, k]:.10f} fd={fd:.10f} diff={g[a, k] - fd:.1e}")
+            print(f"atom {a} comp {k}: g={g[a, k]:.10f} fd={fd:.10f} diff={g[a, k] - fd:.1e} "
+                  f"mixed diff={gm[a, k] - fd:.1e}")
             assert abs(g[a, k] - fd) <= 1e-6
+            assert abs(gm[a, k] - fd) <= 1e-6
         assert np.abs(g.sum(0)).max() <= 1e-9
     finally:
         c.close()
