@@ -45,7 +45,7 @@ def kernel_name(precision, stage, shape):
             return "density_x4_i8"
         if shape == 67:
             return "density_x22_i8"
-        if shape in (65, 66):
+        if shape in (65, 66, 71, 72):
             return "density_x2_i8"
         if shape & 32:
             return "density_x_i8"

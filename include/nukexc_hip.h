@@ -284,9 +284,11 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
    1 or 2 batches, plus 16 for two function blocks per step, plus 32 for the
    split form, where contraction, functional and Z are three kernels and
    the workgroups of one batch group share an XCD; 65 and 66 for the split
-   form with 32 rows per wave; 67 for two batches x two blocks per
-   workgroup with both operands in LDS; 129 for the split form over quads of
-   function blocks, one batch and eight waves per workgroup), then the split
+   form with 32 rows per wave; 71 and 72 for 65 on absolute block pairs dealt
+   to XCDs in tiles of 4 x 16 / 8 x 8 (pairs x batches); 67 for two batches
+   x two blocks per workgroup with both operands in LDS; 129 for the split
+   form over quads of function blocks, one batch and eight waves per
+   workgroup; the autotuner times 65, 71, 67 and 129), then the split
    count of the V contraction over batches (the automatic choice x 1/4 .. 4)
    -- keeps the fastest for later builds and captures, and returns them
    (either pointer may be NULL) with the best build time.  V receives the
@@ -297,7 +299,7 @@ int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_i
 /* Launch parameters (no counterpart in the reference): set or read the
    values the autotuner picks, e.g. to reuse a tuned pair without timing
    again.  vgemm_split 0 = automatic; density_shape 0 = default, else one of
-   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66, 67, 129 (int8
+   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66, 67, 71, 72, 129 (int8
    form); the FP64 form reads the int8 codes modulo 16, the int8 form reads
    4 as 65. */
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);

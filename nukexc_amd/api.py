@@ -316,7 +316,7 @@ class Context:
     def set_launch(self, vgemm_split=0, density_shape=0):
         """Set the launch parameters the autotuner picks (0 = automatic /
         default); density shapes 1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50,
-        65, 66, 67, 129 (int8 form)."""
+        65, 66, 67, 71, 72, 129 (int8 form)."""
         self._chk(lib().nkx_set_launch(self.h, int(vgemm_split), int(density_shape)))
 
     def set_memory_budget(self, nbytes=0):

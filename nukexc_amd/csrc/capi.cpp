@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "nkx_internal.hpp"
+#include "vxc_kernels.hpp"
 
 struct nkx_ctx {
   nkx::Ctx c;
@@ -784,11 +785,9 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
   return guarded(ctx, [&] {
     NKX_REQUIRE(vgemm_split >= 0, NKX_ERR_INVALID_ARGUMENT, "vgemm_split must be >= 0");
     const int sh = density_shape;
-    NKX_REQUIRE(sh == 0 || sh == 1 || sh == 2 || sh == 4 || sh == 17 || sh == 18 || sh == 33 ||
-                    sh == 34 || sh == 49 || sh == 50 || sh == 65 || sh == 66 || sh == 67 ||
-                    sh == 129,
-                NKX_ERR_INVALID_ARGUMENT,
-                "density shape must be 0, 1, 2, 4, 17, 18, 33, 34, 49, 50, 65, 66, 67 or 129");
+    NKX_REQUIRE(sh == 0 || sh == 4 || nkx::mixed_density_shape_ok(sh), NKX_ERR_INVALID_ARGUMENT,
+                "density shape must be 0, 1, 2, 4, 17, 18, 33, 34, 49, 50, 65, 66, 67, 71, 72 "
+                "or 129");
     ctx->c.opt.vgemm_split = vgemm_split;
     ctx->c.opt.density_group = density_shape;
   });

@@ -642,15 +642,33 @@ __global__ void __launch_bounds__(256 * MB) density_x_i8(
 // block m_(w / 2), so each B fragment read from LDS feeds two tile products.
 // The active k of the union list and their masks are listed in LDS once;
 // XCD placement, staging and the per-slot output are those of density_x_i8.
-template <int GBT>
+// ABS (one batch per workgroup): the pair is an absolute pair of the
+// molecule's blocks (2 p, 2 p + 1; a workgroup whose two blocks are both dead
+// in its batch exits), and the workgroups are dealt in tiles of tpn pairs x
+// tbn consecutive batches, one whole tile to an XCD at a time, so that the
+// tile's P sections and Pm sections are read from that XCD's L2 by all its
+// batches and pairs rather than once per workgroup from the memory side.
+template <int GBT, bool ABS>
 __global__ void __launch_bounds__(256) density_x2_i8(
     MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
-    const signed char *__restrict__ store, double *__restrict__ part) {
+    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int tpn,
+    int tbn, int gga, const signed char *__restrict__ store, double *__restrict__ part) {
+  static_assert(!ABS || GBT == 1, "absolute pairs take one batch per workgroup");
   const int xcd = blockIdx.x & 7;
   const int rr = blockIdx.x >> 3;
-  const int mgi = rr % nmg;
-  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
+  int mgi = 0;
+  int64_t bg;
+  if (ABS) {  // nmg = pairs of the largest molecule; tile = (pair group, batch group)
+    const int tsz = tpn * tbn;
+    const int tile = (rr / tsz) * 8 + xcd, li = rr % tsz;
+    const int ntp = (nmg + tpn - 1) / tpn;
+    mgi = (tile % ntp) * tpn + li % tpn;
+    bg = b0 + (int64_t)(tile / ntp) * tbn + li / tpn;
+    if (mgi >= nmg) return;
+  } else {
+    mgi = rr % nmg;
+    bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
+  }
   if (bg >= b1) return;
   const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -688,8 +706,28 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   }
   __syncthreads();
   const int nu = nu_s;
-  if (mgi * 2 >= nu) return;  // uniform: no block of this pair
-  const int fm0 = ublk[mgi * 2], fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
+  int fm0, fm1;
+  if (ABS) {  // blocks 2 mgi, 2 mgi + 1 of the batch's molecule, if live
+    const int lo = b_off[bg];
+    if (lo == b_off[bg + 1]) return;  // uniform: empty batch
+    const int f0 = b_fbl[lo];
+    const int blk0 = f0 - mt.blk_local[f0], nfm = mt.nfb[mt.blk_mol[f0]];
+    int a = 2 * mgi < nfm ? blk0 + 2 * mgi : -1;
+    int c = 2 * mgi + 1 < nfm ? blk0 + 2 * mgi + 1 : -1;
+    if (a >= 0 && tab[0][a] < 0) a = -1;
+    if (c >= 0 && tab[0][c] < 0) c = -1;
+    if (a < 0) {
+      a = c;
+      c = -1;
+    }
+    if (a < 0) return;  // uniform: both blocks dead here
+    fm0 = a;
+    fm1 = c;
+  } else {
+    if (mgi * 2 >= nu) return;  // uniform: no block of this pair
+    fm0 = ublk[mgi * 2];
+    fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
+  }
   if (wv == 0) {
     int cnt = 0;
     for (int base = 0; base < nu; base += 64) {
@@ -1616,25 +1654,49 @@ static void launch_density_x2_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t 
   const int nmg = (umax + 1) / 2;
   const int64_t nwg = ngq * 8 * nmg;
   NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
-  hipLaunchKernelGGL((density_x2_i8<GBT>), dim3((unsigned)nwg), dim3(256), 0, s, pl.tab(),
-                     pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store, part);
+  hipLaunchKernelGGL((density_x2_i8<GBT, false>), dim3((unsigned)nwg), dim3(256), 0, s,
+                     pl.tab(), pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, 0, 0, gga,
+                     store, part);
+}
+
+// absolute block pairs in XCD tiles of tpn pairs x tbn batches
+static void launch_density_x2a_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
+                                  const Plan &pl, const signed char *Ps, int gga,
+                                  const signed char *store, double *part, int tpn, int tbn) {
+  const int npair = (pl.max_mol_nfb + 1) / 2;
+  const int64_t ntiles = (int64_t)((npair + tpn - 1) / tpn) * ((b1 - b0 + tbn - 1) / tbn);
+  const int64_t nwg = (ntiles + 7) / 8 * 8 * tpn * tbn;
+  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
+  hipLaunchKernelGGL((density_x2_i8<1, true>), dim3((unsigned)nwg), dim3(256), 0, s, pl.tab(),
+                     pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, npair, tpn, tbn, gga,
+                     store, part);
 }
 
 // shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1),
 // + 32 for the split form (contraction, functional, Z as three kernels),
 // 65 / 66: the split form with two row fragments per wave (1 / 2 batches),
+// 71 / 72: 65 on absolute block pairs in XCD tiles of 4 x 16 / 8 x 8
+// (pairs x batches),
 // 67: the split form for two batches x two blocks with both operands in LDS,
 // 129: the split form over block quads (one batch, eight waves);
 // other values take 65
+bool mixed_density_shape_ok(int shape) {
+  switch (shape) {
+    case 1: case 2: case 17: case 18: case 33: case 34: case 49: case 50:
+    case 65: case 66: case 67: case 71: case 72: case 129:
+      return true;
+    default:
+      return false;
+  }
+}
+
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                  const signed char *Ps, int gga, signed char *store, const XcList &xl,
                  double *coef, double *blockEN, double *part) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
-  if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
-      shape != 49 && shape != 50 && shape != 65 && shape != 66 && shape != 67 && shape != 129)
-    shape = 65;
+  if (!mixed_density_shape_ok(shape)) shape = 65;
   if (shape & 224) {
     NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "split density form without a partial buffer");
     if (shape == 67) {
@@ -1642,10 +1704,12 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
     } else if (shape == 129) {
       launch_density_x4_i8(s, b0, b1, s0, pl, Ps, gga, store, part);
     } else if (shape & 64) {
-      if (shape == 65)
-        launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part);
-      else
-        launch_density_x2_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, part);
+      switch (shape) {
+        case 66: launch_density_x2_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+        case 71: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 4, 16); break;
+        case 72: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 8, 8); break;
+        default: launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+      }
     } else {
     switch (shape & 31) {
       case 1: launch_density_x_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
@@ -1674,9 +1738,7 @@ void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_
                         const Plan &pl, int gga, signed char *store, const XcList &xl,
                         double *coef, double *blockEN, const double *part) {
   if (b1 <= b0) return;
-  if (shape != 1 && shape != 2 && shape != 17 && shape != 18 && shape != 33 && shape != 34 &&
-      shape != 49 && shape != 50 && shape != 65 && shape != 66 && shape != 67 && shape != 129)
-    shape = 65;
+  if (!mixed_density_shape_ok(shape)) shape = 65;
   if (!(shape & 224)) return;
   hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
                      pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN, nullptr);

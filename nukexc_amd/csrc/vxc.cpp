@@ -693,11 +693,12 @@ double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precisi
     vxc_build(c, P, xl, gga, precision, &E, &N, V);  // warm caches and workspace
     int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 65 : 2);
     // density shapes: batches per workgroup (int8 form: the split-form
-    // shapes 65 (32 rows per wave), 67 (two batches x two blocks, both
-    // operands in LDS), 129 (block quads) and 33; the fused shapes and 34,
-    // 49, 50, 66 are never the fastest on large grids and are not timed)
+    // shapes 65 (32 rows per wave), 71 (65 in XCD tiles of absolute block
+    // pairs), 67 (two batches x two blocks, both operands in LDS) and 129
+    // (block quads); the fused shapes and 33, 34, 49, 50, 66, 72 are never
+    // the fastest on large grids and are not timed)
     const std::vector<int> shapes =
-        precision == NKX_PREC_MIXED ? std::vector<int>{65, 67, 129, 33}
+        precision == NKX_PREC_MIXED ? std::vector<int>{65, 71, 67, 129}
                                     : std::vector<int>{1, 2, 4};
     if (!bf16)
       for (int g : shapes) {

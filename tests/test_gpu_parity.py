@@ -365,8 +365,9 @@ def test_vxc_mixed_density_shapes(ctx):
     order and sums); two function blocks per step in the fused form (17, 18)
     and two row fragments per wave in the split form (65, 66; the block-quad
     form 129 groups rows the same way) change only the order of the FP64
-    column sums.  Out-of-range shapes are
-    rejected by nkx_set_launch."""
+    column sums; the XCD-tiled absolute-pair forms 71 and 72 compute exactly as
+    65.
+    Out-of-range shapes are rejected by nkx_set_launch."""
     rng = np.random.Generator(np.random.PCG64(8))
     xyz, Z, _ = configs.water_cluster(8, rng)
     basis = nk.make_manual_basis(configs.sto_functions(Z, xyz, rng))
@@ -376,7 +377,7 @@ def test_vxc_mixed_density_shapes(ctx):
     C, P = configs.loewdin_orbitals(S, 40, rng)
     res = {}
     try:
-        for shape in (2, 1, 17, 18, 33, 34, 49, 50, 65, 66, 67, 129):
+        for shape in (2, 1, 17, 18, 33, 34, 49, 50, 65, 66, 67, 71, 72, 129):
             ctx.set_launch(0, shape)
             assert ctx.get_launch() == (0, shape)
             r = ctx.vxc(dev(P), funcs=(101, 130), precision="mixed")
@@ -394,7 +395,7 @@ def test_vxc_mixed_density_shapes(ctx):
         assert np.array_equal(res[shape][2], V2), shape
     # block quads (129) and two batches x two blocks (67) group rows per wave
     # as 65 does
-    for shape in (67, 129):
+    for shape in (67, 71, 72, 129):
         assert res[shape][0] == res[65][0] and np.array_equal(res[shape][2], res[65][2]), shape
     for shape in (17, 18, 65, 66):
         E, N, V = res[shape]
