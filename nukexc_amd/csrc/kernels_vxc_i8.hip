@@ -223,12 +223,8 @@ __global__ void __launch_bounds__(256 * MB) density_fused_i8(
   // copies), and this wave's P fragment of block pair (m, k) -> registers
   auto issue = [&](int st, unsigned pm, int dbuf) {
     const int mg = st / nu, fk = ublk[st - mg * nu];
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (!colbit(pm, sb)) continue;
-      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
-                  (wv + 5 * sb) % NW, NW, lane);
-    }
+    // the P fragment first: its address needs table loads, and waiting for
+    // them must not wait for the section copies as well
     if ((pm >> (mb * GBT)) & ROW) {
       const int fm = ublk[mg * MB + mb];
       const int mol = mt.blk_mol[fm];
@@ -238,6 +234,12 @@ __global__ void __launch_bounds__(256 * MB) density_fused_i8(
 #pragma unroll
       for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
       eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
+    }
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!colbit(pm, sb)) continue;
+      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                  (wv + 5 * sb) % NW, NW, lane);
     }
   };
 
@@ -508,12 +510,7 @@ __global__ void __launch_bounds__(256 * MB) density_x_i8(
   int eAn = 0;
   auto issue = [&](int ki, unsigned pm, int dbuf) {
     const int fk = ublk[ki];
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (!colbit(pm, sb)) continue;
-      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
-                  (wv + 5 * sb) % NW, NW, lane);
-    }
+    // the P fragment first (see density_fused_i8)
     if ((pm >> (mb * GBT)) & ROW) {
       const int fm = ublk[mgi * MB + mb];
       const int mol = mt.blk_mol[fm];
@@ -523,6 +520,12 @@ __global__ void __launch_bounds__(256 * MB) density_x_i8(
 #pragma unroll
       for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
       eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
+    }
+#pragma unroll
+    for (int sb = 0; sb < GBT; ++sb) {
+      if (!colbit(pm, sb)) continue;
+      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                  (wv + 5 * sb) % NW, NW, lane);
     }
   };
 
@@ -757,11 +760,15 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   const int nq = gga ? 4 : 1;
   const int fm = mb ? fm1 : fm0;
   constexpr unsigned ROW = (1u << GBT) - 1u;
-  // this wave's P row: sections (fm, k) of its molecule
+  // this wave's P row: sections (fm, k) of its molecule, whose blocks are
+  // numbered consecutively from mbase (so no table load per step: such a load
+  // would make the wave wait for the section copies issued before it)
   const signed char *prow = nullptr;
+  int mbase = 0;
   if (fm >= 0) {
     const int mol = mt.blk_mol[fm];
-    prow = Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
+    mbase = fm - mt.blk_local[fm];
+    prow = Ps + mt.Ps_off[mol] + (int64_t)(fm - mbase) * mt.nfb[mol] * SEC;
   }
 
   // P fragments and row exponents of a step (two register sets, used in turn)
@@ -779,7 +786,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
                   (wv + sb) & 3, 4, lane);
     }
     if ((pm >> (mb * GBT)) & ROW) {
-      const signed char *pa = prow + (int64_t)mt.blk_local[fk] * SEC;
+      const signed char *pa = prow + (int64_t)(fk - mbase) * SEC;
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
 #pragma unroll
@@ -991,12 +998,14 @@ __global__ void __launch_bounds__(512) density_x22_i8(
   const int nq = gga ? 4 : 1;
   const int fm = mb ? fm1 : fm0;
   const signed char *prow[2] = {nullptr, nullptr};
+  int mbase[2] = {0, 0};  // first block of each row block's molecule
 #pragma unroll
   for (int m2 = 0; m2 < 2; ++m2) {
     const int f = m2 ? fm1 : fm0;
     if (f >= 0) {
       const int mol = mt.blk_mol[f];
-      prow[m2] = Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[f] * mt.nfb[mol] * SEC;
+      mbase[m2] = f - mt.blk_local[f];
+      prow[m2] = Ps + mt.Ps_off[mol] + (int64_t)(f - mbase[m2]) * mt.nfb[mol] * SEC;
     }
   }
   // P sections of the live blocks and Pm sections of the live batches of step
@@ -1007,7 +1016,7 @@ __global__ void __launch_bounds__(512) density_x22_i8(
 #pragma unroll
     for (int m2 = 0; m2 < 2; ++m2)
       if ((pm >> (2 * m2)) & 3u)
-        dma_section(prow[m2] + (int64_t)mt.blk_local[fk] * SEC, &buf4[dbuf][m2][0],
+        dma_section(prow[m2] + (int64_t)(fk - mbase[m2]) * SEC, &buf4[dbuf][m2][0],
                     (wv + 3 * m2) & 7, 8, lane);
 #pragma unroll
     for (int s2 = 0; s2 < 2; ++s2)
@@ -1434,7 +1443,10 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   const int l16 = lane & 15, q = lane >> 4;
   const bool diag = blk.x == blk.z;
   const int64_t sb_bytes = slot_bytes_i8(gga);
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][4][SEC];  // Z_J0 F_J0 Z_J1 F_J1
+  // J sections, two buffers (Z_J0 F_J0 Z_J1 F_J1 each), declared apart so
+  // that a copy into one is known not to touch the other
+  __shared__ __attribute__((aligned(16))) signed char bufA[4][SEC];
+  __shared__ __attribute__((aligned(16))) signed char bufB[4][SEC];
   __shared__ int ls[4][512];  // slots of I_0, I_1, J_0, J_1 per listed batch
   __shared__ int wcnt[8];
   double v64[2][4][4];
@@ -1460,15 +1472,15 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     v4i F[4], Z[4];
     int eF, eZ;
   };
-  // J sections -> LDS buffer dbuf (direct copies); the I fragments -> an
-  auto issue = [&](int i, int dbuf, AFrag &an) {
+  // J sections -> LDS buffer dst (direct copies); the I fragments -> an
+  auto issue = [&](int i, signed char (*dst)[SEC], AFrag &an) {
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb) {
       const int sJ = ls[2 + jb][i];
       if (sJ < 0) continue;
       const signed char *tJ = store + (sJ - s0) * sb_bytes;
-      dma_section(tJ + OFF_Z, &bbuf[dbuf][2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
-      dma_section(tJ + OFF_F, &bbuf[dbuf][2 * jb + 1][0], (wv + 4 * jb + 2) & 7, 8, lane);
+      dma_section(tJ + OFF_Z, &dst[2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
+      dma_section(tJ + OFF_F, &dst[2 * jb + 1][0], (wv + 4 * jb + 2) & 7, 8, lane);
     }
     const int sI = ls[half][i];
     if (sI >= 0) {
@@ -1484,12 +1496,13 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   };
   // one batch of the list: wait for its operands, start the next batch's
   // copies into the other buffer and register set, then the tile products
-  auto step = [&](int i, int n, int buf, const AFrag &ac, AFrag &an) {
+  auto step = [&](int i, int n, signed char (*cur)[SEC], signed char (*nxt)[SEC],
+                  const AFrag &ac, AFrag &an) {
     wait_vm0();
     __syncthreads();  // buffer buf complete, buf ^ 1 free
     const bool mine = ls[half][i] >= 0;
     const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
-    if (i + 1 < n) issue(i + 1, buf ^ 1, an);
+    if (i + 1 < n) issue(i + 1, nxt, an);
     if (!mine) return;
     // rows and columns stored as zero have zero slices: no zero test on the
     // scales
@@ -1502,8 +1515,8 @@ __global__ void __launch_bounds__(512) vgemm_i8(
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb) {
       if (!jl[jb]) continue;
-      const signed char *bz = &bbuf[buf][2 * jb][0];
-      const signed char *bf = &bbuf[buf][2 * jb + 1][0];
+      const signed char *bz = &cur[2 * jb][0];
+      const signed char *bf = &cur[2 * jb + 1][0];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         v4i BZ[4], BF[4];
@@ -1554,10 +1567,10 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     }
     __syncthreads();
     if (n == 0) continue;
-    issue(0, 0, a0);
+    issue(0, bufA, a0);
     for (int i = 0; i < n; i += 2) {
-      step(i, n, 0, a0, a1);
-      if (i + 1 < n) step(i + 1, n, 1, a1, a0);
+      step(i, n, bufA, bufB, a0, a1);
+      if (i + 1 < n) step(i + 1, n, bufB, bufA, a1, a0);
     }
   }
   const int4 pix = sp_pidx[sp];
