@@ -296,6 +296,14 @@ int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
 int nkx_autotune(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                  int precision, double *V_dev, int *vgemm_split, int *density_group,
                  double *best_ms);
+/* The same search for the molecule set of the ctx (no counterpart in the
+   reference): candidates timed on nkx_molset_vxc builds of every molecule
+   (P_dev[k], V_dev[k] as there; FP64 or NKX_PREC_MIXED).  The pair found is
+   kept in the ctx options, so later set builds and set graph captures use
+   it. */
+int nkx_molset_autotune(nkx_ctx *ctx, const double *const *P_dev, int nfunc,
+                        const int *func_ids, int precision, double *const *V_dev,
+                        int *vgemm_split, int *density_group, double *best_ms);
 /* Launch parameters (no counterpart in the reference): set or read the
    values the autotuner picks, e.g. to reuse a tuned pair without timing
    again.  vgemm_split 0 = automatic; density_shape 0 = default, else one of

@@ -73,6 +73,8 @@ SIGNATURES = {
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_xc_grad_mixed": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_autotune": [_vp, _vp, _c_int, _pi, _c_int, _vp, _pi, _pi, _pd],
+    "nkx_molset_autotune": [_vp, ctypes.POINTER(_vp), _c_int, _pi, _c_int, ctypes.POINTER(_vp),
+                            _pi, _pi, _pd],
     "nkx_set_launch": [_vp, _c_int, _c_int],
     "nkx_set_memory_budget": [_vp, _c_i64],
     "nkx_get_launch": [_vp, _pi, _pi],

@@ -396,6 +396,24 @@ class Context:
         self._chk(lib().nkx_molset_vxc(self.h, Pp, nf, ids, prec, E, N, Vp))
         return [XC_result(E[k], Vs[k], N[k]) for k in range(nmol)]
 
+    def molset_autotune(self, Ps, funcs=PBE, precision="mixed"):
+        """nkx_molset_autotune: fastest density shape and V split count for
+        the molecule set (kept for later set builds and captures); returns
+        (split, shape, ms per set build)."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        nmol = len(self._molset_nbf)
+        assert len(Ps) == nmol
+        Pd = [_dev_f64(P, self.device).contiguous() for P in Ps]
+        Vs = [torch.empty((k, k), dtype=torch.float64, device=self.device)
+              for k in self._molset_nbf]
+        Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Pd])
+        Vp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Vs])
+        split, shape, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        ids, nf = _ids(funcs)
+        self._chk(lib().nkx_molset_autotune(self.h, Pp, nf, ids, prec, Vp, ctypes.byref(split),
+                                            ctypes.byref(shape), ctypes.byref(ms)))
+        return split.value, shape.value, ms.value
+
     def molset_graph_capture(self, Ps, Vs, funcs=PBE, precision="mixed"):
         """Capture one build of the molecule set for the device tensors Ps
         (inputs) and Vs (outputs); replay with molset_graph_launch()."""

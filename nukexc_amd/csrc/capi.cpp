@@ -780,6 +780,28 @@ int nkx_autotune(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int p
   });
 }
 
+int nkx_molset_autotune(nkx_ctx *ctx, const double *const *P, int nfunc, const int *ids,
+                        int precision, double *const *V, int *vgemm_split, int *density_group,
+                        double *best_ms) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::Ctx &c = ctx->c;
+    NKX_REQUIRE(c.mplan.nmol > 0, NKX_ERR_STATE, "no molecule set: call nkx_molset_create");
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_INVALID_ARGUMENT, "molecule sets take FP64 or NKX_PREC_MIXED");
+    NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "molset_autotune: bad arguments");
+    for (int k = 0; k < c.mplan.nmol; ++k)
+      NKX_REQUIRE(P[k] && V[k], NKX_ERR_INVALID_ARGUMENT, "molset_autotune: null P or V");
+    const double ms =
+        nkx::molset_autotune(c, P, xl, gga, precision, V, vgemm_split, density_group);
+    if (best_ms) *best_ms = ms;
+  });
+}
+
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {

@@ -316,6 +316,8 @@ void molset_graph_capture(Ctx &c, const double *const *P, const XcList &xl, bool
                           int precision, double *const *V);
 double autotune(Ctx &c, const double *P, const XcList &xl, bool gga, int precision, double *V,
                 int *best_split, int *best_group);
+double molset_autotune(Ctx &c, const double *const *P, const XcList &xl, bool gga,
+                       int precision, double *const *V, int *best_split, int *best_group);
 void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
                        const std::vector<std::vector<double>> &mol_w,
                        const std::vector<BasisHost> &bases);

@@ -37,11 +37,13 @@ def setup():
 @pytest.mark.parametrize("precision,shape", [("fp64", 0), ("mixed", 0), ("mixed", 1),
                                              ("mixed", 17), ("mixed", 18), ("mixed", 33),
                                              ("mixed", 50), ("mixed", 65), ("mixed", 66),
-                                             ("mixed", 67), ("mixed", 129)])
+                                             ("mixed", 67), ("mixed", 71), ("mixed", 72),
+                                             ("mixed", 129)])
 def test_molset_matches_single_builds(setup, precision, shape):
     """shape: the density workgroup shape of the set build (the single builds
     use the default); 17, 18 and 50 put blocks of two molecules in one step,
-    33 and 50 are the split form."""
+    33 and 50 are the split form, 71 and 72 take absolute block pairs of each
+    batch's molecule."""
     c, mols, Ps, singles = setup
     npts = c.molset_create(mols, **GRID)
     assert np.all(npts > 0)
@@ -104,3 +106,23 @@ def test_molset_graph_replays_match_eager(setup):
     c.molset_create(mols[:2], **GRID)
     with pytest.raises(RuntimeError):
         c.molset_graph_launch()
+
+
+def test_molset_autotune(setup):
+    """nkx_molset_autotune times the candidates on set builds, keeps a valid
+    pair, and the set build with it matches the single builds."""
+    c, mols, Ps, singles = setup
+    c.molset_create(mols, **GRID)
+    Pd = [torch.tensor(P, device="cuda") for P in Ps]
+    try:
+        split, shape, ms = c.molset_autotune(Pd, funcs=(101, 130), precision="mixed")
+        print(f"molset autotune: split {split}, shape {shape}, {ms:.2f} ms")
+        assert split >= 1 and shape in (65, 67, 71, 129) and ms > 0
+        assert c.get_launch() == (split, shape)
+        res = c.molset_vxc(Pd, funcs=(101, 130), precision="mixed")
+    finally:
+        c.set_launch(0, 0)
+    for r, one in zip(res, singles):
+        ref = one["mixed"]
+        assert abs(r.energy - ref.energy) <= 1e-11 * max(1.0, abs(ref.energy))
+        assert (r.potential - ref.potential).abs().max().item() <= 1e-12
