@@ -118,6 +118,20 @@ def test_xc_grad_benzene_capped_fd(ctx):
     assert np.abs(gm.sum(0)).max() <= 1e-10
 
 
+@pytest.mark.parametrize("partition", ["fixed", "becke", "becke_capped"])
+def test_xc_grad_mixed_matches_fp64_partitions(ctx, partition):
+    """The mixed gradient against the FP64 one for each weight model (the
+    weight-derivative kernel is shared; the basis-centre term and the energy
+    density per point come from the int8 contractions)."""
+    w, P, fa = _setup(ctx, "benzene", 20, 11)
+    Pt = torch.tensor(P, device="cuda")
+    E, g = ctx.xc_grad(Pt, funcs=(101, 130), partition=partition)
+    Em, gm = ctx.xc_grad(Pt, funcs=(101, 130), partition=partition, precision="mixed")
+    print(f"{partition}: |dE| = {abs(Em - E):.1e}, max|dg| = {np.abs(gm - g).max():.1e}")
+    assert abs(Em - E) <= 1e-7
+    assert np.abs(gm - g).max() <= 1e-7
+
+
 def test_xc_grad_rejects_off_atom_functions(ctx):
     w, P, fa = _setup(ctx, "water", 20, 11)
     b = dataclasses.replace(w["basis"], O=w["basis"].O + 0.1)
