@@ -936,14 +936,15 @@ __global__ void __launch_bounds__(512) density_x22_i8(
 
   // [buffer][P (m_0, k), P (m_1, k), Pm (k, b_0), Pm (k, b_1)]; the column-sum
   // exchange of the epilogue reuses the first buffer
-  __shared__ __attribute__((aligned(16))) signed char buf4[2][4][SEC];
+  __shared__ __attribute__((aligned(16))) signed char bufA[4][SEC];
+  __shared__ __attribute__((aligned(16))) signed char bufB[4][SEC];
   __shared__ short tab[2][MAXNFB];  // slot - b_off[batch], -1 if not live
   __shared__ short ublk[MAXNFB];
   __shared__ short sk[MAXNFB];
   __shared__ unsigned char spm[MAXNFB];
   __shared__ int nu_s, nst_s;
-  static_assert(sizeof(buf4[0]) >= sizeof(double) * 8 * 4 * PT, "red alias");
-  double(*red)[4][PT] = reinterpret_cast<double(*)[4][PT]>(&buf4[0][0][0]);
+  static_assert(sizeof(bufA) >= sizeof(double) * 8 * 4 * PT, "red alias");
+  double(*red)[4][PT] = reinterpret_cast<double(*)[4][PT]>(&bufA[0][0]);
 
   for (int e = threadIdx.x; e < 2 * nfb; e += 512) tab[e / nfb][e % nfb] = -1;
   __syncthreads();
@@ -1009,20 +1010,20 @@ __global__ void __launch_bounds__(512) density_x22_i8(
     }
   }
   // P sections of the live blocks and Pm sections of the live batches of step
-  // i -> buffer dbuf; 8 waves, pieces spread over them
-  auto issue = [&](int i, int dbuf) {
+  // i -> buffer dst; 8 waves, pieces spread over them
+  auto issue = [&](int i, signed char (*dst)[SEC]) {
     const int fk = ublk[sk[i]];
     const unsigned pm = spm[i];
 #pragma unroll
     for (int m2 = 0; m2 < 2; ++m2)
       if ((pm >> (2 * m2)) & 3u)
-        dma_section(prow[m2] + (int64_t)(fk - mbase[m2]) * SEC, &buf4[dbuf][m2][0],
+        dma_section(prow[m2] + (int64_t)(fk - mbase[m2]) * SEC, &dst[m2][0],
                     (wv + 3 * m2) & 7, 8, lane);
 #pragma unroll
     for (int s2 = 0; s2 < 2; ++s2)
       if (((pm >> s2) | (pm >> (2 + s2))) & 1u)
         dma_section(store + (boff[s2] + tab[s2][fk] - s0) * sb_bytes + OFF_PM,
-                    &buf4[dbuf][2 + s2][0], (wv + 6 + 3 * s2) & 7, 8, lane);
+                    &dst[2 + s2][0], (wv + 6 + 3 * s2) & 7, 8, lane);
   };
 
   double x64[2][4][4];
@@ -1032,36 +1033,51 @@ __global__ void __launch_bounds__(512) density_x22_i8(
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) x64[f][j][r] = 0.0;
-  if (nst > 0) issue(0, 0);
-  int bufi = 0;
-  for (int i = 0; i < nst; ++i) {
+  // One step: wait for its operands, start the next step's copies into the
+  // other buffer, then the tile products.  The two buffers are separate
+  // arrays, so the compiler sees that the copies and the reads touch
+  // different memory and does not wait for the copies.
+  auto step = [&](int i, signed char (*cur)[SEC], signed char (*nxt)[SEC]) {
     wait_vm0();
-    __syncthreads();  // buffer bufi complete, bufi ^ 1 free
+    __syncthreads();  // buffer cur complete, nxt free
     const unsigned pmc = spm[i];
-    if (i + 1 < nst) issue(i + 1, bufi ^ 1);
-    if ((pmc >> (2 * mb + sb)) & 1u) {
-      const signed char *ab = &buf4[bufi][mb][0];
-      const signed char *bb = &buf4[bufi][2 + sb][0];
-      v4i A[2][4];
-      double sA[2][4];
+    const bool act = (pmc >> (2 * mb + sb)) & 1u;
+    // every operand of this step from LDS into registers before the next
+    // step's copies are issued: reads after a copy into LDS make the
+    // compiler wait for that copy
+    v4i A[2][4], B[4][4];
+    int ea[2] = {0, 0}, eb[4] = {0, 0, 0, 0};
+    if (act) {
+      const signed char *ab = &cur[mb][0];
+      const signed char *bb = &cur[2 + sb][0];
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) A[f][p] = ld16(ab + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
-        const int ea = *reinterpret_cast<const int *>(ab + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ea >> (8 * r)));
+        ea[f] = *reinterpret_cast<const int *>(ab + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        v4i B[4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-        const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
+        for (int p = 0; p < 4; ++p) B[j][p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
+        eb[j] = bb[4 * PLANE + 16 * j + l16];
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the reads are done
+    if (i + 1 < nst) issue(i + 1, nxt);
+    if (act) {
+      double sA[2][4];
+#pragma unroll
+      for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ea[f] >> (8 * r)));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double sB = pow2(eb[j] - 19);
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
           int hi4[4], lo4[4];
-          tile_product(A[f], B, hi4, lo4);
+          tile_product(A[f], B[j], hi4, lo4);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
             x64[f][j][r] =
@@ -1069,7 +1085,11 @@ __global__ void __launch_bounds__(512) density_x22_i8(
         }
       }
     }
-    bufi ^= 1;
+  };
+  if (nst > 0) issue(0, bufA);
+  for (int i = 0; i < nst; i += 2) {
+    step(i, bufA, bufB);
+    if (i + 1 < nst) step(i + 1, bufB, bufA);
   }
   __syncthreads();  // every wave is done with the buffers (red aliases them)
 
