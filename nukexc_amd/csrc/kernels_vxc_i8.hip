@@ -673,6 +673,10 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
   }
   if (bg >= b1) return;
+  // one batch: its union list is its own block list, so a pair index past
+  // it has nothing to do (common in molecule sets, where nmg follows the
+  // largest molecule)
+  if (!ABS && GBT == 1 && 2 * mgi >= b_off[bg + 1] - b_off[bg]) return;
   const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int mb = wv >> 1, rh = wv & 1;
