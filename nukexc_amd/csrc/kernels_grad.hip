@@ -228,7 +228,20 @@ __device__ double becke_cell(int i, int natoms, const double *C, double x, doubl
   return Pi;
 }
 
+// sum over the 64 lanes of a wave (every lane receives it)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // One thread per sorted point; per-atom sums in LDS (3 natoms doubles).
+// Every loop over atoms runs in step across the wave (lanes without a
+// contribution add zero), so each per-atom term is summed over the wave by
+// shuffles and added to LDS once per wave: all lanes of a wave touch the
+// same atoms, and one LDS atomic per lane would serialise on them.  The
+// points of a wave are 64 consecutive sorted points (npts_pad and the grid
+// stride are multiples of 64), so every wave runs the same trip count.
 __global__ void __launch_bounds__(256) grad_weights_kernel(
     int64_t npts_pad, int natoms, const double *__restrict__ C, const double *__restrict__ px,
     const double *__restrict__ py, const double *__restrict__ pz, const int *__restrict__ pown,
@@ -238,18 +251,24 @@ __global__ void __launch_bounds__(256) grad_weights_kernel(
   for (int e = threadIdx.x; e < 3 * natoms; e += blockDim.x) acc[e] = 0.0;
   __syncthreads();
   const bool cap = partition == 1;
+  const int lane = threadIdx.x & 63;
   for (int64_t sp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; sp < npts_pad;
        sp += (int64_t)gridDim.x * blockDim.x) {
     const int B = pown[sp];
-    if (B < 0 || B >= natoms) continue;  // padding (or an owner outside the atom list)
-    double S[3] = {-tpt[3 * sp], -tpt[3 * sp + 1], -tpt[3 * sp + 2]};  // owner gets -S
-    const double wf = epoint[sp];
-    if (partition != 0 && wf != 0.0) {
-      const double x = px[sp], y = py[sp], z = pz[sp];
-      double rmin = 1e300;
+    const bool act = B >= 0 && B < natoms;  // not padding
+    double S[3] = {0.0, 0.0, 0.0};          // the owner receives -S
+    double x = 0.0, y = 0.0, z = 0.0, wf = 0.0, rmin = 1e300, Z = 0.0, PB = 0.0;
+    if (act) {
+      for (int k = 0; k < 3; ++k) S[k] = -tpt[3 * sp + k];
+      wf = epoint[sp];
+      x = px[sp];
+      y = py[sp];
+      z = pz[sp];
+    }
+    const bool wact = act && partition != 0 && wf != 0.0;
+    if (wact) {
       if (cap)
         for (int i = 0; i < natoms; ++i) rmin = fmin(rmin, gdist(x, y, z, C + 3 * i));
-      double Z = 0.0, PB = 0.0;
       for (int i = 0; i < natoms; ++i) {
         const double ri = gdist(x, y, z, C + 3 * i);
         if (cap && ri - rmin >= 5.0) continue;
@@ -257,51 +276,83 @@ __global__ void __launch_bounds__(256) grad_weights_kernel(
         Z += Pi;
         if (i == B) PB = Pi;
       }
-      if (PB > 0.0 && Z > 0.0) {
-        for (int i = 0; i < natoms; ++i) {
-          const double ri = gdist(x, y, z, C + 3 * i);
-          if (cap && ri - rmin >= 5.0) continue;
-          const double Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
-          if (Pi == 0.0) continue;
-          const double coefi = wf * ((i == B ? 1.0 / PB : 0.0) - 1.0 / Z) * Pi;
-          const double *Ci = C + 3 * i;
-          const double ui[3] = {(x - Ci[0]) / ri, (y - Ci[1]) / ri, (z - Ci[2]) / ri};
-          double gi[3] = {0.0, 0.0, 0.0};
-          for (int j = 0; j < natoms; ++j) {
-            if (j == i) continue;
-            const double *Cj = C + 3 * j;
-            const double rj = gdist(x, y, z, Cj);
-            if (cap && ri - rj <= -5.0) continue;
+    }
+    const bool gact = wact && PB > 0.0 && Z > 0.0;
+    if (__ballot(gact)) {
+      for (int i = 0; i < natoms; ++i) {
+        bool li = gact;
+        double ri = 1.0, Pi = 0.0;
+        if (li) {
+          ri = gdist(x, y, z, C + 3 * i);
+          li = !(cap && ri - rmin >= 5.0);
+        }
+        if (li) {
+          Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
+          li = Pi != 0.0;
+        }
+        if (!__ballot(li)) continue;
+        const double coefi = li ? wf * ((i == B ? 1.0 / PB : 0.0) - 1.0 / Z) * Pi : 0.0;
+        const double *Ci = C + 3 * i;
+        const double ui[3] = {(x - Ci[0]) / ri, (y - Ci[1]) / ri, (z - Ci[2]) / ri};
+        double gi[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < natoms; ++j) {
+          if (j == i) continue;
+          const double *Cj = C + 3 * j;
+          double gj[3] = {0.0, 0.0, 0.0};
+          bool lj = li;
+          double rj = 1.0;
+          if (lj) {
+            rj = gdist(x, y, z, Cj);
+            lj = !(cap && ri - rj <= -5.0);
+          }
+          if (lj) {
             const double d[3] = {Ci[0] - Cj[0], Ci[1] - Cj[1], Ci[2] - Cj[2]};
             const double Rij = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             const bool capped_here = cap && Rij >= 5.0;
             const double Rt = capped_here ? 5.0 : Rij;
             const double mu = (ri - rj) / Rt;
-            if (mu <= -1.0 || mu >= 1.0) continue;  // clamped: s' = 0
-            const double p1 = bp(mu), p2 = bp(p1), p3 = bp(p2);
-            const double sv = 0.5 * (1.0 - p3);
-            const double ds = -0.5 * bdp(p2) * bdp(p1) * bdp(mu);
-            const double f = coefi * ds / sv;  // s > 0 because P_i > 0
-            const double ct = capped_here ? 0.0 : mu / (Rij * Rij);
-            const double uj[3] = {(x - Cj[0]) / rj, (y - Cj[1]) / rj, (z - Cj[2]) / rj};
-            for (int k = 0; k < 3; ++k) {
-              gi[k] += f * (-ui[k] / Rt - ct * d[k]);
-              if (j != B) {
-                const double gj = f * (uj[k] / Rt + ct * d[k]);
-                atomicAdd(&acc[3 * j + k], gj);
-                S[k] += gj;
+            if (mu > -1.0 && mu < 1.0) {  // clamped: s' = 0
+              const double p1 = bp(mu), p2 = bp(p1), p3 = bp(p2);
+              const double sv = 0.5 * (1.0 - p3);
+              const double ds = -0.5 * bdp(p2) * bdp(p1) * bdp(mu);
+              const double f = coefi * ds / sv;  // s > 0 because P_i > 0
+              const double ct = capped_here ? 0.0 : mu / (Rij * Rij);
+              const double uj[3] = {(x - Cj[0]) / rj, (y - Cj[1]) / rj, (z - Cj[2]) / rj};
+              for (int k = 0; k < 3; ++k) {
+                gi[k] += f * (-ui[k] / Rt - ct * d[k]);
+                if (j != B) gj[k] = f * (uj[k] / Rt + ct * d[k]);
               }
+            } else {
+              lj = false;
             }
           }
-          if (i != B)
-            for (int k = 0; k < 3; ++k) {
-              atomicAdd(&acc[3 * i + k], gi[k]);
-              S[k] += gi[k];
-            }
+          if (!__ballot(lj && j != B)) continue;
+          for (int k = 0; k < 3; ++k) {
+            S[k] += gj[k];
+            const double v = wave_sum(gj[k]);
+            if (lane == 0) atomicAdd(&acc[3 * j + k], v);
+          }
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double gv = (li && i != B) ? gi[k] : 0.0;
+          S[k] += gv;
+          const double v = wave_sum(gv);
+          if (lane == 0) atomicAdd(&acc[3 * i + k], v);
         }
       }
     }
-    for (int k = 0; k < 3; ++k) atomicAdd(&acc[3 * B + k], -S[k]);
+    // owner terms, one wave sum per distinct owner among the wave's points
+    unsigned long long pend = __ballot(act);
+    while (pend) {
+      const int src = __ffsll((long long)pend) - 1;
+      const int Bs = __shfl(B, src);
+      const bool mine = act && B == Bs;
+      for (int k = 0; k < 3; ++k) {
+        const double v = wave_sum(mine ? -S[k] : 0.0);
+        if (lane == 0) atomicAdd(&acc[3 * Bs + k], v);
+      }
+      pend &= ~__ballot(mine);
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 3 * natoms; e += blockDim.x)
