@@ -4,6 +4,9 @@
 // and the functional runs on the GPU.
 #pragma once
 
+#include <stdexcept>
+#include <vector>
+
 #include "density.hpp"
 #include "grid.hpp"
 #include "stobasis.hpp"
@@ -161,6 +164,35 @@ inline XC_result_polarized compute_vxc_polarized(const FlatGrid &grid, const STO
   check(nkx_vxc_polarized(ctx, P_alpha.data(), P_beta.data(), (int)func.ids.size(),
                           func.ids.data(), NKX_PREC_FP64, &r.energy, r.n_el,
                           r.potential_alpha.data(), r.potential_beta.data()),
+        ctx);
+  return r;
+}
+
+// XC nuclear gradient at fixed P (no counterpart in the reference): E_xc and
+// dE_xc/dR_A for every atom of the grid (atom-major x, y, z), with every basis
+// function centred on a grid atom.  partition says how the grid weights were
+// made (capped Becke for make_flat_grid).  FP64 or the int8 mixed form.
+enum class Partition { Fixed = 0, BeckeCapped = 1, Becke = 2 };
+
+struct XC_gradient {
+  double energy = 0.0;
+  std::vector<double> gradient;  // 3 natoms
+};
+
+inline XC_gradient compute_xc_gradient(const FlatGrid &grid, const STOBasisSet &basis,
+                                       const DeviceView2DLeft &P, const Functional &func,
+                                       Partition partition = Partition::BeckeCapped,
+                                       Precision prec = Precision::FP64) {
+  if (prec == Precision::MixedBF16)
+    throw std::runtime_error("the nuclear gradient is FP64 or the int8 mixed form");
+  nkx_ctx *ctx = context().get();
+  detail::use_grid(ctx, grid);
+  set_basis(basis);
+  XC_gradient r;
+  r.gradient.assign(3 * grid.atom_centers.size(), 0.0);
+  const auto fn = prec == Precision::FP64 ? nkx_xc_grad : nkx_xc_grad_mixed;
+  check(fn(ctx, P.data(), (int)func.ids.size(), func.ids.data(), (int)partition, &r.energy,
+           r.gradient.data()),
         ctx);
   return r;
 }

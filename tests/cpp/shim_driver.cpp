@@ -1,8 +1,9 @@
 // Exercises the C++ shim (include/nukexc/*.hpp) the way the reference's unit
 // tests exercise its headers (test_hydrogen.cxx:815-912, test_helium.cxx:
 // 67-121): grid, basis, collocation, compute_lda / compute_gga, the
-// polarised compute_lsda / compute_gga_lsda, the batched compute_vxc(_polarized)
-// and the error path.  Prints one line per check and
+// polarised compute_lsda / compute_gga_lsda, the batched compute_vxc(_polarized),
+// the nuclear gradient (FP64 and mixed) and the error path.  Prints one line
+// per check and
 // exits nonzero on the first failure.
 #include <cmath>
 #include <cstdio>
@@ -81,6 +82,28 @@ int main() {
   }
   if (!threw) {
     std::printf("FAIL compute_gga accepted an LDA functional\n");
+    return 1;
+  }
+  // nuclear gradient of H2 (1s on each atom, P = 0.6 everywhere, PBE):
+  // translational invariance, the mirror symmetry of the bond, and the mixed
+  // form against FP64
+  Molecule H2(std::vector<std::vector<double>>{{0., 0., -0.7}, {0., 0., 0.7}},
+              std::vector<unsigned>{1u, 1u});
+  auto grid2 = make_flat_grid<Becke, LebedevLaikov>(H2, 60, 17);
+  STOBasisSet basis2 =
+      make_manual_basis({{1, 0, 0, 1.0, 0., 0., -0.7}, {1, 0, 0, 1.0, 0., 0., 0.7}});
+  DeviceView2DLeft P2(2, 2);
+  P2.from_host(std::vector<double>{0.6, 0.6, 0.6, 0.6});
+  auto g64 = compute_xc_gradient(grid2, basis2, P2, Functional({101, 130}));
+  auto gmx = compute_xc_gradient(grid2, basis2, P2, Functional({101, 130}),
+                                 Partition::BeckeCapped, Precision::Mixed);
+  const auto &g = g64.gradient;
+  std::printf("H2 dE/dz: % .12f % .12f\n", g[2], g[5]);
+  double worst = 0.0;
+  for (int k = 0; k < 6; ++k) worst = std::fmax(worst, std::fabs(gmx.gradient[k] - g[k]));
+  if (std::fabs(g[2] + g[5]) > 1e-10 || std::fabs(g[0]) > 1e-10 || std::fabs(g[1]) > 1e-10 ||
+      std::fabs(g[2]) < 1e-4 || worst > 1e-7) {
+    std::printf("FAIL H2 gradient (sum %.2e, mixed vs FP64 %.2e)\n", g[2] + g[5], worst);
     return 1;
   }
   std::printf("shim OK\n");
