@@ -235,14 +235,16 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One thread per sorted point; per-atom sums in LDS (3 natoms doubles).
-// Every loop over atoms runs in step across the wave (lanes without a
-// contribution add zero), so each per-atom term is summed over the wave by
-// shuffles and added to LDS once per wave: all lanes of a wave touch the
-// same atoms, and one LDS atomic per lane would serialise on them.  The
-// points of a wave are 64 consecutive sorted points (npts_pad and the grid
-// stride are multiples of 64), so every wave runs the same trip count.
-__global__ void __launch_bounds__(256) grad_weights_kernel(
+// One thread per sorted point, one wave per workgroup; per-atom sums in LDS
+// (3 natoms doubles).  Every loop over atoms runs in step across the wave
+// (lanes without a contribution add zero), so each per-atom term is summed
+// over the wave by shuffles and added to LDS by lane 0: all lanes of a wave
+// touch the same atoms, and one LDS atomic per lane would serialise on them.
+// With one writer per workgroup (no atomics) the sums, and so the gradient,
+// come out the same on every run.  The points of a wave are 64 consecutive sorted points
+// (npts_pad and the grid stride are multiples of 64), so every lane runs the
+// same trip count.
+__global__ void __launch_bounds__(64) grad_weights_kernel(
     int64_t npts_pad, int natoms, const double *__restrict__ C, const double *__restrict__ px,
     const double *__restrict__ py, const double *__restrict__ pz, const int *__restrict__ pown,
     const double *__restrict__ epoint, const double *__restrict__ tpt, int partition,
@@ -330,14 +332,14 @@ __global__ void __launch_bounds__(256) grad_weights_kernel(
           for (int k = 0; k < 3; ++k) {
             S[k] += gj[k];
             const double v = wave_sum(gj[k]);
-            if (lane == 0) atomicAdd(&acc[3 * j + k], v);
+            if (lane == 0) acc[3 * j + k] += v;
           }
         }
         for (int k = 0; k < 3; ++k) {
           const double gv = (li && i != B) ? gi[k] : 0.0;
           S[k] += gv;
           const double v = wave_sum(gv);
-          if (lane == 0) atomicAdd(&acc[3 * i + k], v);
+          if (lane == 0) acc[3 * i + k] += v;
         }
       }
     }
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(256) grad_weights_kernel(
       const bool mine = act && B == Bs;
       for (int k = 0; k < 3; ++k) {
         const double v = wave_sum(mine ? -S[k] : 0.0);
-        if (lane == 0) atomicAdd(&acc[3 * Bs + k], v);
+        if (lane == 0) acc[3 * Bs + k] += v;
       }
       pend &= ~__ballot(mine);
     }
@@ -398,7 +400,7 @@ void vkg_owners(hipStream_t s, const Plan &pl, const int *owner, int *pown) {
 void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *centers,
                  const int *pown, const double *epoint, const double *tpt, int partition,
                  int nblk, double *wpart) {
-  hipLaunchKernelGGL(grad_weights_kernel, dim3((unsigned)nblk), dim3(256),
+  hipLaunchKernelGGL(grad_weights_kernel, dim3((unsigned)nblk), dim3(64),
                      (size_t)3 * natoms * sizeof(double), s, pl.npts_pad, natoms, centers,
                      pl.px.p, pl.py.p, pl.pz.p, pown, epoint, tpt, partition, wpart);
   NKX_HIP_CHECK(hipGetLastError());

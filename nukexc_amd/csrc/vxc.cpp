@@ -904,7 +904,7 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   const int nseg = 32;
-  const int nblk = 1024;
+  const int nblk = 4096;  // one-wave workgroups of the weight-derivative kernel
   if (mixed) {
     ws.mstore.reserve((size_t)slots_alloc * mixed_slot_bytes(gga ? 1 : 0));
     ws.dpart.reserve((size_t)slots_alloc * 4 * PT);

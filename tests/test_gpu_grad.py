@@ -132,6 +132,17 @@ def test_xc_grad_mixed_matches_fp64_partitions(ctx, partition):
     assert np.abs(gm - g).max() <= 1e-7
 
 
+def test_xc_grad_is_reproducible(ctx):
+    """Every sum of the gradient runs in a fixed order: two calls agree bit
+    for bit (FP64 and mixed)."""
+    w, P, fa = _setup(ctx, "benzene", 20, 11)
+    Pt = torch.tensor(P, device="cuda")
+    for prec in ("fp64", "mixed"):
+        E1, g1 = ctx.xc_grad(Pt, funcs=(101, 130), precision=prec)
+        E2, g2 = ctx.xc_grad(Pt, funcs=(101, 130), precision=prec)
+        assert E1 == E2 and np.array_equal(g1, g2), prec
+
+
 def test_xc_grad_rejects_off_atom_functions(ctx):
     w, P, fa = _setup(ctx, "water", 20, 11)
     b = dataclasses.replace(w["basis"], O=w["basis"].O + 0.1)
