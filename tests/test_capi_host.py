@@ -107,3 +107,19 @@ def test_null_context_is_rejected():
     L = nk.lib()
     assert L.nkx_sync(None) == 1
     assert b"null" in L.nkx_last_error(None)
+
+
+def test_function_atoms_map_and_rejection():
+    """The function -> atom map of the geometry gradcheck binding: water's 25
+    functions sit on O (15) and the two H (5 each); a basis moved off the
+    atoms is rejected."""
+    import dataclasses
+    torch = pytest.importorskip("torch")  # noqa: F841 (the module imports torch)
+    from nukexc_amd import configs
+    from nukexc_amd.autograd import function_atoms
+    w = configs.workload("water")
+    fa = function_atoms(w["basis"], w["xyz"])
+    assert list(np.bincount(fa, minlength=3)) == [15, 5, 5]
+    moved = dataclasses.replace(w["basis"], O=w["basis"].O + 1e-6)
+    with pytest.raises(ValueError):
+        function_atoms(moved, w["xyz"])
