@@ -513,3 +513,36 @@ def test_mfma_i8_lane_layout():
     for r in range(4):
         got[4 * (lane // 16) + r, lane % 16] = d.reshape(64, 4)[:, r]
     assert np.array_equal(got, D)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chunked_build_matches_single_chunk(precision):
+    """A workspace cap small enough to split the build into many chunks of
+    batches (collocation, density and V partials per chunk) gives the same
+    E_xc, N_el and V as one chunk, up to the order of the per-chunk partial
+    sums of V; the nuclear gradient likewise."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    xyz, Z, _ = configs.water_cluster(8, rng)
+    basis = nk.make_manual_basis(configs.sto_functions(Z, xyz, rng))
+    c = nk.Context(0)
+    try:
+        c.make_flat_grid(xyz, Z, nrad=40, nang_order=23)
+        c.set_basis(basis)
+        S = c.overlap().cpu().numpy()
+        C, P = configs.loewdin_orbitals(S, 40, rng)
+        Pd = dev(P)
+        one = c.vxc(Pd, funcs=(101, 130), precision=precision)
+        E1, g1 = c.xc_grad(Pd, funcs=(101, 130), precision=precision)
+        c.set_memory_budget(64 << 20)  # the minimum: tens of chunks here
+        many = c.vxc(Pd, funcs=(101, 130), precision=precision)
+        E2, g2 = c.xc_grad(Pd, funcs=(101, 130), precision=precision)
+    finally:
+        c.close()
+    dV = (many.potential - one.potential).abs().max().item()
+    print(f"{precision}: dE = {many.energy - one.energy:.1e}, dV = {dV:.1e}, "
+          f"dg = {np.abs(g2 - g1).max():.1e}")
+    assert abs(many.energy - one.energy) <= 1e-12 * abs(one.energy)
+    assert abs(many.n_el - one.n_el) <= 1e-12 * one.n_el
+    assert dV <= 1e-13
+    assert abs(E2 - E1) <= 1e-12 * abs(E1)
+    assert np.abs(g2 - g1).max() <= 1e-12
