@@ -150,11 +150,13 @@ inline XC_result compute_vxc(const FlatGrid &grid, const STOBasisSet &basis,
   return r;
 }
 
-// Open-shell batched build from the spin density matrices (FP64 form).
+// Open-shell batched build from the spin density matrices (FP64 or the int8
+// mixed form).
 inline XC_result_polarized compute_vxc_polarized(const FlatGrid &grid, const STOBasisSet &basis,
                                                  const DeviceView2DLeft &P_alpha,
                                                  const DeviceView2DLeft &P_beta,
-                                                 const Functional &func) {
+                                                 const Functional &func,
+                                                 Precision prec = Precision::FP64) {
   nkx_ctx *ctx = context().get();
   detail::use_grid(ctx, grid);
   set_basis(basis);
@@ -162,7 +164,7 @@ inline XC_result_polarized compute_vxc_polarized(const FlatGrid &grid, const STO
   r.potential_alpha.allocate(basis.nbf(), basis.nbf());
   r.potential_beta.allocate(basis.nbf(), basis.nbf());
   check(nkx_vxc_polarized(ctx, P_alpha.data(), P_beta.data(), (int)func.ids.size(),
-                          func.ids.data(), NKX_PREC_FP64, &r.energy, r.n_el,
+                          func.ids.data(), (int)prec, &r.energy, r.n_el,
                           r.potential_alpha.data(), r.potential_beta.data()),
         ctx);
   return r;

@@ -221,7 +221,9 @@ int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
    321-360 -> standalone_helpers.hpp:229-256): spin density matrices Pa, Pb,
    E_xc = sum_g w (rho_a + rho_b) eps, N_el[0..1] = per-spin electron counts
    (may be NULL), V_alpha and V_beta (row-major N_bf x N_bf, device).
-   FP64 form only: precision must be NKX_PREC_FP64. */
+   precision: NKX_PREC_FP64 or NKX_PREC_MIXED (int8 slices: one density
+   contraction and one V contraction per spin, the polarised functional in
+   FP64); NKX_PREC_MIXED_BF16 is rejected with NKX_ERR_UNSUPPORTED. */
 int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, int nfunc,
                       const int *func_ids, int precision, double *E_xc, double *N_el,
                       double *Va_dev, double *Vb_dev);

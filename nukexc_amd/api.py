@@ -267,9 +267,11 @@ class Context:
         return XC_result(E.value, V, N.value)
 
     def vxc_polarized(self, Pa, Pb, funcs=PBE, precision="fp64"):
-        """Spin-polarised build from the spin density matrices (FP64 form).
-        Returns XC_result_polarized(energy, V_alpha, V_beta, (N_alpha, N_beta))."""
-        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        """Spin-polarised build from the spin density matrices (FP64 or the
+        int8 mixed form).  Returns XC_result_polarized(energy, V_alpha,
+        V_beta, (N_alpha, N_beta))."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
+                "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
         Pa = _dev_f64(Pa, self.device).contiguous()
         Pb = _dev_f64(Pb, self.device).contiguous()
         nbf = Pa.shape[0]

@@ -680,8 +680,8 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfun
     NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED ||
                     precision == NKX_PREC_MIXED_BF16,
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
-    NKX_REQUIRE(precision == NKX_PREC_FP64, NKX_ERR_UNSUPPORTED,
-                "the spin-polarised build is available in the FP64 form only");
+    NKX_REQUIRE(precision != NKX_PREC_MIXED_BF16, NKX_ERR_UNSUPPORTED,
+                "the spin-polarised build is available in the FP64 and int8 forms");
     NKX_REQUIRE(Pa && Pb && E && Va && Vb && Va != Vb, NKX_ERR_INVALID_ARGUMENT,
                 "vxc_polarized: bad arguments");
     const double *P[2] = {Pa, Pb};

@@ -7,6 +7,7 @@
 #include "vxc_kernels.hpp"
 #include "slices_i8.hpp"
 #include "xc_functionals.hpp"
+#include "xc_polarized.hpp"
 
 namespace nkx {
 
@@ -1368,15 +1369,76 @@ __global__ void __launch_bounds__(256) density_f_i8(
   }
 }
 
+// Spin-polarised form of density_f_i8: the slot sums of the two spin
+// contractions (part_a, part_b) give rho_s and grad rho_s per point; the
+// polarised functional gives the Z coefficients of each spin,
+//   a_s = w v_rho_s / 2,  b_s = w (2 v_sigma_ss grad rho_s + v_sigma_ab grad rho_s'),
+// (xc_integrals.hpp:655-664) into coef_a, coef_b ([point][4] each), and
+// (E, N_alpha, N_beta) per batch.
+__global__ void __launch_bounds__(256) density_fp_i8(
+    int64_t s0, int64_t b0, const int *__restrict__ b_off, int gga,
+    const double *__restrict__ part_a, const double *__restrict__ part_b,
+    const double *__restrict__ pw, XcList xl, double *__restrict__ coef_a,
+    double *__restrict__ coef_b, double *__restrict__ blockEN) {
+  const int64_t b = b0 + blockIdx.x;
+  __shared__ double cs[2][4][PT];
+  const int c = threadIdx.x / PT, g = threadIdx.x % PT;
+  const int lo = b_off[b], hi = b_off[b + 1];
+  double acc_a = 0.0, acc_b = 0.0;
+  for (int k = lo; k < hi; ++k) {
+    acc_a += part_a[(k - s0) * (4 * PT) + c * PT + g];
+    acc_b += part_b[(k - s0) * (4 * PT) + c * PT + g];
+  }
+  cs[0][c][g] = acc_a;
+  cs[1][c][g] = acc_b;
+  __syncthreads();
+  if (threadIdx.x < PT) {
+    const int64_t gp = b * PT + g;
+    const double w = pw[gp];
+    const double r2[2] = {cs[0][0][g], cs[1][0][g]};
+    double gr[2][3];
+    for (int s2 = 0; s2 < 2; ++s2)
+      for (int k = 0; k < 3; ++k) gr[s2][k] = 2.0 * cs[s2][k + 1][g];
+    double sg[3] = {0.0, 0.0, 0.0};
+    if (gga) {
+      sg[0] = gr[0][0] * gr[0][0] + gr[0][1] * gr[0][1] + gr[0][2] * gr[0][2];
+      sg[1] = gr[0][0] * gr[1][0] + gr[0][1] * gr[1][1] + gr[0][2] * gr[1][2];
+      sg[2] = gr[1][0] * gr[1][0] + gr[1][1] * gr[1][1] + gr[1][2] * gr[1][2];
+    }
+    double exc, vr[2], vs[3];
+    xc_eval_polarized(xl.n, xl.ids, xl.thr, r2, sg, exc, vr, vs);
+    double *cf[2] = {coef_a, coef_b};
+    for (int s2 = 0; s2 < 2; ++s2) {
+      const double vss = gga ? 2.0 * w * vs[2 * s2] : 0.0, vab = gga ? w * vs[1] : 0.0;
+      cf[s2][4 * gp] = 0.5 * w * vr[s2];
+      for (int k = 0; k < 3; ++k) cf[s2][4 * gp + 1 + k] = vss * gr[s2][k] + vab * gr[1 - s2][k];
+    }
+    double e = w * (r2[0] + r2[1]) * exc, na = w * r2[0], nb = w * r2[1];
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      e += __shfl_down(e, o2);
+      na += __shfl_down(na, o2);
+      nb += __shfl_down(nb, o2);
+    }
+    if (g == 0) {
+      blockEN[3 * b] = e;
+      blockEN[3 * b + 1] = na;
+      blockEN[3 * b + 2] = nb;
+    }
+  }
+}
+
 // Z = a Phi + b . grad Phi of one live slot in FP64, per-function exponents,
-// slices (F order); one workgroup per slot.  Thread t owns the 16 entries of
+// slices (F order) written at zout + (slot - s0) zstride (the slot's own Z
+// section, or a separate array for the second spin); one workgroup per slot.  Thread t owns the 16 entries of
 // fragment item t (row 16 (t / 64) + t % 16, points 16 ((t % 64) / 16) ..
 // +15): 16-byte loads of the four F planes, 64-byte loads of each gradient
 // row piece, the row maximum over the four threads of a row by shuffles,
 // 16-byte stores of the Z planes.
 __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restrict__ slot_b,
                                                   int gga, const double *__restrict__ coef,
-                                                  signed char *__restrict__ store) {
+                                                  signed char *__restrict__ store,
+                                                  signed char *__restrict__ zout,
+                                                  int64_t zstride) {
   const int64_t sl = s0 + blockIdx.x;
   const int64_t b = slot_b[sl];
   __shared__ double cf[PT][4];
@@ -1433,11 +1495,11 @@ __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restr
 #pragma unroll
     for (int p = 0; p < 4; ++p) out[p][e] = sc[p];
   }
+  signed char *zs = zout + (int64_t)blockIdx.x * zstride;
 #pragma unroll
   for (int p = 0; p < 4; ++p)
-    *reinterpret_cast<v4i *>(tb + OFF_Z + p * PLANE + t * 16) =
-        *reinterpret_cast<const v4i *>(out[p]);
-  if (l < 16) tb[OFF_Z + 4 * PLANE + row] = (signed char)ez;
+    *reinterpret_cast<v4i *>(zs + p * PLANE + t * 16) = *reinterpret_cast<const v4i *>(out[p]);
+  if (l < 16) zs[4 * PLANE + row] = (signed char)ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -1456,7 +1518,8 @@ __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restr
 __global__ void __launch_bounds__(512) vgemm_i8(
     int nsp, int nsplit, const int4 *__restrict__ sp_blk, const int4 *__restrict__ sp_pidx,
     int npairs, const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1,
-    int64_t s0, int gga, const signed char *__restrict__ store, double *__restrict__ vpart) {
+    int64_t s0, int gga, const signed char *__restrict__ store,
+    const signed char *__restrict__ zbase, int64_t zstride, double *__restrict__ vpart) {
   // split-major: the super-pairs of one split run together and walk the same
   // batches, sharing each batch's slots in the caches
   const int sp = blockIdx.x % nsp;
@@ -1503,19 +1566,20 @@ __global__ void __launch_bounds__(512) vgemm_i8(
       const int sJ = ls[2 + jb][i];
       if (sJ < 0) continue;
       const signed char *tJ = store + (sJ - s0) * sb_bytes;
-      dma_section(tJ + OFF_Z, &dst[2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
+      dma_section(zbase + (sJ - s0) * zstride, &dst[2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
       dma_section(tJ + OFF_F, &dst[2 * jb + 1][0], (wv + 4 * jb + 2) & 7, 8, lane);
     }
     const int sI = ls[half][i];
     if (sI >= 0) {
       const signed char *tI = store + (sI - s0) * sb_bytes;
+      const signed char *zI = zbase + (sI - s0) * zstride;
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) {
         an.F[pl] = ld16(tI + OFF_F + pl * PLANE + (rw * 64 + lane) * 16);
-        an.Z[pl] = ld16(tI + OFF_Z + pl * PLANE + (rw * 64 + lane) * 16);
+        an.Z[pl] = ld16(zI + pl * PLANE + (rw * 64 + lane) * 16);
       }
       an.eF = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * rw + 4 * q);
-      an.eZ = *reinterpret_cast<const int *>(tI + OFF_Z + 4 * PLANE + 16 * rw + 4 * q);
+      an.eZ = *reinterpret_cast<const int *>(zI + 4 * PLANE + 16 * rw + 4 * q);
     }
   };
   // one batch of the list: wait for its operands, start the next batch's
@@ -1783,9 +1847,30 @@ void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_
   const int64_t s1 = pl.b_off_h[b1];
   if (s1 > s0)
     hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
-                       pl.slot_b.p, gga, coef, store);
+                       pl.slot_b.p, gga, coef, store, store + OFF_Z, slot_bytes_i8(gga));
   NKX_HIP_CHECK(hipGetLastError());
 }
+
+void vkm_density_finish_pol(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                            int gga, signed char *store, signed char *zstore_b,
+                            const XcList &xl, double *coef_a, double *coef_b,
+                            double *blockEN, const double *part_a, const double *part_b) {
+  if (b1 <= b0) return;
+  hipLaunchKernelGGL(density_fp_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
+                     pl.b_off.p, gga, part_a, part_b, pl.pw.p, xl, coef_a, coef_b, blockEN);
+  NKX_HIP_CHECK(hipGetLastError());
+  const int64_t s1 = pl.b_off_h[b1];
+  if (s1 > s0) {
+    hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+                       pl.slot_b.p, gga, coef_a, store, store + OFF_Z, slot_bytes_i8(gga));
+    NKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,
+                       pl.slot_b.p, gga, coef_b, store, zstore_b, (int64_t)SEC);
+  }
+  NKX_HIP_CHECK(hipGetLastError());
+}
+
+int64_t mixed_zsection_bytes() { return SEC; }
 
 void vkm_density_functional(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                             int gga, const XcList &xl, double *coef, double *blockEN,
@@ -1797,12 +1882,15 @@ void vkm_density_functional(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, c
 }
 
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
-               int gga, const signed char *store, double *vpart) {
+               int gga, const signed char *store, double *vpart, const signed char *zstore_b) {
   if (b1 <= b0 || pl.nspairs == 0) return;
+  // Z sections: each slot's own, or (second spin) SEC-strided in zstore_b
+  const signed char *zbase = zstore_b ? zstore_b : store + OFF_Z;
+  const int64_t zstride = zstore_b ? (int64_t)SEC : slot_bytes_i8(gga);
   hipLaunchKernelGGL(vgemm_i8, dim3((unsigned)(pl.nspairs * nsplit)), dim3(512), 0, s,
                      pl.nspairs, nsplit, reinterpret_cast<const int4 *>(pl.sp_blk.p),
                      reinterpret_cast<const int4 *>(pl.sp_pidx.p), pl.npairs, pl.tile_slot.p,
-                     pl.tab(), b0, b1, s0, gga, store, vpart);
+                     pl.tab(), b0, b1, s0, gga, store, zbase, zstride, vpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

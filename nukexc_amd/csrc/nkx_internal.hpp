@@ -168,7 +168,8 @@ struct Workspace {
   DevBuf<double> coef;             // per padded point and spin: a, bx, by, bz
   DevBuf<double> blockEN;          // per batch: E, N_alpha, N_beta partial sums
   DevBuf<double> vpart;            // split-K partial V tiles
-  DevBuf<double> dpart;            // per slot: column sums of the density contraction (int8 split form)
+  DevBuf<double> dpart;            // per spin and slot: column sums of the density contraction (int8 split form)
+  DevBuf<signed char> zstore_b;    // int8 form, second spin: Z sections per slot
   DevBuf<double> red;              // small reduction outputs
   DevBuf<double> scratch;
   DevBuf<double> gbuf;             // nuclear-gradient buffers
@@ -226,6 +227,7 @@ struct BuildSetup {
   int nspin = 1, narr = 0, nsplit = 1, group = 4;
   bool gga = false, bf16 = false, mixed = false;
   int64_t slots_alloc = 0;
+  int64_t psb = 0;  // int8 form: bytes of one spin's P sections (all molecules)
   size_t vsz = 0;
 };
 

@@ -532,24 +532,37 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
   b.nsplit = choose_nsplit(pl, tuned_split, precision == NKX_PREC_MIXED);
   b.bf16 = precision == NKX_PREC_MIXED_BF16;
   b.mixed = precision == NKX_PREC_MIXED || b.bf16;
-  NKX_REQUIRE(!b.mixed || nspin == 1, NKX_ERR_UNSUPPORTED,
-              "the spin-polarised build is available in the FP64 form only");
+  NKX_REQUIRE(!b.bf16 || nspin == 1, NKX_ERR_UNSUPPORTED,
+              "the spin-polarised build is available in the FP64 and int8 forms");
+  // the polarised int8 build needs the split density form (contraction per
+  // spin, then one polarised functional)
+  if (b.mixed && !b.bf16 && nspin == 2 && !(b.group & 224)) b.group = 65;
   NKX_REQUIRE(!b.bf16 || nmol == 1, NKX_ERR_UNSUPPORTED,
               "the split-bf16 form builds one molecule at a time");
   const size_t slot_bytes = b.bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
                             : b.mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0)
                                       : (size_t)b.narr * FB * PT * sizeof(double);
-  int64_t max_slots = chunk_slot_budget(slot_bytes, mem_cap);
+  // per slot beyond the tile store: density partials per spin and the second
+  // spin's Z section (int8 form)
+  const size_t slot_extra =
+      b.mixed && !b.bf16
+          ? (size_t)nspin * 4 * PT * sizeof(double) +
+                (nspin == 2 ? (size_t)mixed_zsection_bytes() : 0)
+          : 0;
+  int64_t max_slots = chunk_slot_budget(slot_bytes + slot_extra, mem_cap);
   // at least the largest single batch
   for (int64_t bb = 0; bb < pl.nb; ++bb)
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[bb + 1] - pl.b_off_h[bb]);
   b.slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   if (b.mixed) {
     ws.mstore.reserve((size_t)b.slots_alloc * slot_bytes);
-    if (!b.bf16) ws.dpart.reserve((size_t)b.slots_alloc * 4 * PT);
+    if (!b.bf16) ws.dpart.reserve((size_t)nspin * b.slots_alloc * 4 * PT);
+    if (!b.bf16 && nspin == 2)
+      ws.zstore_b.reserve((size_t)b.slots_alloc * mixed_zsection_bytes());
     int64_t psb = 0;
     for (int m = 0; m < nmol; ++m) psb += mixed_p_bytes(pl.mol_nfb_h[m]);
-    ws.Ps.reserve(b.bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)psb);
+    b.psb = psb;
+    ws.Ps.reserve(b.bf16 ? (size_t)6 * pl.nbf_pad * pl.nbf_pad : (size_t)nspin * psb);
   } else {
     ws.tiles64.reserve((size_t)b.slots_alloc * b.narr * FB * PT);
     ws.Pd.reserve((size_t)nspin * pl.P_total);
@@ -585,7 +598,8 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
       if (bf16)
         vkb_gather_p(s, nbf, pl.nfb, fp, Pm, bPs);
       else if (mixed)
-        vkm_gather_p(s, nbf, pl.mol_nfb_h[m], fp, Pm, ws.Ps.p + pl.mol_Ps_off_h[m]);
+        vkm_gather_p(s, nbf, pl.mol_nfb_h[m], fp, Pm,
+                     ws.Ps.p + (size_t)sp * b.psb + pl.mol_Ps_off_h[m]);
       else
         vk_gather_p(s, nbf, pl.mol_ld_h[m], fp, Pm,
                     ws.Pd.p + (size_t)sp * pl.P_total + pl.mol_P_off_h[m]);
@@ -607,23 +621,33 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
     if (bf16)
       vkb_density(s, 6, b0, b1, s0, pl, bPs, ig, bstore, xl, ws.coef.p, ws.blockEN.p);
     else if (mixed)
-      vkm_density(s, b.group, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p,
-                  ws.blockEN.p, ws.dpart.p);
+      for (int sp = 0; sp < nspin; ++sp)  // one contraction per spin
+        vkm_density(s, b.group, b0, b1, s0, pl, ws.Ps.p + (size_t)sp * b.psb, ig,
+                    ws.mstore.p, xl, ws.coef.p, ws.blockEN.p,
+                    ws.dpart.p + (size_t)sp * b.slots_alloc * 4 * PT);
     else
       vk_density(s, nspin, b.group, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl,
                  ws.coef.p, ws.blockEN.p, nullptr);
     tm.end(tt, s);
     if (mixed && !bf16) {
       tt = tm.begin(kStageZ, s);
-      vkm_density_finish(s, b.group, b0, b1, s0, pl, ig, ws.mstore.p, xl, ws.coef.p,
-                         ws.blockEN.p, ws.dpart.p);
+      if (nspin == 2)
+        vkm_density_finish_pol(s, b0, b1, s0, pl, ig, ws.mstore.p, ws.zstore_b.p, xl,
+                               ws.coef.p, ws.coef.p + (size_t)4 * pl.npts_pad, ws.blockEN.p,
+                               ws.dpart.p, ws.dpart.p + (size_t)b.slots_alloc * 4 * PT);
+      else
+        vkm_density_finish(s, b.group, b0, b1, s0, pl, ig, ws.mstore.p, xl, ws.coef.p,
+                           ws.blockEN.p, ws.dpart.p);
       tm.end(tt, s);
     }
     tt = tm.begin(kStageVgemm, s);
     if (bf16)
       vkb_vgemm(s, 6, b0, b1, s0, pl, nsplit, ig, bstore, ws.vpart.p);
-    else if (mixed)
+    else if (mixed) {
       vkm_vgemm(s, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p);
+      if (nspin == 2)
+        vkm_vgemm(s, b0, b1, s0, pl, nsplit, ig, ws.mstore.p, ws.vpart.p + vsz, ws.zstore_b.p);
+    }
     else
       vk_vgemm(s, b0, b1, s0, pl, nsplit, narr, nspin, ws.tiles64.p, ws.vpart.p);
     tm.end(tt, s);

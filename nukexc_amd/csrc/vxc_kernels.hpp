@@ -50,8 +50,18 @@ void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_
 void vkm_density_functional(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                             int gga, const XcList &xl, double *coef, double *blockEN,
                             const double *part, double *epoint);
+// zstore_b (may be NULL): Z sections of the second spin, mixed_zsection_bytes()
+// per slot, instead of the slots' own
 void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nsplit,
-               int gga, const signed char *store, double *vpart);
+               int gga, const signed char *store, double *vpart,
+               const signed char *zstore_b = nullptr);
+// spin-polarised functional and Z slices of both spins (split density form):
+// Z_alpha into the slots, Z_beta into zstore_b
+void vkm_density_finish_pol(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
+                            int gga, signed char *store, signed char *zstore_b,
+                            const XcList &xl, double *coef_a, double *coef_b,
+                            double *blockEN, const double *part_a, const double *part_b);
+int64_t mixed_zsection_bytes();
 
 // split-bf16 mixed form (kernels_vxc_bf16.hip): three bf16 planes per value,
 // six products; bf16_slot_bytes(gga) per slot, 3 nbf_pad^2 bf16 for P

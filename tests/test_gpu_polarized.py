@@ -134,10 +134,43 @@ def test_vxc_polarized_spin_symmetric_equals_closed_shell(ctx):
     assert np.abs(r2.potential_beta.cpu().numpy() - V).max() < 1e-11
 
 
+@pytest.mark.parametrize("name", ["water", "benzene"])
+def test_vxc_polarized_mixed_matches_fp64(ctx, name):
+    """The int8 form of the open-shell build (one density and one V
+    contraction per spin, polarised functional in FP64) against the FP64
+    form: the mixed-form bars on E_xc and both potentials, and N per spin."""
+    kw = dict(nrad=40, nang_order=23) if name == "benzene" else {}
+    w, grid, pts, wts, Ca, Cb, Pa, Pb = _open_shell(ctx, name, **kw)
+    r64 = ctx.vxc_polarized(dev(Pa), dev(Pb), funcs=(101, 130))
+    rmx = ctx.vxc_polarized(dev(Pa), dev(Pb), funcs=(101, 130), precision="mixed")
+    dVa = (rmx.potential_alpha - r64.potential_alpha).abs().max().item()
+    dVb = (rmx.potential_beta - r64.potential_beta).abs().max().item()
+    print(f"{name}: dE = {rmx.energy - r64.energy:.1e}, dV_a = {dVa:.1e}, dV_b = {dVb:.1e}, "
+          f"dN = {rmx.n_el[0] - r64.n_el[0]:.1e}, {rmx.n_el[1] - r64.n_el[1]:.1e}")
+    assert abs(rmx.energy - r64.energy) <= 1e-6
+    assert dVa <= 1e-7 and dVb <= 1e-7
+    assert abs(rmx.n_el[0] - r64.n_el[0]) <= 1e-6 and abs(rmx.n_el[1] - r64.n_el[1]) <= 1e-6
+
+
+def test_vxc_polarized_mixed_spin_symmetric_equals_closed_shell(ctx):
+    """Pa = Pb = P/2 in the int8 form: the slices of P/2 are those of P one
+    exponent down, so the polarised build repeats the closed-shell int8 build
+    up to the rounding of the two functional forms."""
+    w, grid, pts, wts, Ca, Cb, Pa, Pb = _open_shell(ctx, "water")
+    r1 = ctx.vxc(dev(2 * Pa), funcs=(101, 130), precision="mixed")
+    r2 = ctx.vxc_polarized(dev(Pa), dev(Pa), funcs=(101, 130), precision="mixed")
+    V = r1.potential.cpu().numpy()
+    print(f"dE = {r2.energy - r1.energy:.1e}, "
+          f"dV = {np.abs(r2.potential_alpha.cpu().numpy() - V).max():.1e}")
+    assert abs(r2.energy - r1.energy) < 1e-10 * abs(r1.energy)
+    assert np.abs(r2.potential_alpha.cpu().numpy() - V).max() < 1e-9
+    assert np.abs(r2.potential_beta.cpu().numpy() - V).max() < 1e-9
+
+
 def test_polarized_errors(ctx):
     w, grid, pts, wts, Ca, Cb, Pa, Pb = _open_shell(ctx, "water", nrad=20, nang_order=11)
     with pytest.raises(nk.NukexcError) as ei:
-        ctx.vxc_polarized(dev(Pa), dev(Pb), funcs=w["funcs"], precision="mixed")
+        ctx.vxc_polarized(dev(Pa), dev(Pb), funcs=w["funcs"], precision="mixed_bf16")
     assert ei.value.status == "NKX_ERR_UNSUPPORTED"
     phi = nk.fill_collocation(ctx, w["basis"], grid.quad_points)
     with pytest.raises(nk.NukexcError) as ei:
