@@ -270,6 +270,13 @@ int nkx_molset_vxc(nkx_ctx *ctx, const double *const *P_dev, int nfunc, const in
    NKX_ERR_STATE until the next capture. */
 int nkx_graph_capture(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                       int precision, double *V_dev);
+/* Open-shell form of nkx_graph_capture (no counterpart in the reference):
+   one nkx_vxc_polarized build (FP64 or NKX_PREC_MIXED) for these spin
+   density matrices and potentials; nkx_graph_launch then returns E_xc and,
+   when N is not NULL, N[0] = N_alpha and N[1] = N_beta. */
+int nkx_graph_capture_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev,
+                                int nfunc, const int *func_ids, int precision, double *Va_dev,
+                                double *Vb_dev);
 /* The same for one nkx_molset_vxc build of the ctx molecule set (no
    counterpart in the reference): P_dev[m], V_dev[m] per molecule; replays
    with nkx_graph_launch, which then fills E_xc[m] and N_el[m] for every

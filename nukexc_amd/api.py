@@ -343,14 +343,32 @@ class Context:
         self._graph_keep = (P, V)
         self._chk(lib().nkx_graph_capture(self.h, _ptr(P), nf, ids, prec, _ptr(V)))
         self._graph_nmol = 1
+        self._graph_nspin = 1
+
+    def graph_capture_polarized(self, Pa, Pb, Va, Vb, funcs=PBE, precision="fp64"):
+        """Capture one open-shell build for the device tensors Pa, Pb (inputs)
+        and Va, Vb (outputs); graph_launch() then returns (E_xc, (N_a, N_b))."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        for t in (Pa, Pb, Va, Vb):
+            assert t.is_contiguous() and t.dtype == torch.float64
+        ids, nf = _ids(funcs)
+        self._graph_keep = (Pa, Pb, Va, Vb)
+        self._chk(lib().nkx_graph_capture_polarized(self.h, _ptr(Pa), _ptr(Pb), nf, ids, prec,
+                                                    _ptr(Va), _ptr(Vb)))
+        self._graph_nmol = 1
+        self._graph_nspin = 2
 
     def graph_launch(self):
-        """Replay the captured build; returns (E_xc, N_el)."""
+        """Replay the captured build; returns (E_xc, N_el), or (E_xc,
+        (N_alpha, N_beta)) for an open-shell capture."""
         if getattr(self, "_graph_nmol", 1) != 1:
             raise RuntimeError("the captured build is a molecule set: use molset_graph_launch")
-        E, N = ctypes.c_double(), ctypes.c_double()
-        self._chk(lib().nkx_graph_launch(self.h, ctypes.byref(E), ctypes.byref(N)))
-        return E.value, N.value
+        E = ctypes.c_double()
+        N = (ctypes.c_double * 2)()
+        self._chk(lib().nkx_graph_launch(self.h, ctypes.byref(E), N))
+        if getattr(self, "_graph_nspin", 1) == 2:
+            return E.value, (N[0], N[1])
+        return E.value, N[0]
 
     # -- molecule sets ---------------------------------------------------------
     def molset_create(self, mols, nrad=75, nang_order=29, weight_threshold=1e-30,
@@ -430,6 +448,7 @@ class Context:
         ids, nf = _ids(funcs)
         self._chk(lib().nkx_molset_graph_capture(self.h, Pp, nf, ids, prec, Vp))
         self._graph_nmol = nmol
+        self._graph_nspin = 1
 
     def molset_graph_launch(self):
         """Replay the captured molecule-set build; returns (E list, N list)."""

@@ -732,7 +732,25 @@ int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, 
                     precision == NKX_PREC_MIXED_BF16,
                 NKX_ERR_INVALID_ARGUMENT, "unknown precision");
     NKX_REQUIRE(P && V, NKX_ERR_INVALID_ARGUMENT, "graph_capture: bad arguments");
-    nkx::graph_capture(ctx->c, P, xl, gga, precision, V);
+    nkx::graph_capture(ctx->c, 1, &P, xl, gga, precision, &V);
+  });
+}
+
+int nkx_graph_capture_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfunc,
+                                const int *ids, int precision, double *Va, double *Vb) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    nkx::XcList xl;
+    bool gga = false;
+    NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
+                NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_UNSUPPORTED, "the open-shell build takes FP64 or NKX_PREC_MIXED");
+    NKX_REQUIRE(Pa && Pb && Va && Vb && Va != Vb, NKX_ERR_INVALID_ARGUMENT,
+                "graph_capture_polarized: bad arguments");
+    const double *P[2] = {Pa, Pb};
+    double *V[2] = {Va, Vb};
+    nkx::graph_capture(ctx->c, 2, P, xl, gga, precision, V);
   });
 }
 

@@ -211,6 +211,7 @@ struct GraphState {
   hipGraphExec_t exec = nullptr;
   double *en_pinned = nullptr;  // (E, N_alpha, N_beta) per molecule, written by the graph
   int nmol = 1;
+  int nspin = 1;                // 2: an open-shell capture (one molecule)
   bool molset = false;          // captured on the molecule-set plan
   uint64_t mplan_version = 0;
   std::vector<double *> vtab_h;
@@ -311,8 +312,9 @@ void vxc_build_spin(Ctx &c, int nspin, const double *const *P, const XcList &xl,
                     int precision, double *E, double *N, double *const *V);
 void build_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
                    bool gga, int precision, double *E, double *N, double *const *V);
-void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
-                   double *V);
+// nspin = 1: P[0] -> V[0]; nspin = 2: (P_alpha, P_beta) -> (V_alpha, V_beta)
+void graph_capture(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+                   int precision, double *const *V);
 void graph_launch(Ctx &c, double *E, double *N);
 void molset_graph_capture(Ctx &c, const double *const *P, const XcList &xl, bool gga,
                           int precision, double *const *V);

@@ -796,19 +796,20 @@ double molset_autotune(Ctx &c, const double *const *P, const XcList &xl, bool gg
 // ---- HIP graph of one build on the ctx grid and basis ----------------------
 namespace {
 // Capture one build on plan pl (nmol molecules: P[m], V[m]) into c.graph.
-void capture_on_plan(Ctx &c, Plan &pl, const double *const *P, const XcList &xl, bool gga,
-                     int precision, double *const *V) {
+void capture_on_plan(Ctx &c, Plan &pl, int nspin, const double *const *P, const XcList &xl,
+                     bool gga, int precision, double *const *V) {
   GraphState &gs = c.graph;
   gs.release();
   const int nmol = pl.nmol;
   const BuildSetup b =
-      prepare_build(gs.ws, pl, 1, gga, precision, c.opt.vgemm_split, c.opt.density_group,
+      prepare_build(gs.ws, pl, nspin, gga, precision, c.opt.vgemm_split, c.opt.density_group,
                     c.opt.mem_budget);
   NKX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&gs.en_pinned),
                               3 * (size_t)nmol * sizeof(double), hipHostMallocDefault));
   gs.nmol = nmol;
-  gs.vtab_h.assign(V, V + nmol);
-  gs.vtab.upload(gs.vtab_h.data(), nmol, c.stream);
+  gs.nspin = nspin;
+  gs.vtab_h.assign(V, V + (size_t)nspin * nmol);
+  gs.vtab.upload(gs.vtab_h.data(), (size_t)nspin * nmol, c.stream);
   NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
   const bool timing = c.timing.on;
   c.timing.on = false;  // no event records inside the graph
@@ -828,10 +829,10 @@ void capture_on_plan(Ctx &c, Plan &pl, const double *const *P, const XcList &xl,
 }
 }  // namespace
 
-void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int precision,
-                   double *V) {
+void graph_capture(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+                   int precision, double *const *V) {
   ensure_plan(c);
-  capture_on_plan(c, c.plan, &P, xl, gga, precision, &V);
+  capture_on_plan(c, c.plan, nspin, P, xl, gga, precision, V);
   GraphState &gs = c.graph;
   gs.molset = false;
   gs.grid_version = c.grid.version;
@@ -842,14 +843,15 @@ void graph_capture(Ctx &c, const double *P, const XcList &xl, bool gga, int prec
 void molset_graph_capture(Ctx &c, const double *const *P, const XcList &xl, bool gga,
                           int precision, double *const *V) {
   NKX_REQUIRE(c.mplan.nmol > 0, NKX_ERR_STATE, "no molecule set: call nkx_molset_create");
-  capture_on_plan(c, c.mplan, P, xl, gga, precision, V);
+  capture_on_plan(c, c.mplan, 1, P, xl, gga, precision, V);
   GraphState &gs = c.graph;
   gs.molset = true;
   gs.mplan_version = c.mplan_version;
   gs.screen_tol = c.mplan.screen_tol;
 }
 
-// Replay the captured build; E[m], N[m] for each molecule of the capture.
+// Replay the captured build; E[m], N[m] for each molecule of the capture
+// (an open-shell capture: E[0], N[0] = N_alpha, N[1] = N_beta).
 void graph_launch(Ctx &c, double *E, double *N) {
   GraphState &gs = c.graph;
   NKX_REQUIRE(gs.exec, NKX_ERR_STATE, "no captured build: call nkx_graph_capture");
@@ -862,6 +864,14 @@ void graph_launch(Ctx &c, double *E, double *N) {
                 NKX_ERR_STATE, "grid, basis or screening changed since the capture");
   NKX_HIP_CHECK(hipGraphLaunch(gs.exec, c.stream));
   NKX_HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (gs.nspin == 2) {
+    E[0] = gs.en_pinned[0];
+    if (N) {
+      N[0] = gs.en_pinned[1];
+      N[1] = gs.en_pinned[2];
+    }
+    return;
+  }
   for (int m = 0; m < gs.nmol; ++m) {
     E[m] = gs.en_pinned[3 * m];
     if (N) N[m] = gs.en_pinned[3 * m + 1];

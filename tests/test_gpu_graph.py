@@ -45,6 +45,32 @@ def test_graph_replays_match_eager(setup, precision):
         assert dV <= 1e-13
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_open_shell_graph_replays_match_eager(setup, precision):
+    """nkx_graph_capture_polarized: every replay, with the spin density
+    matrices changed in place and eager builds in between, matches the eager
+    open-shell build bit for bit."""
+    c, w, P1, P2 = setup
+    nbf = P1.shape[0]
+    Pa = torch.tensor(P1 / 2, device="cuda")
+    Pb = torch.tensor(P2 / 2, device="cuda")
+    Va = torch.empty((nbf, nbf), dtype=torch.float64, device="cuda")
+    Vb = torch.empty_like(Va)
+    c.graph_capture_polarized(Pa, Pb, Va, Vb, funcs=w["funcs"], precision=precision)
+    for it in range(4):
+        pa, pb = (P1 / 2, P2 / 2) if it % 2 == 0 else (P2 / 2, P1 / 2)
+        Pa.copy_(torch.tensor(pa, device="cuda"))
+        Pb.copy_(torch.tensor(pb, device="cuda"))
+        torch.cuda.synchronize()
+        E, (Na, Nb) = c.graph_launch()
+        Vga, Vgb = Va.clone(), Vb.clone()
+        r = c.vxc_polarized(torch.tensor(pa, device="cuda"), torch.tensor(pb, device="cuda"),
+                            funcs=w["funcs"], precision=precision)
+        print(f"{precision} open-shell replay {it}: dE={E - r.energy:.1e}")
+        assert E == r.energy and Na == r.n_el[0] and Nb == r.n_el[1]
+        assert torch.equal(Vga, r.potential_alpha) and torch.equal(Vgb, r.potential_beta)
+
+
 def test_graph_invalidated_by_new_grid(setup):
     c, w, P1, P2 = setup
     nbf = P1.shape[0]
