@@ -244,6 +244,15 @@ int nkx_xc_grad(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_id
    and results as nkx_xc_grad; E_xc is the mixed-form energy. */
 int nkx_xc_grad_mixed(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
                       int partition, double *E_xc, double *grad_host);
+/* Open-shell nuclear gradient (no counterpart in the reference): E_xc of the
+   spin density matrices Pa, Pb (device, nbf x nbf) with the polarised
+   functional, differentiated as nkx_xc_grad.  precision: NKX_PREC_FP64, or
+   NKX_PREC_MIXED for the int8 contractions of nkx_xc_grad_mixed (one density
+   and one basis-centre contraction per spin); anything else is
+   NKX_ERR_INVALID_ARGUMENT. */
+int nkx_xc_grad_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, int nfunc,
+                          const int *func_ids, int partition, int precision, double *E_xc,
+                          double *grad_host);
 /* Molecule set (no counterpart in the reference, whose driver handles one
    molecule): nmol molecules of different sizes, each with its own grid built
    as nkx_make_flat_grid does (shared grid parameters) and its own basis.

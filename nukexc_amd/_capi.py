@@ -72,6 +72,7 @@ SIGNATURES = {
     "nkx_vxc_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _pd, _pd, _vp, _vp],
     "nkx_xc_grad": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
     "nkx_xc_grad_mixed": [_vp, _vp, _c_int, _pi, _c_int, _pd, _pd],
+    "nkx_xc_grad_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _c_int, _pd, _pd],
     "nkx_autotune": [_vp, _vp, _c_int, _pi, _c_int, _vp, _pi, _pi, _pd],
     "nkx_molset_autotune": [_vp, ctypes.POINTER(_vp), _c_int, _pi, _c_int, ctypes.POINTER(_vp),
                             _pi, _pi, _pd],

@@ -302,6 +302,22 @@ class Context:
                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return E.value, g.reshape(natoms, 3)
 
+    def xc_grad_polarized(self, Pa, Pb, funcs=PBE, partition="becke_capped", precision="fp64"):
+        """Open-shell nuclear gradient of E_xc[P_alpha, P_beta] at fixed
+        spin density matrices, in FP64 or the mixed form.  Returns (E_xc,
+        grad ndarray (natoms, 3))."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
+        Pa = _dev_f64(Pa, self.device).contiguous()
+        Pb = _dev_f64(Pb, self.device).contiguous()
+        natoms = self.grid.atom_centers.shape[0]
+        g = np.zeros(3 * natoms)
+        ids, nf = _ids(funcs)
+        E = ctypes.c_double()
+        self._chk(lib().nkx_xc_grad_polarized(
+            self.h, _ptr(Pa), _ptr(Pb), nf, ids, self.PARTITION[partition], prec,
+            ctypes.byref(E), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return E.value, g.reshape(natoms, 3)
+
     def autotune(self, P, funcs=PBE, precision="fp64"):
         """Pick the fastest density workgroup shape and V-contraction split
         count for this workload; returns (split, shape, ms per build)."""

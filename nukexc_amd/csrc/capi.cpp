@@ -695,29 +695,42 @@ int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfun
   });
 }
 
-static int xc_grad_entry(nkx_ctx *ctx, const double *P, int nfunc, const int *ids,
-                         int partition, bool mixed, double *E, double *grad) {
+static int xc_grad_entry(nkx_ctx *ctx, int nspin, const double *const *P, int nfunc,
+                         const int *ids, int partition, int precision, double *E,
+                         double *grad) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
+    NKX_REQUIRE(precision == NKX_PREC_FP64 || precision == NKX_PREC_MIXED,
+                NKX_ERR_INVALID_ARGUMENT, "xc_grad: precision must be FP64 or MIXED");
+    const bool mixed = precision == NKX_PREC_MIXED;
     nkx::XcList xl;
     bool gga = false;
     NKX_REQUIRE(nkx::xc_list_make(nfunc, ids, ctx->c.opt.dens_threshold, &xl, &gga),
                 NKX_ERR_UNKNOWN_FUNCTIONAL, "unknown functional id or list longer than 4");
-    NKX_REQUIRE(P && grad, NKX_ERR_INVALID_ARGUMENT, "xc_grad: bad arguments");
+    NKX_REQUIRE(grad, NKX_ERR_INVALID_ARGUMENT, "xc_grad: bad arguments");
+    for (int sp = 0; sp < nspin; ++sp)
+      NKX_REQUIRE(P[sp], NKX_ERR_INVALID_ARGUMENT, "xc_grad: bad arguments");
     double e = 0.0;
-    nkx::xc_grad(ctx->c, P, xl, gga, partition, mixed, &e, grad);
+    nkx::xc_grad(ctx->c, nspin, P, xl, gga, partition, mixed, &e, grad);
     if (E) *E = e;
   });
 }
 
 int nkx_xc_grad(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
                 double *E, double *grad) {
-  return xc_grad_entry(ctx, P, nfunc, ids, partition, false, E, grad);
+  return xc_grad_entry(ctx, 1, &P, nfunc, ids, partition, NKX_PREC_FP64, E, grad);
 }
 
 int nkx_xc_grad_mixed(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int partition,
                       double *E, double *grad) {
-  return xc_grad_entry(ctx, P, nfunc, ids, partition, true, E, grad);
+  return xc_grad_entry(ctx, 1, &P, nfunc, ids, partition, NKX_PREC_MIXED, E, grad);
+}
+
+int nkx_xc_grad_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int nfunc,
+                          const int *ids, int partition, int precision, double *E,
+                          double *grad) {
+  const double *P[2] = {Pa, Pb};
+  return xc_grad_entry(ctx, 2, P, nfunc, ids, partition, precision, E, grad);
 }
 
 int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,

@@ -34,14 +34,17 @@ constexpr int GC = FB * 3;  // per-slot partial: 64 functions x 3 components
 
 // ---------------------------------------------------------------------------
 // One workgroup per batch.  Tile store arrays: Phi, dPhi/dx, dPhi/dy, dPhi/dz
-// (narr >= 4).  coef per point: (w v_rho / 2, b_x, b_y, b_z).
+// (narr >= 4).  coef per point (stride cstride doubles): (w v_rho / 2, b_x,
+// b_y, b_z) of the spin whose P is Pd; accum adds the point sums to tpt (the
+// second spin of an open shell) instead of writing them.
 __global__ void __launch_bounds__(256) grad_basis_f64(
     int nbf_pad, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int narr, int gga,
     const double *__restrict__ tiles, const double *__restrict__ coef,
     const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
     const double *__restrict__ px, const double *__restrict__ py,
-    const double *__restrict__ pz, double *__restrict__ cpart, double *__restrict__ tpt) {
+    const double *__restrict__ pz, int cstride, int accum, double *__restrict__ cpart,
+    double *__restrict__ tpt) {
   const int b = (int)(b0 + blockIdx.x);
   const int lo = b_off[b], hi = b_off[b + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -56,7 +59,7 @@ __global__ void __launch_bounds__(256) grad_basis_f64(
 
   for (int e = threadIdx.x; e < 4 * PT; e += 256) {
     const int c = e / PT, g = e % PT;
-    cfs[c][g] = coef[4 * ((int64_t)b * PT + g) + c];
+    cfs[c][g] = coef[cstride * ((int64_t)b * PT + g) + c];
   }
   for (int e = threadIdx.x; e < 3 * PT; e += 256) (&tsum[0][0])[e] = 0.0;
   __syncthreads();
@@ -168,7 +171,8 @@ __global__ void __launch_bounds__(256) grad_basis_f64(
   }
   if (threadIdx.x < 3 * PT) {
     const int k = threadIdx.x / PT, g = threadIdx.x % PT;
-    tpt[3 * ((int64_t)b * PT + g) + k] = -tsum[k][g];
+    const int64_t e = 3 * ((int64_t)b * PT + g) + k;
+    tpt[e] = (accum ? tpt[e] : 0.0) - tsum[k][g];
   }
 }
 
@@ -374,11 +378,11 @@ __global__ void __launch_bounds__(256) colsum_kernel(int nrow, int ncol,
 // ---------------------------------------------------------------------------
 void vkg_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                const double *Pd, int narr, int gga, const double *tiles, const double *coef,
-               double *cpart, double *tpt) {
+               int cstride, bool accum, double *cpart, double *tpt) {
   if (b1 <= b0) return;
   hipLaunchKernelGGL(grad_basis_f64, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, pl.nbf_pad, Pd,
                      pl.b_off.p, pl.b_fbl.p, s0, b0, narr, gga, tiles, coef, pl.funcs.p,
-                     pl.terms.p, pl.px.p, pl.py.p, pl.pz.p, cpart, tpt);
+                     pl.terms.p, pl.px.p, pl.py.p, pl.pz.p, cstride, accum ? 1 : 0, cpart, tpt);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

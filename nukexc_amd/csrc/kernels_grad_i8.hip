@@ -24,13 +24,17 @@ namespace {
 constexpr int GC = FB * 3;  // per-slot partial: 64 functions x 3 components
 }  // namespace
 
-// Y slices of one live slot; one workgroup per slot.  Thread t owns fragment
-// item t: point 16 (t / 64) + t % 16, functions 16 ((t % 64) / 16) .. +15.
+// Y slices of one live slot, written at yout + (slot - s0) ystride (the
+// slot's Z section, or a separate array for the second spin); one workgroup
+// per slot.  Thread t owns fragment item t: point 16 (t / 64) + t % 16,
+// functions 16 ((t % 64) / 16) .. +15.
 // Phi comes from the Pm slices (per-point exponent), grad Phi from the FP32
 // planes; the row maximum over the four threads of a point by shuffles.
 __global__ void __launch_bounds__(256) y_slots_i8(int64_t s0, const int *__restrict__ slot_b,
                                                   int gga, const double *__restrict__ coef,
-                                                  signed char *__restrict__ store) {
+                                                  signed char *__restrict__ store,
+                                                  signed char *__restrict__ yout,
+                                                  int64_t ystride) {
   const int64_t sl = s0 + blockIdx.x;
   const int64_t b = slot_b[sl];
   signed char *tb = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
@@ -76,11 +80,11 @@ __global__ void __launch_bounds__(256) y_slots_i8(int64_t s0, const int *__restr
 #pragma unroll
     for (int p = 0; p < 4; ++p) out[p][e] = sc[p];
   }
+  signed char *ys = yout + (int64_t)blockIdx.x * ystride;
 #pragma unroll
   for (int p = 0; p < 4; ++p)
-    *reinterpret_cast<v4i *>(tb + OFF_Z + p * PLANE + t * 16) =
-        *reinterpret_cast<const v4i *>(out[p]);
-  if (l < 16) tb[OFF_Z + 4 * PLANE + g] = (signed char)ey;
+    *reinterpret_cast<v4i *>(ys + p * PLANE + t * 16) = *reinterpret_cast<const v4i *>(out[p]);
+  if (l < 16) ys[4 * PLANE + g] = (signed char)ey;
 }
 
 // One workgroup (four waves) per batch.  For each live block I of the batch,
@@ -97,7 +101,8 @@ __global__ void __launch_bounds__(256) grad_basis_i8(
     const signed char *__restrict__ store, const double *__restrict__ coef,
     const FuncRec *__restrict__ F, const HarmTerm *__restrict__ T,
     const double *__restrict__ px, const double *__restrict__ py,
-    const double *__restrict__ pz, double *__restrict__ cpart, double *__restrict__ tpt) {
+    const double *__restrict__ pz, const signed char *__restrict__ ybase, int64_t ystride,
+    int accum, double *__restrict__ cpart, double *__restrict__ tpt) {
   const int b = (int)(b0 + blockIdx.x);
   const int lo = b_off[b], hi = b_off[b + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -139,7 +144,7 @@ __global__ void __launch_bounds__(256) grad_basis_i8(
       for (int j = 0; j < 4; ++j) {
 #pragma unroll
         for (int op = 0; op < 2; ++op) {
-          const signed char *sec = tb + (op ? OFF_Z : OFF_PM);
+          const signed char *sec = op ? ybase + (ki - s0) * ystride : tb + OFF_PM;
           v4i B[4];
 #pragma unroll
           for (int p = 0; p < 4; ++p) B[p] = ld16(sec + p * PLANE + (j * 64 + lane) * 16);
@@ -228,26 +233,31 @@ __global__ void __launch_bounds__(256) grad_basis_i8(
   }
   if (threadIdx.x < 3 * PT) {
     const int k = threadIdx.x / PT, g = threadIdx.x % PT;
-    tpt[3 * ((int64_t)b * PT + g) + k] = -tsum[k][g];
+    const int64_t e = 3 * ((int64_t)b * PT + g) + k;
+    tpt[e] = (accum ? tpt[e] : 0.0) - tsum[k][g];
   }
 }
 
 // ---------------------------------------------------------------------------
 void vkgm_y_slots(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
-                  const double *coef, signed char *store) {
+                  const double *coef, signed char *store, signed char *ystore_b) {
   if (s1 <= s0) return;
+  signed char *yout = ystore_b ? ystore_b : store + OFF_Z;
+  const int64_t ystride = ystore_b ? (int64_t)SEC : slot_bytes_i8(gga);
   hipLaunchKernelGGL(y_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0, pl.slot_b.p,
-                     gga, coef, store);
+                     gga, coef, store, yout, ystride);
   NKX_HIP_CHECK(hipGetLastError());
 }
 
 void vkgm_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const signed char *Ps, int gga, const signed char *store, const double *coef,
-                double *cpart, double *tpt) {
+                const signed char *ystore_b, bool accum, double *cpart, double *tpt) {
   if (b1 <= b0) return;
+  const signed char *ybase = ystore_b ? ystore_b : store + OFF_Z;
+  const int64_t ystride = ystore_b ? (int64_t)SEC : slot_bytes_i8(gga);
   hipLaunchKernelGGL(grad_basis_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, pl.tab(), Ps,
                      pl.b_off.p, pl.b_fbl.p, s0, b0, gga, store, coef, pl.funcs.p, pl.terms.p,
-                     pl.px.p, pl.py.p, pl.pz.p, cpart, tpt);
+                     pl.px.p, pl.py.p, pl.pz.p, ybase, ystride, accum ? 1 : 0, cpart, tpt);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -1379,7 +1379,7 @@ __global__ void __launch_bounds__(256) density_fp_i8(
     int64_t s0, int64_t b0, const int *__restrict__ b_off, int gga,
     const double *__restrict__ part_a, const double *__restrict__ part_b,
     const double *__restrict__ pw, XcList xl, double *__restrict__ coef_a,
-    double *__restrict__ coef_b, double *__restrict__ blockEN) {
+    double *__restrict__ coef_b, double *__restrict__ blockEN, double *__restrict__ epoint) {
   const int64_t b = b0 + blockIdx.x;
   __shared__ double cs[2][4][PT];
   const int c = threadIdx.x / PT, g = threadIdx.x % PT;
@@ -1414,6 +1414,7 @@ __global__ void __launch_bounds__(256) density_fp_i8(
       for (int k = 0; k < 3; ++k) cf[s2][4 * gp + 1 + k] = vss * gr[s2][k] + vab * gr[1 - s2][k];
     }
     double e = w * (r2[0] + r2[1]) * exc, na = w * r2[0], nb = w * r2[1];
+    if (epoint) epoint[gp] = e;  // w e_xc of this point
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       e += __shfl_down(e, o2);
       na += __shfl_down(na, o2);
@@ -1854,11 +1855,14 @@ void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_
 void vkm_density_finish_pol(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                             int gga, signed char *store, signed char *zstore_b,
                             const XcList &xl, double *coef_a, double *coef_b,
-                            double *blockEN, const double *part_a, const double *part_b) {
+                            double *blockEN, const double *part_a, const double *part_b,
+                            double *epoint) {
   if (b1 <= b0) return;
   hipLaunchKernelGGL(density_fp_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
-                     pl.b_off.p, gga, part_a, part_b, pl.pw.p, xl, coef_a, coef_b, blockEN);
+                     pl.b_off.p, gga, part_a, part_b, pl.pw.p, xl, coef_a, coef_b, blockEN,
+                     epoint);
   NKX_HIP_CHECK(hipGetLastError());
+  if (!store) return;  // functional only (the gradient forms its own slices)
   const int64_t s1 = pl.b_off_h[b1];
   if (s1 > s0) {
     hipLaunchKernelGGL(z_slots_i8, dim3((unsigned)(s1 - s0)), dim3(256), 0, s, s0,

@@ -326,8 +326,9 @@ void build_molset_plan(Ctx &c, const std::vector<std::vector<double>> &mol_pts,
                        const std::vector<std::vector<double>> &mol_w,
                        const std::vector<BasisHost> &bases);
 void overlap_build(Ctx &c, double *S);
-// nuclear gradient of E_xc at fixed P; mixed: int8-slice contractions
-void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, bool mixed,
-             double *E, double *grad);
+// nuclear gradient of E_xc at fixed P (nspin spin density matrices, or the
+// total one); mixed: int8-slice contractions
+void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+             int partition, bool mixed, double *E, double *grad);
 
 }  // namespace nkx

@@ -900,10 +900,12 @@ void vxc_build(Ctx &c, const double *P, const XcList &xl, bool gga, int precisio
   vxc_build_spin(c, 1, &P, xl, gga, precision, E, N, &V);
 }
 
-// Nuclear gradient of E_xc at fixed P (kernels_grad.hip).  Every basis
-// function must sit on an atom of the grid; grad is host memory, 3 natoms.
-void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition, bool mixed,
-             double *E, double *grad) {
+// Nuclear gradient of E_xc at fixed P (kernels_grad.hip).  nspin = 1: P[0]
+// is the total density matrix; nspin = 2: P[0], P[1] the spin density
+// matrices.  Every basis function must sit on an atom of the grid; grad is
+// host memory, 3 natoms.
+void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
+             int partition, bool mixed, double *E, double *grad) {
   ensure_plan(c);
   Plan &pl = c.plan;
   Workspace &ws = c.ws;
@@ -913,6 +915,7 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
               "the gradient supports 1 .. 2048 atoms");
   NKX_REQUIRE(partition >= 0 && partition <= 2, NKX_ERR_INVALID_ARGUMENT,
               "partition: 0 fixed weights, 1 capped Becke, 2 Becke");
+  NKX_REQUIRE(nspin == 1 || nspin == 2, NKX_ERR_NSPIN, "nspin must be 1 or 2");
   // function (sorted order) -> atom
   std::vector<int> fatom(pl.nbf, -1);
   for (int sp = 0; sp < pl.nbf; ++sp) {
@@ -927,11 +930,14 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
                 "basis function " + std::to_string(i) + " is not centred on an atom of the grid");
   }
   NKX_REQUIRE(pl.nmol == 1, NKX_ERR_STATE, "the gradient runs on the ctx grid and basis");
-  const int narr = 5;  // Phi, grad Phi (3), Z
-  // FP64: the tile arrays; mixed: the int8 slot (sections and FP32 grad Phi)
-  // and the density partials; both plus the per-slot gradient partials
-  const size_t slot_bytes = mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0) + 4 * PT * sizeof(double)
-                                  : (size_t)narr * FB * PT * sizeof(double);
+  const int narr = 4 + nspin;  // Phi, grad Phi (3), Z per spin
+  // FP64: the tile arrays; mixed: the int8 slot (sections and FP32 grad Phi),
+  // the second spin's Y sections and the density partials; both plus the
+  // per-slot gradient partials
+  const size_t slot_bytes =
+      mixed ? (size_t)mixed_slot_bytes(gga ? 1 : 0) + (size_t)nspin * 4 * PT * sizeof(double) +
+                  (nspin == 2 ? (size_t)mixed_zsection_bytes() : 0)
+            : (size_t)narr * FB * PT * sizeof(double);
   int64_t max_slots =
       chunk_slot_budget(slot_bytes + 3 * FB * sizeof(double), c.opt.mem_budget);
   for (int64_t b = 0; b < pl.nb; ++b)
@@ -939,15 +945,18 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   const int nseg = 32;
   const int nblk = 4096;  // one-wave workgroups of the weight-derivative kernel
+  const size_t psb = (size_t)mixed_p_bytes(pl.mol_nfb_h[0]);
+  const size_t dsz = (size_t)slots_alloc * 4 * PT;
   if (mixed) {
     ws.mstore.reserve((size_t)slots_alloc * mixed_slot_bytes(gga ? 1 : 0));
-    ws.dpart.reserve((size_t)slots_alloc * 4 * PT);
-    ws.Ps.reserve((size_t)mixed_p_bytes(pl.mol_nfb_h[0]));
+    ws.dpart.reserve(nspin * dsz);
+    ws.Ps.reserve(nspin * psb);
+    if (nspin == 2) ws.zstore_b.reserve((size_t)slots_alloc * mixed_zsection_bytes());
   } else {
     ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
-    ws.Pd.reserve((size_t)pl.nbf_pad * pl.nbf_pad);
+    ws.Pd.reserve((size_t)nspin * pl.P_total);
   }
-  ws.coef.reserve((size_t)4 * pl.npts_pad);
+  ws.coef.reserve((size_t)4 * nspin * pl.npts_pad);
   ws.blockEN.reserve((size_t)3 * pl.nb);
   ws.red.reserve(3);
   // gradient buffers: epoint, tpt, cpart, gacc, gfun, wpart, wsum
@@ -959,30 +968,53 @@ void xc_grad(Ctx &c, const double *P, const XcList &xl, bool gga, int partition,
          *gfun = gacc + n_ga, *wpart = gfun + n_gf, *wsum = wpart + n_wp;
   ws.gint.reserve(pl.npts_pad);
 
-  if (mixed)
-    vkm_gather_p(s, pl.nbf, pl.mol_nfb_h[0], pl.fperm_d.p, P, ws.Ps.p);
-  else
-    vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
+  for (int sp = 0; sp < nspin; ++sp) {
+    if (mixed)
+      vkm_gather_p(s, pl.nbf, pl.mol_nfb_h[0], pl.fperm_d.p, P[sp], ws.Ps.p + sp * psb);
+    else
+      vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P[sp],
+                  ws.Pd.p + (size_t)sp * pl.P_total);
+  }
   vk_fill0(s, gacc, (int64_t)n_ga);
   const int ig = gga ? 1 : 0;
+  double *coef_b = ws.coef.p + (size_t)4 * pl.npts_pad;
   for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
     if (mixed) {
-      // density by the split form (contraction, then the functional with w e
-      // per point), Y slices over the Z sections, then the basis-centre term
+      // density by the split form (contraction per spin, then the functional
+      // with w e per point), Y slices (the first spin's over the Z sections),
+      // then the basis-centre term per spin
       vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
-      vkm_density(s, 65, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, xl, ws.coef.p, ws.blockEN.p,
-                  ws.dpart.p);
-      vkm_density_functional(s, b0, b1, s0, pl, ig, xl, ws.coef.p, ws.blockEN.p, ws.dpart.p,
-                             epoint);
+      for (int sp = 0; sp < nspin; ++sp)
+        vkm_density(s, 65, b0, b1, s0, pl, ws.Ps.p + sp * psb, ig, ws.mstore.p, xl, ws.coef.p,
+                    ws.blockEN.p, ws.dpart.p + sp * dsz);
+      if (nspin == 1) {
+        vkm_density_functional(s, b0, b1, s0, pl, ig, xl, ws.coef.p, ws.blockEN.p, ws.dpart.p,
+                               epoint);
+      } else {
+        vkm_density_finish_pol(s, b0, b1, s0, pl, ig, nullptr, nullptr, xl, ws.coef.p, coef_b,
+                               ws.blockEN.p, ws.dpart.p, ws.dpart.p + dsz, epoint);
+        vkgm_y_slots(s, s0, s1, pl, ig, coef_b, ws.mstore.p, ws.zstore_b.p);
+      }
       vkgm_y_slots(s, s0, s1, pl, ig, ws.coef.p, ws.mstore.p);
-      vkgm_basis(s, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, ws.coef.p, cpart, tpt);
+      vkgm_basis(s, b0, b1, s0, pl, ws.Ps.p, ig, ws.mstore.p, ws.coef.p, nullptr, false, cpart,
+                 tpt);
+      vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
+      if (nspin == 2) {
+        vkgm_basis(s, b0, b1, s0, pl, ws.Ps.p + psb, ig, ws.mstore.p, coef_b, ws.zstore_b.p,
+                   true, cpart, tpt);
+        vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
+      }
     } else {
+      // the density kernel writes coef per point as [spin][4]
       vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
-      vk_density(s, 1, 4, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
+      vk_density(s, nspin, 4, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, xl, ws.coef.p,
                  ws.blockEN.p, epoint);
-      vkg_basis(s, b0, b1, s0, pl, ws.Pd.p, narr, ig, ws.tiles64.p, ws.coef.p, cpart, tpt);
+      for (int sp = 0; sp < nspin; ++sp) {
+        vkg_basis(s, b0, b1, s0, pl, ws.Pd.p + (size_t)sp * pl.P_total, narr, ig, ws.tiles64.p,
+                  ws.coef.p + 4 * sp, 4 * nspin, sp > 0, cpart, tpt);
+        vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
+      }
     }
-    vkg_fold(s, b0, b1, s0, pl, nseg, cpart, gacc);
   });
   vkg_colsum(s, nseg, pl.nbf_pad * 3, gacc, gfun);
   vk_en_reduce(s, pl, ws.blockEN.p, ws.red.p);

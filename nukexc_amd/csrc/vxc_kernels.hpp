@@ -57,10 +57,12 @@ void vkm_vgemm(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl
                const signed char *zstore_b = nullptr);
 // spin-polarised functional and Z slices of both spins (split density form):
 // Z_alpha into the slots, Z_beta into zstore_b
+// (store NULL: the functional only; epoint, may be NULL: w e_xc per point)
 void vkm_density_finish_pol(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                             int gga, signed char *store, signed char *zstore_b,
                             const XcList &xl, double *coef_a, double *coef_b,
-                            double *blockEN, const double *part_a, const double *part_b);
+                            double *blockEN, const double *part_a, const double *part_b,
+                            double *epoint = nullptr);
 int64_t mixed_zsection_bytes();
 
 // split-bf16 mixed form (kernels_vxc_bf16.hip): three bf16 planes per value,
@@ -79,18 +81,22 @@ void vkb_vgemm(hipStream_t s, int nprod, int64_t b0, int64_t b1, int64_t s0, con
 // grad_basis: per batch, per-function partial sums of the basis-centre
 // derivative over the batch (cpart[(slot - s0) * 64 * 3 + mu * 3 + k]) and the
 // per-point sum tpt[3 * gp + k] with the sign of the point-motion term.
+// coef of the spin of Pd with a stride of cstride doubles per point; accum
+// adds into tpt (second spin) instead of writing it
 void vkg_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                const double *Pd, int narr, int gga, const double *tiles, const double *coef,
-               double *cpart, double *tpt);
+               int cstride, bool accum, double *cpart, double *tpt);
 // mixed form (kernels_grad_i8.hip), on the int8 slot store after vkm_colloc,
 // vkm_density (split shape) and vkm_density_functional: Y = 2 a Phi + b . grad
 // Phi slices per slot (rows = points), then X = P Phi and W = P Y on integer
 // MFMA with the same per-function and per-point sums as vkg_basis
+// ystore_b (may be NULL): Y sections of the second spin, SEC bytes per slot,
+// instead of the slots' Z sections; accum as vkg_basis
 void vkgm_y_slots(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
-                  const double *coef, signed char *store);
+                  const double *coef, signed char *store, signed char *ystore_b = nullptr);
 void vkgm_basis(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                 const signed char *Ps, int gga, const signed char *store, const double *coef,
-                double *cpart, double *tpt);
+                const signed char *ystore_b, bool accum, double *cpart, double *tpt);
 // fold the per-slot partials of batches [b0, b1) into gacc[seg][nbf_pad * 3]
 void vkg_fold(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int nseg,
               const double *cpart, double *gacc);
