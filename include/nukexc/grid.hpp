@@ -31,6 +31,15 @@ struct FlatGrid {
   DeviceArray<int> point_owner;
 };
 
+namespace detail {
+// Bumped whenever a grid's points are reordered in place (create_bounding_boxes),
+// so that a ctx holding that grid is handed the new order.
+inline unsigned &grid_epoch() {
+  thread_local unsigned epoch = 0;
+  return epoch;
+}
+}  // namespace detail
+
 template <typename radial_type, typename angular_type>
 FlatGrid make_flat_grid(const Molecule &mol, const size_t nrad = 50,
                         const size_t nang_order = 30, const double weight_threshold = 1e-30,

@@ -9,6 +9,7 @@
 
 #include "density.hpp"
 #include "grid.hpp"
+#include "octree.hpp"
 #include "stobasis.hpp"
 
 namespace Nukexc {
@@ -123,7 +124,9 @@ namespace detail {
 inline void use_grid(nkx_ctx *ctx, const FlatGrid &grid) {
   thread_local const void *last_grid = nullptr;
   thread_local size_t last_npts = 0;
-  if (last_grid != grid.weights.data() || last_npts != grid.weights.size()) {
+  thread_local unsigned last_epoch = 0;
+  if (last_grid != grid.weights.data() || last_npts != grid.weights.size() ||
+      last_epoch != grid_epoch()) {
     check(nkx_set_grid(ctx, (int)grid.atom_centers.size(),
                        reinterpret_cast<const double *>(grid.atom_centers.data()),
                        (int64_t)grid.weights.size(),
@@ -132,6 +135,7 @@ inline void use_grid(nkx_ctx *ctx, const FlatGrid &grid) {
           ctx);
     last_grid = grid.weights.data();
     last_npts = grid.weights.size();
+    last_epoch = grid_epoch();
   }
 }
 }  // namespace detail
@@ -197,6 +201,105 @@ inline XC_gradient compute_xc_gradient(const FlatGrid &grid, const STOBasisSet &
            r.gradient.data()),
         ctx);
   return r;
+}
+
+// ---- screened ("sparse") forms ---------------------------------------------
+// The batched build screens on its own 64-point batches; the neighbor list
+// only has to belong to the grid (same number of points).
+namespace detail {
+inline void sparse_setup(const FlatGrid &grid, const STOBasisSet &basis, const NeighborList &nl) {
+  if ((size_t)nl.total_points != grid.weights.size())
+    throw std::runtime_error("the neighbor list was built for another grid");
+  use_grid(context().get(), grid);
+  set_basis(basis);
+}
+
+// P = C diag(f) C^T on the device
+inline DeviceView2DLeft density_matrix(const DeviceView2DLeft &C, const DeviceView1D &f) {
+  const size_t nbf = C.extent(0);
+  DeviceView2DLeft P(nbf, nbf);
+  check(nkx_density_matrix(context().get(), (int)nbf, (int)C.extent(1), C.data(), f.data(),
+                           P.data()),
+        context().get());
+  return P;
+}
+}  // namespace detail
+
+// density.hpp:97-148
+inline DeviceView1D compute_density_sparse(const STOBasisSet &basis, const FlatGrid &grid,
+                                           const NeighborList &nl,
+                                           const DeviceView2DLeft &mo_orbitals,
+                                           const DeviceView1D &mo_coeff) {
+  detail::sparse_setup(grid, basis, nl);
+  DeviceView2DLeft P = detail::density_matrix(mo_orbitals, mo_coeff);
+  DeviceView1D rho(grid.weights.size());
+  check(nkx_density_points(context().get(), P.data(), rho.data(), nullptr, nullptr, nullptr,
+                           nullptr),
+        context().get());
+  context().fence();
+  return rho;
+}
+
+// density.hpp:261-328
+inline void compute_density_and_sigma_sparse(const STOBasisSet &basis, const FlatGrid &grid,
+                                             const NeighborList &nl,
+                                             const DeviceView2DLeft &mo_orbitals,
+                                             const DeviceView1D &mo_coeff, DeviceView1D &rho,
+                                             DeviceView1D &gx_rho, DeviceView1D &gy_rho,
+                                             DeviceView1D &gz_rho, DeviceView1D &sigma) {
+  detail::sparse_setup(grid, basis, nl);
+  DeviceView2DLeft P = detail::density_matrix(mo_orbitals, mo_coeff);
+  check(nkx_density_points(context().get(), P.data(), rho.data(), gx_rho.data(), gy_rho.data(),
+                           gz_rho.data(), sigma.data()),
+        context().get());
+  context().fence();
+}
+
+namespace detail {
+inline XC_result_polarized lsda_sparse(const STOBasisSet &basis, const FlatGrid &grid,
+                                       const NeighborList &nl, const DeviceView2DLeft &mo_alpha,
+                                       const DeviceView1D &occ_alpha,
+                                       const DeviceView2DLeft &mo_beta,
+                                       const DeviceView1D &occ_beta, const Functional &func) {
+  sparse_setup(grid, basis, nl);
+  DeviceView2DLeft Pa = density_matrix(mo_alpha, occ_alpha);
+  DeviceView2DLeft Pb = density_matrix(mo_beta, occ_beta);
+  XC_result_polarized r = compute_vxc_polarized(grid, basis, Pa, Pb, func);
+  context().fence();
+  return r;
+}
+}  // namespace detail
+
+// xc_integrals.hpp:263-422
+inline XC_result_polarized compute_lsda_sparse(const STOBasisSet &basis, const FlatGrid &grid,
+                                               const NeighborList &nl,
+                                               const DeviceView2DLeft &mo_alpha,
+                                               const DeviceView1D &occ_alpha,
+                                               const DeviceView2DLeft &mo_beta,
+                                               const DeviceView1D &occ_beta,
+                                               const Functional &func) {
+  if (func.family != NKX_FAMILY_LDA)
+    throw std::runtime_error("Provided functional is not part of the LDA family");
+  return detail::lsda_sparse(basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, func);
+}
+
+// xc_integrals.hpp:728-977
+inline XC_result_polarized compute_gga_lsda_sparse(const STOBasisSet &basis,
+                                                   const FlatGrid &grid, const NeighborList &nl,
+                                                   const DeviceView2DLeft &mo_alpha,
+                                                   const DeviceView1D &occ_alpha,
+                                                   const DeviceView2DLeft &mo_beta,
+                                                   const DeviceView1D &occ_beta,
+                                                   const Functional &func) {
+  if (func.family != NKX_FAMILY_GGA)
+    throw std::runtime_error("Provided funtional is not a part of the GGA family");
+  for (int id : func.ids) {
+    int fam = 0;
+    check(nkx_functional_info(id, &fam, nullptr));
+    if (fam != NKX_FAMILY_GGA)
+      throw std::runtime_error("Provided funtional is not a part of the GGA family");
+  }
+  return detail::lsda_sparse(basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, func);
 }
 
 }  // namespace Nukexc

@@ -227,6 +227,40 @@ int nkx_vxc(nkx_ctx *ctx, const double *P_dev, int nfunc, const int *func_ids,
 int nkx_vxc_polarized(nkx_ctx *ctx, const double *Pa_dev, const double *Pb_dev, int nfunc,
                       const int *func_ids, int precision, double *E_xc, double *N_el,
                       double *Va_dev, double *Vb_dev);
+/* ------------------------------------------------------- screened grid */
+/* create_bounding_boxes (octree.hpp:71-138): permutes the points (device,
+   3 doubles per point), weights and owners (device; either may be NULL) in
+   place into Morton order (63-bit codes, 21 bits per axis of the position in
+   the points' bounding box; equal codes keep their order) and writes the
+   bounding box of each run of max_points_per_box consecutive points to boxes
+   (device, 6 doubles per box: min x, y, z, max x, y, z; ceil(npts /
+   max_points_per_box) boxes, the last one ragged).  Data derived from the old
+   point order is invalid afterwards, as in the reference.  Synchronous. */
+int nkx_bounding_boxes(nkx_ctx *ctx, int64_t npts, double *points_dev, double *weights_dev,
+                       int *owner_dev, int max_points_per_box, double *boxes_dev);
+/* build_neighbor_list (octree.hpp:317-389): for each box, the ascending
+   indices of the basis functions whose sphere (centre, cutoff radius; host
+   arrays, 3 and 1 doubles per function) meets the box (distance from the
+   centre to the box <= radius).  offsets (device, nboxes + 1 ints) is always
+   written; neighbors (device, offsets[nboxes] ints) when not NULL; *total
+   (may be NULL) = offsets[nboxes].  Call with neighbors NULL to size the list.
+   Synchronous. */
+int nkx_neighbor_list(nkx_ctx *ctx, int64_t nboxes, const double *boxes_dev, int nbf,
+                      const double *centers_host, const double *radii_host, int *offsets_dev,
+                      int *neighbors_dev, int64_t *total);
+/* P = C diag(f) C^T (C LayoutLeft N_bf x N_occ, f N_occ, P N_bf x N_bf; all
+   device): the density matrix of the (C, f) arguments of the reference's
+   density and XC calls (density.hpp:150-186). */
+int nkx_density_matrix(nkx_ctx *ctx, int nbf, int nocc, const double *C_dev, const double *f_dev,
+                       double *P_dev);
+/* rho, grad rho (x, y, z) and sigma = |grad rho|^2 of the density matrix P
+   at every point of the ctx grid, in the grid's order (the outputs of
+   compute_density_and_sigma(_sparse), density.hpp:188-328), by the batched
+   FP64 path with its own screening; every output but rho may be NULL.
+   Stream-ordered. */
+int nkx_density_points(nkx_ctx *ctx, const double *P_dev, double *rho_dev, double *gx_dev,
+                       double *gy_dev, double *gz_dev, double *sigma_dev);
+
 /* Nuclear gradient dE_xc/dR_A at fixed P (FP64; no counterpart in the
    reference, which has no gradients): basis-centre, grid-point-motion and
    partition-weight terms for a grid whose points move with their owner atom.

@@ -12,5 +12,8 @@ from .api import (PBE, TA_M3, TA_M4, Context, FlatGrid, STOBasisSet, XC_result, 
                   functional_info,
                   lebedev_rule, make_manual_basis, partition_becke, partition_becke_team,
                   radial_rule, radial_scale, solid_harmonic, sto_norm_cutoff)
+from .api import (NeighborList, build_neighbor_list, compute_density_and_sigma_sparse,  # noqa: F401
+                  compute_density_sparse, compute_gga_lsda_sparse, compute_lsda_sparse,
+                  create_bounding_boxes, density_matrix)
 
 __version__ = "0.1.0"

@@ -95,6 +95,10 @@ SIGNATURES = {
     "nkx_reset_timing": [_vp],
     "nkx_get_timing": [_vp, _c_int, _pd, _pi64],
     "nkx_plan_work": [_vp, _pd, _pd, _pd],
+    "nkx_bounding_boxes": [_vp, _c_i64, _vp, _vp, _vp, _c_int, _vp],
+    "nkx_neighbor_list": [_vp, _c_i64, _vp, _c_int, _pd, _pd, _vp, _vp, _pi64],
+    "nkx_density_matrix": [_vp, _c_int, _c_int, _vp, _vp, _vp],
+    "nkx_density_points": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"nkx_last_error": ctypes.c_char_p}
 

@@ -663,3 +663,129 @@ def compute_gga(ctx, collocation_values, gx, gy, gz, weights, mo_orbitals, mo_co
     """compute_gga (xc_integrals.hpp:424-541)."""
     return _xc(ctx, collocation_values, (gx, gy, gz), weights, mo_orbitals, mo_coeff, funcs,
                _capi.FAMILY_GGA)
+
+
+# ------------------------------------------------------ screened grid API ----
+@dataclass
+class NeighborList:
+    """octree.hpp:279-284: for box b, the basis functions
+    neighbors[offsets[b]:offsets[b + 1]] (ascending) whose cutoff sphere
+    meets the box."""
+    neighbors: "torch.Tensor"  # int32
+    offsets: "torch.Tensor"  # int32, nboxes + 1
+    max_points_per_box: int
+    total_points: int
+
+
+def create_bounding_boxes(ctx, grid, max_points_per_bb):
+    """create_bounding_boxes (octree.hpp:71-138): permutes grid.quad_points,
+    grid.weights and grid.point_owner in place into Morton order and returns
+    the (nboxes, 6) boxes (min x, y, z, max x, y, z) of consecutive runs of
+    max_points_per_bb points.  Anything derived from the old point order is
+    invalid afterwards; a ctx holding this grid is handed the new order."""
+    pts, w, own = grid.quad_points, grid.weights, grid.point_owner
+    if not (pts.is_contiguous() and pts.dtype == torch.float64 and w.is_contiguous()
+            and w.dtype == torch.float64 and own.is_contiguous() and own.dtype == torch.int32):
+        raise ValueError("the grid arrays must be contiguous (float64 points and weights, "
+                         "int32 owners) to be permuted in place")
+    npts, m = pts.shape[0], int(max_points_per_bb)
+    if m <= 0:
+        raise ValueError("max_points_per_bb must be positive")
+    boxes = torch.empty(((npts + m - 1) // m, 6), dtype=torch.float64, device=pts.device)
+    ctx._chk(lib().nkx_bounding_boxes(ctx.h, npts, _ptr(pts), _ptr(w), _ptr(own), m,
+                                      _ptr(boxes)))
+    if getattr(ctx, "grid", None) is grid:
+        ctx.set_grid(grid)
+    return boxes
+
+
+def build_neighbor_list(ctx, basis, bounding_boxes, max_points_per_box, total_points):
+    """build_neighbor_list (octree.hpp:317-389): the functions of basis whose
+    sphere (centre, cutoff radius) meets each box, as a NeighborList."""
+    boxes = bounding_boxes.contiguous()
+    nbox = boxes.shape[0]
+    O = np.ascontiguousarray(basis.O, np.float64)
+    r = np.ascontiguousarray(basis.cutoff_radii, np.float64)
+    pd = ctypes.POINTER(ctypes.c_double)
+    off = torch.empty(nbox + 1, dtype=torch.int32, device=boxes.device)
+    tot = ctypes.c_int64()
+    args = (ctx.h, nbox, _ptr(boxes), len(r), O.ctypes.data_as(pd), r.ctypes.data_as(pd),
+            _ptr(off))
+    ctx._chk(lib().nkx_neighbor_list(*args, None, ctypes.byref(tot)))
+    nbr = torch.empty(max(tot.value, 1), dtype=torch.int32, device=boxes.device)
+    ctx._chk(lib().nkx_neighbor_list(*args, _ptr(nbr), ctypes.byref(tot)))
+    return NeighborList(nbr[:tot.value], off, int(max_points_per_box), int(total_points))
+
+
+def _sparse_setup(ctx, basis, grid, nl):
+    if nl.total_points != grid.weights.shape[0]:
+        raise ValueError("the neighbor list was built for another grid")
+    if getattr(ctx, "grid", None) is not grid:
+        ctx.set_grid(grid)
+    if getattr(ctx, "basis", None) is not basis:
+        ctx.set_basis(basis)
+
+
+def density_matrix(ctx, mo_orbitals, mo_coeff):
+    """P = C diag(f) C^T on the device (C LayoutLeft N_bf x N_occ)."""
+    C = _dev_f64(mo_orbitals, ctx.device)
+    C = _left(C.reshape(C.shape[0], -1))
+    f = _dev_f64(mo_coeff, ctx.device).contiguous()
+    P = torch.empty((C.shape[0], C.shape[0]), dtype=torch.float64, device=ctx.device)
+    ctx._done(lib().nkx_density_matrix(ctx.h, C.shape[0], C.shape[1], _ptr(C), _ptr(f), _ptr(P)))
+    return P
+
+
+def _density_points(ctx, P, grads):
+    n = ctx.grid.weights.shape[0]
+    out = [torch.empty(n, dtype=torch.float64, device=ctx.device) for _ in range(5 if grads else 1)]
+    ptrs = [_ptr(o) for o in out] + [None] * (5 - len(out))
+    ctx._done(lib().nkx_density_points(ctx.h, _ptr(P.contiguous()), *ptrs))
+    return out
+
+
+def compute_density_sparse(ctx, basis, grid, nl, mo_orbitals, mo_coeff):
+    """compute_density_sparse (density.hpp:97-148): rho at every grid point.
+    The batched path screens on its own 64-point batches; the neighbor list
+    only has to belong to this grid."""
+    _sparse_setup(ctx, basis, grid, nl)
+    return _density_points(ctx, density_matrix(ctx, mo_orbitals, mo_coeff), False)[0]
+
+
+def compute_density_and_sigma_sparse(ctx, basis, grid, nl, mo_orbitals, mo_coeff):
+    """compute_density_and_sigma_sparse (density.hpp:261-328): rho, grad rho
+    (3), sigma."""
+    _sparse_setup(ctx, basis, grid, nl)
+    o = _density_points(ctx, density_matrix(ctx, mo_orbitals, mo_coeff), True)
+    return o[0], (o[1], o[2], o[3]), o[4]
+
+
+def _family_check(funcs, gga):
+    fams = [functional_info(f)[0] for f in funcs]
+    if gga and not all(f == _capi.FAMILY_GGA for f in fams):
+        raise NukexcError(4, "Provided functional is not a part of the GGA family")
+    if not gga and any(f == _capi.FAMILY_GGA for f in fams):
+        raise NukexcError(4, "Provided functional is not part of the LDA family")
+
+
+def _lsda_sparse(ctx, basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, funcs, gga):
+    _family_check(funcs, gga)
+    _sparse_setup(ctx, basis, grid, nl)
+    Pa = density_matrix(ctx, mo_alpha, occ_alpha)
+    Pb = density_matrix(ctx, mo_beta, occ_beta)
+    return ctx.vxc_polarized(Pa, Pb, funcs=funcs)
+
+
+def compute_lsda_sparse(ctx, basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, funcs):
+    """compute_lsda_sparse (xc_integrals.hpp:263-422) on the batched FP64
+    build from P_sigma = C_sigma diag(occ_sigma) C_sigma^T."""
+    return _lsda_sparse(ctx, basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, funcs,
+                        False)
+
+
+def compute_gga_lsda_sparse(ctx, basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta,
+                            funcs):
+    """compute_gga_lsda_sparse (xc_integrals.hpp:728-977), as
+    compute_lsda_sparse."""
+    return _lsda_sparse(ctx, basis, grid, nl, mo_alpha, occ_alpha, mo_beta, occ_beta, funcs,
+                        True)

@@ -733,6 +733,50 @@ int nkx_xc_grad_polarized(nkx_ctx *ctx, const double *Pa, const double *Pb, int 
   return xc_grad_entry(ctx, 2, P, nfunc, ids, partition, precision, E, grad);
 }
 
+int nkx_bounding_boxes(nkx_ctx *ctx, int64_t npts, double *points, double *weights, int *owner,
+                       int max_points_per_box, double *boxes) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(npts >= 0 && max_points_per_box > 0 && (npts == 0 || (points && boxes)),
+                NKX_ERR_INVALID_ARGUMENT, "bounding_boxes: bad arguments");
+    nkx::sparse_bounding_boxes(ctx->c.stream, npts, points, weights, owner, max_points_per_box,
+                               boxes);
+  });
+}
+
+int nkx_neighbor_list(nkx_ctx *ctx, int64_t nboxes, const double *boxes, int nbf,
+                      const double *centers, const double *radii, int *offsets, int *neighbors,
+                      int64_t *total) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nboxes >= 0 && nbf >= 0 && offsets && (nboxes == 0 || boxes) &&
+                    (nbf == 0 || (centers && radii)),
+                NKX_ERR_INVALID_ARGUMENT, "neighbor_list: bad arguments");
+    const int64_t t = nkx::sparse_neighbor_list(ctx->c.stream, nboxes, boxes, nbf, centers,
+                                                radii, offsets, neighbors);
+    if (total) *total = t;
+  });
+}
+
+int nkx_density_matrix(nkx_ctx *ctx, int nbf, int nocc, const double *C, const double *f,
+                       double *P) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(nbf >= 0 && nocc >= 0 && P && (nocc == 0 || (C && f)),
+                NKX_ERR_INVALID_ARGUMENT, "density_matrix: bad arguments");
+    nkx::vk_density_matrix(ctx->c.stream, nbf, nocc, C, f, P);
+  });
+}
+
+int nkx_density_points(nkx_ctx *ctx, const double *P, double *rho, double *gx, double *gy,
+                       double *gz, double *sigma) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(P && rho, NKX_ERR_INVALID_ARGUMENT, "density_points: P and rho are required");
+    nkx::density_points(ctx->c, P, rho, gx, gy, gz, sigma);
+  });
+}
+
 int nkx_graph_capture(nkx_ctx *ctx, const double *P, int nfunc, const int *ids, int precision,
                       double *V) {
   NKX_CTX_CHECK(ctx);

@@ -93,7 +93,8 @@ __global__ void __launch_bounds__(256) density_fused_f64(
     MolTab mt, int64_t pspin, int nfb, const double *__restrict__ Pd, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int narr, int gga,
     double *__restrict__ tiles, const double *__restrict__ pw, XcList xl,
-    double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint) {
+    double *__restrict__ coef, double *__restrict__ blockEN, double *__restrict__ epoint,
+    double *__restrict__ rhopt) {
   const int64_t bg = b0 + (int64_t)blockIdx.x * GBT;
   const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -253,6 +254,12 @@ __global__ void __launch_bounds__(256) density_fused_f64(
       cf[0][sb][3][g] = bs * gzr;
       e = w * o.e;
       na = w * rho;
+      if (rhopt) {  // rho and grad rho of this point
+        rhopt[4 * gp] = rho;
+        rhopt[4 * gp + 1] = gxr;
+        rhopt[4 * gp + 2] = gyr;
+        rhopt[4 * gp + 3] = gzr;
+      }
     } else {
       const double r2[2] = {colsum[0][sb][0][g], colsum[NSPIN - 1][sb][0][g]};
       double gr[2][3];
@@ -488,22 +495,23 @@ void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, 
 template <int NSPIN, int GBT>
 static void launch_density(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
                            const double *Pd, int narr, int gga, double *tiles, const XcList &xl,
-                           double *coef, double *blockEN, double *epoint) {
+                           double *coef, double *blockEN, double *epoint, double *rhopt) {
   const unsigned nwg = (unsigned)((b1 - b0 + GBT - 1) / GBT);
   hipLaunchKernelGGL((density_fused_f64<NSPIN, GBT>), dim3(nwg), dim3(256), 0, s, pl.tab(),
                      pl.P_total, pl.nfb, Pd, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, narr, gga, tiles,
-                     pl.pw.p, xl, coef, blockEN, epoint);
+                     pl.pw.p, xl, coef, blockEN, epoint, rhopt);
 }
 
 void vk_density(hipStream_t s, int nspin, int group, int64_t b0, int64_t b1, int64_t s0,
                 const Plan &pl, const double *Pd, int narr, int gga, double *tiles,
-                const XcList &xl, double *coef, double *blockEN, double *epoint) {
+                const XcList &xl, double *coef, double *blockEN, double *epoint,
+                double *rhopt) {
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
   const int g0 = group & 15;  // the int8 form's shape codes add 16
   const int g = g0 == 1 || g0 == 2 ? g0 : 4;
-#define NKX_DENS(NS, G) launch_density<NS, G>(s, b0, b1, s0, pl, Pd, narr, gga, tiles, xl, coef, blockEN, epoint)
+#define NKX_DENS(NS, G) launch_density<NS, G>(s, b0, b1, s0, pl, Pd, narr, gga, tiles, xl, coef, blockEN, epoint, rhopt)
   if (nspin == 2) {
     if (g == 1) NKX_DENS(2, 1);
     else if (g == 2) NKX_DENS(2, 2);

@@ -330,5 +330,8 @@ void overlap_build(Ctx &c, double *S);
 // total one); mixed: int8-slice contractions
 void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool gga,
              int partition, bool mixed, double *E, double *grad);
+// rho, grad rho, sigma of P per grid point (grid order); all but rho may be NULL
+void density_points(Ctx &c, const double *P, double *rho, double *gx, double *gy, double *gz,
+                    double *sigma);
 
 }  // namespace nkx

@@ -1033,6 +1033,37 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
   *E = en[0];
 }
 
+// rho, grad rho and sigma of P at every grid point, in the grid's order: the
+// FP64 batched density with an empty functional list, then a scatter through
+// the point permutation.  Stream-ordered; outputs other than rho may be NULL.
+void density_points(Ctx &c, const double *P, double *rho, double *gx, double *gy, double *gz,
+                    double *sigma) {
+  ensure_plan(c);
+  Plan &pl = c.plan;
+  NKX_REQUIRE(pl.nmol == 1, NKX_ERR_STATE, "the per-point density runs on the ctx grid and basis");
+  Workspace &ws = c.ws;
+  hipStream_t s = c.stream;
+  const int narr = 5;
+  int64_t max_slots = chunk_slot_budget((size_t)narr * FB * PT * sizeof(double), c.opt.mem_budget);
+  for (int64_t b = 0; b < pl.nb; ++b)
+    max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
+  const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
+  ws.tiles64.reserve((size_t)slots_alloc * narr * FB * PT);
+  ws.Pd.reserve((size_t)pl.P_total);
+  ws.coef.reserve((size_t)4 * pl.npts_pad);
+  ws.blockEN.reserve((size_t)3 * pl.nb);
+  ws.gbuf.reserve((size_t)4 * pl.npts_pad);
+  XcList none{};
+  none.n = 0;
+  vk_gather_p(s, pl.nbf, pl.nbf_pad, pl.fperm_d.p, P, ws.Pd.p);
+  for_each_chunk(pl, slots_alloc, [&](int64_t b0, int64_t b1, int64_t s0, int64_t s1) {
+    vk_colloc(s, s0, s1, pl, narr, 1, ws.tiles64.p);
+    vk_density(s, 1, 4, b0, b1, s0, pl, ws.Pd.p, narr, 1, ws.tiles64.p, none, ws.coef.p,
+               ws.blockEN.p, nullptr, ws.gbuf.p);
+  });
+  vk_scatter_density(s, pl, ws.gbuf.p, rho, gx, gy, gz, sigma);
+}
+
 int Timing::begin(int stage, hipStream_t s) {
   if (!on) return -1;
   const size_t k = pending_stage.size();

@@ -10,10 +10,27 @@ void vk_gather_p(hipStream_t s, int nbf, int nbf_pad, const int *fperm, const do
 void vk_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr, int gga,
                double *tiles);
 // group: batches per workgroup (1, 2 or 4); epoint (may be NULL): w e_xc
-// per sorted point
+// per sorted point; rhopt (may be NULL, nspin 1): rho and grad rho per sorted
+// point
 void vk_density(hipStream_t s, int nspin, int group, int64_t b0, int64_t b1, int64_t s0,
                 const Plan &pl, const double *Pd, int narr, int gga, double *tiles,
-                const XcList &xl, double *coef, double *blockEN, double *epoint);
+                const XcList &xl, double *coef, double *blockEN, double *epoint,
+                double *rhopt = nullptr);
+// screened-grid set-up (kernels_sparse.hip): Morton order of the points
+// (AoS; w, owner may be NULL) in place and the box of each run of per points
+// (6 doubles per box); the CSR list of functions (host centres and radii)
+// meeting each box, offsets always, neighbors when not NULL; returns its length
+void sparse_bounding_boxes(hipStream_t s, int64_t npts, double *pts, double *w, int *owner,
+                           int per, double *boxes);
+int64_t sparse_neighbor_list(hipStream_t s, int64_t nbox, const double *boxes, int nbf,
+                             const double *ctr_h, const double *rad_h, int *offsets,
+                             int *neighbors);
+// P = C diag(f) C^T (C column-major nbf x nocc)
+void vk_density_matrix(hipStream_t s, int nbf, int nocc, const double *C, const double *f,
+                       double *P);
+// rho, grad rho per sorted point (4 per padded point) to the grid's order
+void vk_scatter_density(hipStream_t s, const Plan &pl, const double *r4, double *rho, double *gx,
+                        double *gy, double *gz, double *sigma);
 void vk_zoverlap(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int narr,
                  double *tiles);
 // nz Z arrays (the last nz of the narr per slot), partials for Z array k at

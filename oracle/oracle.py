@@ -247,3 +247,63 @@ def overlap(phi, w):
     S = np.zeros((nbf, nbf))
     lib().orc_overlap(nbf, npts, _p(_f64(phi)), _p(_f64(w)), _p(S))
     return S
+
+
+# ---- screened-grid set-up (numpy restatement) ------------------------------
+def _spread21(v):
+    """Interleave the 21 low bits of v with two zero bits after each."""
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    out = np.zeros_like(v)
+    for b in range(21):
+        out |= ((v >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+    return out
+
+
+def morton_codes(pts):
+    """63-bit Morton code of each point (npts x 3): 21 bits per axis of the
+    position in the points' bounding box, x in the highest bit of each
+    triple.  Stands where ArborX's Morton64 curve orders the points in
+    create_bounding_boxes (octree.hpp:88-106); ArborX is absent, so this
+    curve is this build's own (same order within a 2^21 lattice up to the
+    axis convention)."""
+    pts = np.asarray(pts, dtype=np.float64)
+    lo, hi = pts.min(0), pts.max(0)
+    code = np.zeros(len(pts), dtype=np.uint64)
+    for d in range(3):
+        ext = max(hi[d] - lo[d], 1e-12)
+        t = np.clip((pts[:, d] - lo[d]) / ext, 0.0, 1.0)
+        code |= _spread21((t * 2097151.0).astype(np.uint64)) << np.uint64(2 - d)
+    return code
+
+
+def bounding_boxes(pts, per):
+    """create_bounding_boxes (octree.hpp:71-138): stable Morton order of the
+    points (returned as the permutation) and the (nboxes, 6) boxes (min xyz,
+    max xyz) of each run of per consecutive sorted points."""
+    pts = np.asarray(pts, dtype=np.float64)
+    perm = np.argsort(morton_codes(pts), kind="stable")
+    sp = pts[perm]
+    nbox = (len(sp) + per - 1) // per
+    boxes = np.empty((nbox, 6))
+    for b in range(nbox):
+        blk = sp[b * per:(b + 1) * per]
+        boxes[b, :3] = blk.min(0)
+        boxes[b, 3:] = blk.max(0)
+    return perm, boxes
+
+
+def neighbor_list(boxes, centers, radii):
+    """build_neighbor_list (octree.hpp:317-389): for each box the ascending
+    indices of the spheres (centre, radius) whose distance from the centre
+    to the box is <= the radius; CSR (offsets, neighbors)."""
+    boxes = np.asarray(boxes, dtype=np.float64).reshape(-1, 6)
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    r = np.asarray(radii, dtype=np.float64)
+    off, nbr = [0], []
+    for b in boxes:
+        e = np.maximum(np.maximum(b[:3] - c, c - b[3:]), 0.0)
+        d = np.sqrt((e * e).sum(1))
+        hit = np.nonzero(d <= r)[0]
+        nbr.extend(hit.tolist())
+        off.append(len(nbr))
+    return np.array(off, dtype=np.int32), np.array(nbr, dtype=np.int32)

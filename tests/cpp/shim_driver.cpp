@@ -2,7 +2,8 @@
 // tests exercise its headers (test_hydrogen.cxx:815-912, test_helium.cxx:
 // 67-121): grid, basis, collocation, compute_lda / compute_gga, the
 // polarised compute_lsda / compute_gga_lsda, the batched compute_vxc(_polarized),
-// the nuclear gradient (FP64 and mixed) and the error path.  Prints one line
+// the nuclear gradient (FP64 and mixed), the screened (sparse) forms and the
+// error path.  Prints one line
 // per check and
 // exits nonzero on the first failure.
 #include <cmath>
@@ -106,6 +107,46 @@ int main() {
     std::printf("FAIL H2 gradient (sum %.2e, mixed vs FP64 %.2e)\n", g[2] + g[5], worst);
     return 1;
   }
+  // screened forms (octree.hpp:71-389, density.hpp:97-148,
+  // xc_integrals.hpp:728-977) on the H2 grid, Morton-ordered in place: with
+  // P_alpha = P_beta = P2 / 2 the polarised PBE energy is the closed-shell one,
+  // and sum_g w rho = N_el of the batched build
+  auto boxes = create_bounding_boxes(grid2, 64);
+  NeighborList nl;
+  build_neighbor_list(basis2, boxes, 64, (int)grid2.weights.size(), nl);
+  const auto off = nl.offsets.to_host();
+  const auto nbr = nl.neighbors.to_host();
+  const auto bh = boxes.to_host();
+  std::printf("H2: %zu boxes, %d neighbor entries\n", boxes.size(), off.back());
+  // each box against both spheres on the host (the outermost radial shells
+  // lie beyond the 1s cutoff radius)
+  std::vector<int> expect;
+  for (const Box &b : bh)
+    for (int j = 0; j < 2; ++j) {
+      double d2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double c = basis2.O[j][d];
+        const double e = c < b.lo[d] ? b.lo[d] - c : (c > b.hi[d] ? c - b.hi[d] : 0.0);
+        d2 += e * e;
+      }
+      if (std::sqrt(d2) <= basis2.cutoff_radii[j]) expect.push_back(j);
+    }
+  if (off.back() != (int)nbr.size() || nbr != expect) {
+    std::printf("FAIL neighbor list against the host test\n");
+    return 1;
+  }
+  DeviceView2DLeft Cs(2, 1);
+  DeviceView1D f1(1);
+  Cs.from_host(std::vector<double>{std::sqrt(0.3), std::sqrt(0.3)});
+  f1.from_host(std::vector<double>{1.0});
+  auto sp = compute_gga_lsda_sparse(basis2, grid2, nl, Cs, f1, Cs, f1, Functional({101, 130}));
+  auto cs = compute_vxc(grid2, basis2, P2, Functional({101, 130}));
+  expect_rel("H2 sparse polarised PBE", sp.energy, cs.energy, 1e-12);
+  auto rho = compute_density_sparse(basis2, grid2, nl, Cs, f1).to_host();
+  const auto w2 = grid2.weights.to_host();
+  double ne = 0.0;
+  for (size_t g = 0; g < w2.size(); ++g) ne += w2[g] * rho[g];
+  expect_rel("H2 sparse density N_el", 2.0 * ne, cs.n_el, 1e-12);
   std::printf("shim OK\n");
   return 0;
 }
