@@ -3,8 +3,12 @@
 // is handed to the library when a collocation or XC call needs it.
 #pragma once
 
+#include <cctype>
 #include <cmath>
+#include <fstream>
+#include <sstream>
 
+#include "molecule.hpp"
 #include "nukexc_config.hpp"
 
 namespace Nukexc {
@@ -42,6 +46,80 @@ inline STOBasisSet make_manual_basis(const std::vector<STOFunc> &funcs,
     b.O.push_back(p);
   }
   return b;
+}
+
+// S, P, D, F, G, H, I -> 0..6 (any case); anything else -1 (stobasis.hpp:67-73).
+inline int label_to_l(char label) {
+  static const std::string order = "SPDFGHI";
+  const size_t pos = order.find((char)std::toupper((unsigned char)label));
+  return pos == std::string::npos ? -1 : (int)pos;
+}
+
+// ADF basis files (stobasis.hpp:227-289): per atom data_dir/<Symbol>; the rows
+// after the first line holding BASIS (FIT when fit) up to the first line
+// holding END; a row "nL zeta" adds the functions m = -l..l.  Atoms without a
+// file are skipped.
+inline STOBasisSet load_adf_basis(const Molecule &mol,
+                                  const std::string &data_dir = "input/zorabasis/TZP",
+                                  const double cutoff_tol = 1e-10, const bool fit = false) {
+  std::vector<STOFunc> funcs;
+  const std::string key = fit ? "FIT" : "BASIS";
+  for (size_t a = 0; a < mol.natoms; ++a) {
+    std::string sym = detail::symbols().at(mol.Z[a]);
+    sym[0] = (char)std::toupper((unsigned char)sym[0]);
+    std::ifstream file(data_dir + "/" + sym);
+    if (!file.is_open()) continue;
+    std::string line;
+    while (std::getline(file, line))
+      if (line.find(key) != std::string::npos) break;
+    while (std::getline(file, line)) {
+      if (line.find("END") != std::string::npos) break;
+      if (line.empty()) continue;
+      std::stringstream ss(line);
+      std::string label;
+      double zeta;
+      if (!(ss >> label >> zeta)) continue;
+      const int n = std::stoi(label.substr(0, 1)), l = label_to_l(label[1]);
+      const Point &c = mol.atom_centers[a];
+      for (int m = -l; m <= l; ++m) funcs.push_back({n, l, m, zeta, c[0], c[1], c[2]});
+    }
+  }
+  return make_manual_basis(funcs, cutoff_tol);
+}
+
+// Thakkar Roothaan-Hartree-Fock tables (stobasis.hpp:291-371): per atom
+// data_dir/<symbol> (lower case); a line whose first word is a shell letter
+// sets l; lines holding BASIS or CUSP are headers; a row "nL zeta ..." whose
+// label starts with a digit adds the functions m = -l..l.
+inline STOBasisSet load_thakkar_basis(const Molecule &mol,
+                                      const std::string &data_dir = "input/k99light/neutral",
+                                      const double cutoff_tol = 1e-10) {
+  std::vector<STOFunc> funcs;
+  for (size_t a = 0; a < mol.natoms; ++a) {
+    std::string sym = detail::symbols().at(mol.Z[a]);
+    sym[0] = (char)std::tolower((unsigned char)sym[0]);
+    std::ifstream file(data_dir + "/" + sym);
+    if (!file.is_open()) continue;
+    std::string line;
+    int l = 0;
+    while (std::getline(file, line)) {
+      if (line.empty()) continue;
+      std::stringstream first(line);
+      std::string w;
+      first >> w;
+      if (w.size() == 1 && label_to_l(w[0]) != -1) l = label_to_l(w[0]);
+      if (line.find("BASIS") != std::string::npos || line.find("CUSP") != std::string::npos)
+        continue;
+      std::stringstream ss(line);
+      std::string label;
+      double zeta;
+      if (!(ss >> label >> zeta) || !std::isdigit((unsigned char)label[0])) continue;
+      const int n = std::stoi(label.substr(0, 1));
+      const Point &c = mol.atom_centers[a];
+      for (int m = -l; m <= l; ++m) funcs.push_back({n, l, m, zeta, c[0], c[1], c[2]});
+    }
+  }
+  return make_manual_basis(funcs, cutoff_tol);
 }
 
 inline bool same_basis(const STOBasisSet &a, const STOBasisSet &b) {

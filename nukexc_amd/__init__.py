@@ -15,5 +15,6 @@ from .api import (PBE, TA_M3, TA_M4, Context, FlatGrid, STOBasisSet, XC_result, 
 from .api import (NeighborList, build_neighbor_list, compute_density_and_sigma_sparse,  # noqa: F401
                   compute_density_sparse, compute_gga_lsda_sparse, compute_lsda_sparse,
                   create_bounding_boxes, density_matrix)
+from . import readers  # noqa: F401  (read_xyz, load_adf_basis, load_thakkar_basis)
 
 __version__ = "0.1.0"
