@@ -323,6 +323,64 @@ XcOut gga_c_pbe(double rho, double sigma) {
   return {e, vr, vs};
 }
 
+// B88 exchange (GGA_X_B88, id 106; Becke, PRA 38, 3098 (1988), Eq. 8) and
+// LYP correlation (GGA_C_LYP, id 131; Miehlich, Savin, Stoll, Preuss, CPL 157,
+// 200 (1989), Eq. 2), spin-resolved energy densities for a generic number type
+// (real, or complex for complex-step derivatives).  Parity unpinned against
+// libxc (not installed): pinned by Becke's and Lee-Yang-Parr's published atom
+// values and the one-electron zero of LYP.
+template <typename T>
+T b88_spin(T r, T s) {  // one channel: density r, sigma_ss s
+  const double beta = 0.0042;
+  const double cx = -1.5 * std::cbrt(3.0 / (4.0 * kPi));
+  T r43 = std::pow(r, 4.0 / 3.0);
+  T x = std::sqrt(s) / r43;
+  return r43 * (cx - beta * x * x / (1.0 + 6.0 * beta * x * std::asinh(x)));
+}
+
+template <typename T>
+T lyp_e(T ra, T rb, T saa, T sab, T sbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double cf = 0.3 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  T n = ra + rb;
+  T n13 = std::pow(n, 1.0 / 3.0);
+  T g = 1.0 / (1.0 + d / n13);
+  T omega = std::exp(-c / n13) * g / std::pow(n, 11.0 / 3.0);
+  T delta = (c + d * g) / n13;
+  T grad2 = saa + 2.0 * sab + sbb;
+  T t_same = std::pow(2.0, 11.0 / 3.0) * cf * (std::pow(ra, 8.0 / 3.0) + std::pow(rb, 8.0 / 3.0));
+  T bracket = ra * rb * (t_same + (47.0 - 7.0 * delta) / 18.0 * grad2 -
+                         (2.5 - delta / 18.0) * (saa + sbb) -
+                         (delta - 11.0) / 9.0 * (ra * saa + rb * sbb) / n);
+  bracket += (2.0 / 3.0) * n * n * (saa + sbb - grad2) - ra * ra * sbb - rb * rb * saa;
+  return -4.0 * a * g * ra * rb / n - a * b * omega * bracket;
+}
+
+template <typename T>
+T blyp_e(int id, T ra, T rb, T saa, T sab, T sbb, bool a_on, bool b_on) {
+  if (id == 106) {
+    T e = 0.0;
+    if (a_on) e += b88_spin(ra, saa);
+    if (b_on) e += b88_spin(rb, sbb);
+    return e;
+  }
+  return lyp_e(ra, rb, saa, sab, sbb);
+}
+
+// Unpolarised B88 / LYP: the spin-resolved form at rho_s = rho / 2,
+// sigma_ss = sigma_ab = sigma / 4, differentiated by complex step along rho
+// and along sigma.
+XcOut blyp_unpol(int id, double rho, double sigma) {
+  using C = std::complex<double>;
+  const double h = 1e-30;
+  auto ev = [&](C r, C s) { return blyp_e<C>(id, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s,
+                                             true, true); };
+  const C e0 = ev(C(rho, 0.0), C(sigma, 0.0));
+  const C er = ev(C(rho, h), C(sigma, 0.0));
+  const C es = ev(C(rho, 0.0), C(sigma, h));
+  return {e0.real(), er.imag() / h, es.imag() / h};
+}
+
 // libxc thresholds: points with rho below dens_threshold contribute nothing;
 // sigma is floored at sigma_threshold^2 with sigma_threshold =
 // dens_threshold^(4/3).  Default dens_threshold per functional follows libxc's
@@ -347,11 +405,13 @@ bool xc_eval_point(int id, double rho, double sigma, double thr, XcOut *o) {
     case 13: *o = lda_c_pw(rho, true); return true;
     case 101: *o = gga_x_pbe(rho, s); return true;
     case 130: *o = gga_c_pbe(rho, s); return true;
+    case 106:
+    case 131: *o = blyp_unpol(id, rho, s); return true;
     default: return false;
   }
 }
 
-bool is_gga(int id) { return id == 101 || id == 130; }
+bool is_gga(int id) { return id == 101 || id == 130 || id == 106 || id == 131; }
 
 // ---------------------------------------------------------------------------
 // Spin-polarised functionals (libxc XC_POLARIZED, called by compute_lsda /
@@ -457,6 +517,9 @@ T e_polarized(int id, const T *x, double thr) {
     }
     case 130:
       return pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+    case 106:
+    case 131:
+      return blyp_e(id, ra, rb, saa, sab, sbb, std::real(ra) > thr, std::real(rb) > thr);
   }
   return T(0.0);
 }

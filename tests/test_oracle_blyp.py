@@ -1,0 +1,90 @@
+"""The oracle's B88 exchange and LYP correlation (GGA_X_B88 = 106,
+GGA_C_LYP = 131) against published atom values: Becke's exchange energies of
+H (-0.3098) and He (-1.025) (PRA 38, 3098 (1988), Table I), the LYP
+correlation energy of He with its Hartree-Fock density (-0.0437, Lee, Yang
+and Parr, PRB 37, 785 (1988)), LYP's zero for one-electron (fully
+polarised) densities, and finite differences of the potentials.  libxc is not
+installed: parity with it is unpinned beyond these checks."""
+import math
+import os
+
+import numpy as np
+
+from conftest import ROOT
+from oracle import oracle as O
+
+HE = os.path.join(ROOT, "tests", "golden", "readers", "k99light", "he")
+
+
+def _radial():
+    x, wx = np.polynomial.legendre.leggauss(400)
+    t = (x + 1) / 2
+    r = t / (1 - t)
+    return r, 4 * math.pi * r * r * (wx / 2) / (1 - t) ** 2
+
+
+def _he_density(r):
+    """Roothaan-Hartree-Fock 1s of He from the Thakkar table: rho, d rho/dr."""
+    R = np.zeros_like(r)
+    dR = np.zeros_like(r)
+    for line in open(HE):
+        w = line.split()
+        if len(w) == 3 and w[0][0].isdigit():
+            n, z, c = int(w[0][0]), float(w[1]), float(w[2])
+            f = c * (2 * z) ** (n + 0.5) / math.sqrt(math.factorial(2 * n)) * r ** (n - 1) \
+                * np.exp(-z * r)
+            R += f
+            dR += f * ((n - 1) / r - z)
+    return 2 * R * R / (4 * math.pi), 4 * R * dR / (4 * math.pi)
+
+
+def test_hydrogen_b88_and_lyp():
+    r, w = _radial()
+    rho = np.exp(-2 * r) / math.pi
+    rp = np.stack([rho, np.zeros_like(rho)], 1)
+    sp = np.stack([4 * rho * rho, 0 * rho, 0 * rho], 1)
+    ex = float(w @ (rho * O.xc_eval_polarized([106], rp, sp)[0]))
+    ec = float(w @ (rho * O.xc_eval_polarized([131], rp, sp)[0]))
+    print(f"H: E_x(B88) = {ex:.6f}, E_c(LYP) = {ec:.1e}")
+    assert abs(ex - (-0.3098)) < 1e-4
+    assert abs(ec) < 1e-12
+
+
+def test_helium_b88_and_lyp():
+    r, w = _radial()
+    rho, drho = _he_density(r)
+    assert abs(float(w @ rho) - 2.0) < 1e-6
+    ex = float(w @ (rho * O.xc_eval([106], rho, drho * drho)[0]))
+    ec = float(w @ (rho * O.xc_eval([131], rho, drho * drho)[0]))
+    print(f"He: E_x(B88) = {ex:.6f}, E_c(LYP) = {ec:.6f}")
+    assert abs(ex - (-1.025)) < 1e-3
+    assert abs(ec - (-0.0437)) < 2e-4
+
+
+def test_unpolarised_is_polarised_at_zeta_zero():
+    rng = np.random.default_rng(4)
+    rho = 10 ** rng.uniform(-4, 1, 200)
+    sig = rho ** (8 / 3) * 10 ** rng.uniform(-3, 1, 200)
+    for ids in ([106], [131], [106, 131]):
+        e, vr, vs = O.xc_eval(ids, rho, sig)
+        r2 = np.stack([rho / 2, rho / 2], 1)
+        s3 = np.stack([sig / 4, sig / 4, sig / 4], 1)
+        ep, vrp, vsp = O.xc_eval_polarized(ids, r2, s3)
+        assert np.allclose(e, ep, rtol=1e-13, atol=0)
+        assert np.allclose(vr, vrp[:, 0], rtol=1e-12, atol=1e-300)
+        # de/dsigma = (v_aa + v_ab + v_bb) / 4 at sigma_ss = sigma_ab = sigma / 4
+        assert np.allclose(vs, vsp.sum(1) / 4, rtol=1e-11, atol=1e-300)
+
+
+def test_potentials_by_finite_differences():
+    rng = np.random.default_rng(6)
+    rho = 10 ** rng.uniform(-3, 1, 50)
+    sig = rho ** (8 / 3) * 10 ** rng.uniform(-2, 1, 50)
+    for ids in ([106], [131]):
+        e, vr, vs = O.xc_eval(ids, rho, sig)
+        E = lambda r, s: r * O.xc_eval(ids, r, s)[0]  # noqa: E731
+        h = 1e-6
+        fr = (E(rho * (1 + h), sig) - E(rho * (1 - h), sig)) / (2 * h * rho)
+        fs = (E(rho, sig * (1 + h)) - E(rho, sig * (1 - h))) / (2 * h * sig)
+        assert np.allclose(vr, fr, rtol=1e-6, atol=1e-12)
+        assert np.allclose(vs, fs, rtol=1e-5, atol=1e-12)
