@@ -28,7 +28,7 @@ struct XC_result_polarized {
 };
 
 // libxc-numbered functional list (LDA_X = 1, GGA_X_PBE = 101, GGA_C_PBE = 130,
-// LDA_C_PW = 12, LDA_C_PW_MOD = 13, GGA_X_B88 = 106, GGA_C_LYP = 131); family
+// LDA_C_PW = 12, LDA_C_PW_MOD = 13, LDA_C_VWN = 7, GGA_X_B88 = 106, GGA_C_LYP = 131); family
 // as func.info->family.
 struct Functional {
   std::vector<int> ids;

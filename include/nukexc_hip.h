@@ -20,8 +20,8 @@
  *    reference-shaped entry points use the reference's LayoutLeft (N_bf, N_pts)
  *    view: element (mu, g) at mu + g * N_bf.
  *  - Functional ids follow libxc numbering: LDA_X = 1, LDA_C_PW = 12,
- *    LDA_C_PW_MOD = 13, GGA_X_PBE = 101, GGA_C_PBE = 130, GGA_X_B88 = 106,
- *    GGA_C_LYP = 131 (tests/standalone_helpers.hpp:184-225).
+ *    LDA_C_PW_MOD = 13, LDA_C_VWN = 7, GGA_X_PBE = 101, GGA_C_PBE = 130,
+ *    GGA_X_B88 = 106, GGA_C_LYP = 131 (tests/standalone_helpers.hpp:184-225).
  */
 #ifndef NUKEXC_HIP_H
 #define NUKEXC_HIP_H

@@ -25,7 +25,7 @@ except ImportError:  # pragma: no cover
     torch = None
 
 FUNCTIONALS = {"LDA_X": 1, "LDA_C_PW": 12, "LDA_C_PW_MOD": 13, "GGA_X_PBE": 101,
-               "GGA_C_PBE": 130, "GGA_X_B88": 106, "GGA_C_LYP": 131}
+               "GGA_C_PBE": 130, "GGA_X_B88": 106, "GGA_C_LYP": 131, "LDA_C_VWN": 7}
 PBE = (101, 130)
 TA_M4 = 0.6
 TA_M3 = 0.0

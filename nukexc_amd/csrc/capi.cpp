@@ -52,6 +52,7 @@ int functional_info(int id, int *family, int *kind) {
     case 101: *family = NKX_FAMILY_GGA; *kind = 0; return 0;
     case 130: *family = NKX_FAMILY_GGA; *kind = 1; return 0;
     case 106: *family = NKX_FAMILY_GGA; *kind = 0; return 0;
+    case 7: *family = NKX_FAMILY_LDA; *kind = 1; return 0;
     case 131: *family = NKX_FAMILY_GGA; *kind = 1; return 0;
     default: return 1;
   }

@@ -83,6 +83,7 @@ __host__ __device__ inline D5 dpow(D5 a, double p) {  // a > 0
   return dchain(a, v, p * v / a.v);
 }
 __host__ __device__ inline D5 dmax(D5 a, double floor_) { return a.v >= floor_ ? a : dconst(floor_); }
+__host__ __device__ inline D5 datan(D5 a) { return dchain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
 __host__ __device__ inline D5 dasinh(D5 a) {
   return dchain(a, asinh(a.v), 1.0 / sqrt(1.0 + a.v * a.v));
 }

@@ -1,5 +1,6 @@
-// Becke 88 exchange (libxc GGA_X_B88, id 106) and Lee-Yang-Parr correlation
-// (libxc GGA_C_LYP, id 131) as spin-resolved energy densities over D5.
+// Becke 88 exchange (libxc GGA_X_B88, id 106), Lee-Yang-Parr correlation
+// (libxc GGA_C_LYP, id 131) and Vosko-Wilk-Nusair correlation (libxc LDA_C_VWN,
+// id 7) as spin-resolved energy densities over D5.
 //
 // B88 (A. D. Becke, Phys. Rev. A 38, 3098 (1988), Eq. 8), per spin channel:
 //   e_s = rho_s^(4/3) [c_x - beta x_s^2 / (1 + 6 beta x_s asinh x_s)],
@@ -51,6 +52,43 @@ __host__ __device__ inline D5 lyp(D5 ra, D5 rb, D5 saa, D5 sab, D5 sbb) {
   const D5 brace = ra * rb * inner - (2.0 / 3.0) * rr2 * sig +
                    ((2.0 / 3.0) * rr2 - ra * ra) * sbb + ((2.0 / 3.0) * rr2 - rb * rb) * saa;
   return -4.0 * kLypA / den * ra * rb / rr - kLypA * kLypB * w * brace;
+}
+
+// VWN correlation (S. H. Vosko, L. Wilk, M. Nusair, Can. J. Phys. 58, 1200
+// (1980), the Ceperley-Alder fit "V"): with x = sqrt(rs), X(x) = x^2 + b x + c,
+// Q = sqrt(4c - b^2),
+//   G(x) = A { ln(x^2/X) + 2b/Q atan(Q/(2x + b))
+//            - b x0/X(x0) [ln((x - x0)^2/X) + 2(b + 2 x0)/Q atan(Q/(2x + b))] }
+// for the paramagnetic (P), ferromagnetic (F) and spin-stiffness (alpha)
+// parameter sets, and eps = P + alpha f(z)/f''(0) (1 - z^4) + (F - P) f(z) z^4.
+struct VwnSet {
+  double A, x0, b, c;
+};
+__host__ __device__ inline D5 vwn_g(D5 x, const VwnSet &p) {
+  const double Q = sqrt(4.0 * p.c - p.b * p.b);
+  const double X0 = p.x0 * p.x0 + p.b * p.x0 + p.c;
+  const D5 X = x * x + p.b * x + p.c;
+  const D5 at = datan(Q / (2.0 * x + p.b));
+  return p.A * (dlog(x * x / X) + (2.0 * p.b / Q) * at -
+                (p.b * p.x0 / X0) * (dlog((x - p.x0) * (x - p.x0) / X) +
+                                     (2.0 * (p.b + 2.0 * p.x0) / Q) * at));
+}
+
+// energy density rho eps_c; 1 +- zeta passed as 2 rho_s / rho, floored
+__host__ __device__ inline D5 vwn(D5 ra, D5 rb, double zthr) {
+  const VwnSet P{0.0310907, -0.10498, 3.72744, 12.9352};
+  const VwnSet F{0.01554535, -0.32500, 7.06042, 18.0578};
+  const VwnSet Al{-1.0 / (6.0 * 3.14159265358979323846 * 3.14159265358979323846), -0.0047584,
+                  1.13107, 13.0045};
+  const double fpp = 1.709920934161365617563962776245;  // f''(0) = 4 / (9 (2^(1/3) - 1))
+  const D5 rr = ra + rb;
+  const D5 x = dsqrt(0.62035049089940001667 / dpow(rr, 1.0 / 3.0));  // sqrt(rs)
+  const D5 opz = dmax(2.0 * ra / rr, zthr), omz = dmax(2.0 * rb / rr, zthr);
+  const D5 z = (ra - rb) / rr;
+  const D5 f = (dpow(opz, 4.0 / 3.0) + dpow(omz, 4.0 / 3.0) - 2.0) / 0.5198420997897464;
+  const D5 z4 = z * z * z * z;
+  const D5 eP = vwn_g(x, P), eF = vwn_g(x, F), eA = vwn_g(x, Al);
+  return rr * (eP + eA * f * (1.0 - z4) / fpp + (eF - eP) * f * z4);
 }
 
 }  // namespace xcb

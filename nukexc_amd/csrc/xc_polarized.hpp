@@ -142,6 +142,9 @@ __host__ __device__ inline void xc_eval_polarized(int n, const int *ids, const d
       case 131:
         r = xcb::lyp(ra, rb, saa, sab, sbb);
         break;
+      case 7:
+        r = xcb::vwn(ra, rb, xcp::kZetaThr);
+        break;
       case 130: {
         D5 rr = ra + rb;
         D5 opz = dmax(2.0 * ra / rr, xcp::kZetaThr), omz = dmax(2.0 * rb / rr, xcp::kZetaThr);

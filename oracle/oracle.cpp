@@ -356,8 +356,43 @@ T lyp_e(T ra, T rb, T saa, T sab, T sbb) {
   return -4.0 * a * g * ra * rb / n - a * b * omega * bracket;
 }
 
+// VWN correlation (LDA_C_VWN, id 7; Vosko, Wilk, Nusair, Can. J. Phys. 58,
+// 1200 (1980), fit V) in rs: per parameter set (A, x0, b, c),
+// G = A [ln(rs/X) + 2b/Q atan(Q/(2 sqrt rs + b)) - b x0/X0 (ln((sqrt rs - x0)^2/X)
+// + 2(b + 2 x0)/Q atan(Q/(2 sqrt rs + b)))], X = rs + b sqrt rs + c; spin
+// interpolation through f(z) and the stiffness set.  Pinned against PW92
+// (a separate fit of the same Ceperley-Alder data) and the z = 1 limit.
+template <typename T>
+T vwn_set(T rs, double A, double x0, double b, double c) {
+  const double Q = std::sqrt(4.0 * c - b * b), X0 = x0 * x0 + b * x0 + c;
+  T sx = std::sqrt(rs);
+  T X = rs + b * sx + c;
+  T at = std::atan(Q / (2.0 * sx + b));
+  return A * (std::log(rs / X) + 2.0 * b / Q * at -
+              b * x0 / X0 * (std::log((sx - x0) * (sx - x0) / X) + 2.0 * (b + 2.0 * x0) / Q * at));
+}
+
+template <typename T>
+T vwn_e(T ra, T rb) {
+  T n = ra + rb;
+  T rs = std::pow(3.0 / (4.0 * kPi * n), 1.0 / 3.0);
+  T opz = T(2.0) * ra / n, omz = T(2.0) * rb / n;
+  if (std::real(opz) < 2.220446049250313e-16) opz = 2.220446049250313e-16;
+  if (std::real(omz) < 2.220446049250313e-16) omz = 2.220446049250313e-16;
+  T z = (ra - rb) / n;
+  T f = (std::pow(opz, 4.0 / 3.0) + std::pow(omz, 4.0 / 3.0) - 2.0) /
+        (2.0 * std::cbrt(2.0) - 2.0);
+  const double fpp = 4.0 / (9.0 * (std::cbrt(2.0) - 1.0));
+  T eP = vwn_set(rs, 0.0310907, -0.10498, 3.72744, 12.9352);
+  T eF = vwn_set(rs, 0.01554535, -0.32500, 7.06042, 18.0578);
+  T eA = vwn_set(rs, -1.0 / (6.0 * kPi * kPi), -0.0047584, 1.13107, 13.0045);
+  T z4 = z * z * z * z;
+  return n * (eP + eA * f / fpp * (1.0 - z4) + (eF - eP) * f * z4);
+}
+
 template <typename T>
 T blyp_e(int id, T ra, T rb, T saa, T sab, T sbb, bool a_on, bool b_on) {
+  if (id == 7) return vwn_e(ra, rb);
   if (id == 106) {
     T e = 0.0;
     if (a_on) e += b88_spin(ra, saa);
@@ -405,6 +440,7 @@ bool xc_eval_point(int id, double rho, double sigma, double thr, XcOut *o) {
     case 13: *o = lda_c_pw(rho, true); return true;
     case 101: *o = gga_x_pbe(rho, s); return true;
     case 130: *o = gga_c_pbe(rho, s); return true;
+    case 7:
     case 106:
     case 131: *o = blyp_unpol(id, rho, s); return true;
     default: return false;
@@ -517,6 +553,7 @@ T e_polarized(int id, const T *x, double thr) {
     }
     case 130:
       return pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+    case 7:
     case 106:
     case 131:
       return blyp_e(id, ra, rb, saa, sab, sbb, std::real(ra) > thr, std::real(rb) > thr);

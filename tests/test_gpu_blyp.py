@@ -1,5 +1,5 @@
-"""B88 exchange and LYP correlation (ids 106, 131) on the GPU against the
-oracle: pointwise (unpolarised and polarised, including one-spin points),
+"""B88 exchange, LYP and VWN correlation (ids 106, 131, 7) on the GPU
+against the oracle: pointwise (unpolarised and polarised, including one-spin points),
 in the batched V_xc build (FP64 and int8 mixed), and the helium atom built
 end to end from the Thakkar table (reader, grid, build) against the radial
 integrals the oracle tests pin to Becke's and Lee-Yang-Parr's values."""
@@ -27,7 +27,7 @@ def dev(a):
     return torch.tensor(a, device="cuda")
 
 
-@pytest.mark.parametrize("ids", [[106], [131], [106, 131]])
+@pytest.mark.parametrize("ids", [[106], [131], [106, 131], [7], [1, 7]])
 def test_functional_eval_matches_oracle(ctx, ids):
     rng = np.random.default_rng(11)
     n = 4000
@@ -52,7 +52,8 @@ def test_functional_eval_matches_oracle(ctx, ids):
     print(f"{ids} polarised: e {(np.abs(e1 - e0) / sc).max():.1e}")
     assert (np.abs(e1 - e0) / sc).max() < 1e-11
     assert np.allclose(v1, v0, rtol=1e-10, atol=1e-14 * np.abs(v0).max())
-    assert np.allclose(s1, s0, rtol=1e-9, atol=1e-12 * np.abs(s0).max())
+    if any(i > 100 for i in ids):  # an LDA-only list leaves vsigma unwritten
+        assert np.allclose(s1, s0, rtol=1e-9, atol=1e-12 * np.abs(s0).max())
 
 
 def test_blyp_build_water_matches_oracle(ctx):

@@ -88,3 +88,25 @@ def test_potentials_by_finite_differences():
         fs = (E(rho, sig * (1 + h)) - E(rho, sig * (1 - h))) / (2 * h * sig)
         assert np.allclose(vr, fr, rtol=1e-6, atol=1e-12)
         assert np.allclose(vs, fs, rtol=1e-5, atol=1e-12)
+
+
+def test_vwn_against_pw92():
+    """VWN (id 7) and PW92 (id 12) are two fits of the same Ceperley-Alder
+    correlation energies: per particle they agree to 5e-4 Eh for
+    0.5 <= rs <= 20 at zeta = 0, 0.6 and 1, the gap largest at small rs where
+    both fits extrapolate (a wrong constant or term of either breaks this by
+    far more)."""
+    rs = np.geomspace(0.5, 20.0, 200)
+    rho = 3.0 / (4.0 * math.pi * rs ** 3)
+    for z in (0.0, 0.6, 1.0):
+        r2 = np.stack([rho * (1 + z) / 2, rho * (1 - z) / 2], 1)
+        d = np.abs(O.xc_eval_polarized([7], r2, None)[0] - O.xc_eval_polarized([12], r2, None)[0])
+        print(f"zeta {z}: max |eps_VWN - eps_PW92| = {d.max():.1e}")
+        assert d.max() < 5e-4
+    e7, v7, _ = O.xc_eval([7], rho, None)
+    assert np.allclose(e7, O.xc_eval_polarized([7], np.stack([rho / 2, rho / 2], 1), None)[0],
+                       rtol=1e-13)
+    h = 1e-6  # potential by finite differences
+    fd = (rho * (1 + h) * O.xc_eval([7], rho * (1 + h), None)[0]
+          - rho * (1 - h) * O.xc_eval([7], rho * (1 - h), None)[0]) / (2 * h * rho)
+    assert np.allclose(v7, fd, rtol=1e-6)
