@@ -239,8 +239,16 @@ void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nr
   nkx::lebedev_lookup(nang_order, &nang, &actual);
   std::vector<double> ang(4 * (size_t)nang);
   nkx::lebedev_expand(actual, ang.data());
-  std::vector<double> P, W;
-  std::vector<int> O;
+  // radial rules per atom on the host (small); the product points on the device
+  std::vector<double> rr, rw;
+  std::vector<int> roff(1, 0);
+  std::vector<int64_t> poff(1, 0);
+  int max_nr = 0;
+  double wang_min = 0.0, wang_max = 0.0;
+  for (int j = 0; j < nang; ++j) {
+    wang_min = std::min(wang_min, ang[4 * j + 3]);
+    wang_max = std::max(wang_max, ang[4 * j + 3]);
+  }
   for (int a = 0; a < natoms; ++a) {
     int nr = nrad;
     if (radial_sizing == NKX_SIZING_PYSCF) {
@@ -250,25 +258,31 @@ void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nr
     std::vector<double> r(nr), wr(nr);
     nkx::radial_rule(radial_scheme, nr, nkx::radial_scale(radial_scheme, Z[a]), ta_alpha,
                      r.data(), wr.data());
-    for (int i = 0; i < nr; ++i)
-      for (int j = 0; j < nang; ++j) {
-        P.push_back(xyz[3 * a] + r[i] * ang[4 * j]);
-        P.push_back(xyz[3 * a + 1] + r[i] * ang[4 * j + 1]);
-        P.push_back(xyz[3 * a + 2] + r[i] * ang[4 * j + 2]);
-        const double ww = wr[i] * ang[4 * j + 3];
-        NKX_REQUIRE(ww >= 0, NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
-        W.push_back(ww);
-        O.push_back(a);
-      }
+    for (int i = 0; i < nr; ++i)  // some product w_r w_ang < 0
+      NKX_REQUIRE(!((wr[i] < 0 && wang_max > 0) || (wr[i] > 0 && wang_min < 0)),
+                  NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
+    rr.insert(rr.end(), r.begin(), r.end());
+    rw.insert(rw.end(), wr.begin(), wr.end());
+    roff.push_back(roff.back() + nr);
+    poff.push_back(poff.back() + (int64_t)nr * nang);
+    max_nr = std::max(max_nr, nr);
   }
-  const int64_t n = (int64_t)W.size();
+  const int64_t n = poff.back();
   hipStream_t s = c.stream;
-  nkx::DevBuf<double> p0, w0;
-  nkx::DevBuf<int> o0;
-  nkx::DevBuf<int64_t> cnt;
-  p0.upload(P.data(), P.size(), s);
-  w0.upload(W.data(), W.size(), s);
-  o0.upload(O.data(), O.size(), s);
+  nkx::DevBuf<double> p0, w0, xr, xw, xa, xc;
+  nkx::DevBuf<int> o0, xro;
+  nkx::DevBuf<int64_t> cnt, xpo;
+  xr.upload(rr.data(), rr.size(), s);
+  xw.upload(rw.data(), rw.size(), s);
+  xa.upload(ang.data(), ang.size(), s);
+  xc.upload(xyz, 3 * (size_t)natoms, s);
+  xro.upload(roff.data(), roff.size(), s);
+  xpo.upload(poff.data(), poff.size(), s);
+  p0.reserve(3 * n);
+  w0.reserve(n);
+  o0.reserve(n);
+  nkx::launch_grid_points(s, natoms, nang, max_nr, xpo.p, xro.p, xc.p, xr.p, xw.p, xa.p, p0.p,
+                          w0.p, o0.p);
   out.centers_host.assign(xyz, xyz + 3 * natoms);
   out.centers.upload(xyz, 3 * (size_t)natoms, s);
   out.natoms = natoms;
