@@ -17,7 +17,7 @@ struct TreutlerAhlrichs {};
 struct Becke {};
 struct LebedevLaikov {};
 
-enum class PruningScheme { Unpruned };  // pruned grids: not built yet
+enum class PruningScheme { Unpruned, Treutler, Robust };
 enum class RadialSizing { Uniform, PySCF };
 
 constexpr double TA_M4 = 0.6;
@@ -50,7 +50,6 @@ FlatGrid make_flat_grid(const Molecule &mol, const size_t nrad = 50,
   constexpr int scheme = std::is_same_v<radial_type, TreutlerAhlrichs>
                              ? NKX_RADIAL_TREUTLER_AHLRICHS
                              : NKX_RADIAL_BECKE;
-  (void)pruning;
   nkx_ctx *ctx = context().get();
   std::vector<double> xyz;
   std::vector<int> Z;
@@ -59,11 +58,11 @@ FlatGrid make_flat_grid(const Molecule &mol, const size_t nrad = 50,
     Z.push_back((int)mol.Z[i]);
   }
   int64_t npts = 0;
-  check(nkx_make_flat_grid(ctx, (int)mol.natoms, xyz.data(), Z.data(), (int)nrad,
-                           (int)nang_order, scheme, ta_alpha, weight_threshold,
-                           radial_sizing == RadialSizing::PySCF ? NKX_SIZING_PYSCF
-                                                                : NKX_SIZING_UNIFORM,
-                           &npts),
+  check(nkx_make_flat_grid_pruned(ctx, (int)mol.natoms, xyz.data(), Z.data(), (int)nrad,
+                                  (int)nang_order, scheme, ta_alpha, weight_threshold,
+                                  radial_sizing == RadialSizing::PySCF ? NKX_SIZING_PYSCF
+                                                                       : NKX_SIZING_UNIFORM,
+                                  (int)pruning, &npts),
         ctx);
   FlatGrid g;
   g.quad_points.allocate(npts);

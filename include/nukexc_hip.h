@@ -109,6 +109,14 @@ int nkx_radial_scale(int scheme, int Z, double *R);
    ctx: atom-major, radial, angular order; Becke-partitioned with the capped
    partition (partition_becke_team); points with w <= w_thr removed by an
    order-preserving compaction.  Returns the number of points. */
+/* nkx_make_flat_grid with angular pruning (grid.hpp:164-166, 256-265):
+   pruning 0 none, 1 Treutler (radial shells counted from the nucleus: below
+   nrad/3 + 1 Lebedev order 7, below nrad/2 + 1 order 11, the rest the full
+   order), 2 Robust (below nrad/4 + 1 order 7, below nrad/2 + 1 the full
+   order - 6).  The point counts match the reference's pruning study. */
+int nkx_make_flat_grid_pruned(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z,
+                              int nrad, int nang_order, int radial_scheme, double ta_alpha,
+                              double w_thr, int radial_sizing, int pruning, int64_t *npts_out);
 int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz_host,
                        const int *Z_host, int nrad, int nang_order, int radial_scheme,
                        double ta_alpha, double weight_threshold, int radial_sizing,

@@ -48,6 +48,8 @@ SIGNATURES = {
     "nkx_radial_scale": [_c_int, _c_int, _pd],
     "nkx_make_flat_grid": [_vp, _c_int, _pd, _pi, _c_int, _c_int, _c_int, _c_double, _c_double,
                            _c_int, _pi64],
+    "nkx_make_flat_grid_pruned": [_vp, _c_int, _pd, _pi, _c_int, _c_int, _c_int, _c_double,
+                                  _c_double, _c_int, _c_int, _pi64],
     "nkx_set_grid": [_vp, _c_int, _vp, _c_i64, _vp, _vp, _vp],
     "nkx_grid_npts": [_vp, _pi64],
     "nkx_grid_export": [_vp, _vp, _vp, _vp],

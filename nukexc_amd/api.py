@@ -192,17 +192,18 @@ class Context:
 
     # -- grid -----------------------------------------------------------------
     def make_flat_grid(self, xyz, Z, nrad=50, nang_order=30, weight_threshold=1e-30,
-                       ta_alpha=TA_M4, radial="ta", sizing="uniform"):
-        """make_flat_grid<TA|Becke, LebedevLaikov> (grid.hpp:172-348)."""
+                       ta_alpha=TA_M4, radial="ta", sizing="uniform", pruning="none"):
+        """make_flat_grid<TA|Becke, LebedevLaikov> (grid.hpp:172-348); pruning
+        "none", "treutler" or "robust" (grid.hpp:164-166)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         Zc = np.ascontiguousarray(Z, dtype=np.int32)
         n = ctypes.c_int64()
         scheme = {"ta": _capi.RADIAL_TA, "becke": _capi.RADIAL_BECKE}[radial]
-        self._chk(lib().nkx_make_flat_grid(
+        self._chk(lib().nkx_make_flat_grid_pruned(
             self.h, len(Zc), xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             Zc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(nrad), int(nang_order), scheme,
             float(ta_alpha), float(weight_threshold), {"uniform": 0, "pyscf": 1}[sizing],
-            ctypes.byref(n)))
+            {"none": 0, "treutler": 1, "robust": 2}[pruning], ctypes.byref(n)))
         npts = n.value
         pts = torch.empty((npts, 3), dtype=torch.float64, device=self.device)
         w = torch.empty(npts, dtype=torch.float64, device=self.device)

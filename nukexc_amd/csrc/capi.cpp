@@ -230,25 +230,42 @@ namespace {
 // atom, capped Becke partition, compaction of w > w_thr.
 void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nrad,
                 int nang_order, int radial_scheme, double ta_alpha, double w_thr,
-                int radial_sizing, nkx::GridState &out) {
+                int radial_sizing, nkx::GridState &out, int pruning = 0) {
   NKX_REQUIRE(natoms > 0 && xyz && Z && nrad > 0, NKX_ERR_INVALID_ARGUMENT,
               "make_flat_grid: need atoms, coordinates, charges and nrad > 0");
   NKX_REQUIRE(radial_sizing == 0 || radial_sizing == 1, NKX_ERR_INVALID_ARGUMENT,
               "unknown radial sizing");
-  int nang = 0, actual = 0;
-  nkx::lebedev_lookup(nang_order, &nang, &actual);
-  std::vector<double> ang(4 * (size_t)nang);
-  nkx::lebedev_expand(actual, ang.data());
-  // radial rules per atom on the host (small); the product points on the device
-  std::vector<double> rr, rw;
-  std::vector<int> roff(1, 0);
-  std::vector<int64_t> poff(1, 0);
-  int max_nr = 0;
-  double wang_min = 0.0, wang_max = 0.0;
-  for (int j = 0; j < nang; ++j) {
-    wang_min = std::min(wang_min, ang[4 * j + 3]);
-    wang_max = std::max(wang_max, ang[4 * j + 3]);
+  NKX_REQUIRE(pruning >= 0 && pruning <= 2, NKX_ERR_INVALID_ARGUMENT,
+              "pruning: 0 none, 1 Treutler, 2 Robust");
+  // angular rules: 0 full order; pruned grids (IntegratorXX's Treutler and
+  // Robust schemes, grid.hpp:164-166, 256-265) also 1 low (order 7) and 2
+  // medium (order 11 for Treutler, the full order - 6 for Robust)
+  const int orders[3] = {nang_order, std::min(7, nang_order),
+                         pruning == 1 ? std::min(11, nang_order) : std::max(nang_order - 6, 1)};
+  const int nrules = pruning ? 3 : 1;
+  std::vector<double> ang;
+  std::vector<int> rule_off, rule_n;
+  double wang_min[3], wang_max[3];
+  for (int k = 0; k < nrules; ++k) {
+    int nk = 0, actual = 0;
+    nkx::lebedev_lookup(orders[k], &nk, &actual);
+    rule_off.push_back((int)(ang.size() / 4));
+    rule_n.push_back(nk);
+    ang.resize(ang.size() + 4 * (size_t)nk);
+    double *u = ang.data() + 4 * (size_t)rule_off[k];
+    nkx::lebedev_expand(actual, u);
+    wang_min[k] = wang_max[k] = 0.0;
+    for (int j = 0; j < nk; ++j) {
+      wang_min[k] = std::min(wang_min[k], u[4 * j + 3]);
+      wang_max[k] = std::max(wang_max[k], u[4 * j + 3]);
+    }
   }
+  // radial shells on the host (small); the product points on the device.
+  // Counting shells from the nucleus: shells below nr/3 + 1 (Treutler) or
+  // nr/4 + 1 (Robust) take rule 1, shells below nr/2 + 1 rule 2.
+  std::vector<int> sh_atom, sh_rule;
+  std::vector<double> sh_r, sh_w;
+  std::vector<int64_t> poff(1, 0);
   for (int a = 0; a < natoms; ++a) {
     int nr = nrad;
     if (radial_sizing == NKX_SIZING_PYSCF) {
@@ -258,31 +275,39 @@ void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nr
     std::vector<double> r(nr), wr(nr);
     nkx::radial_rule(radial_scheme, nr, nkx::radial_scale(radial_scheme, Z[a]), ta_alpha,
                      r.data(), wr.data());
-    for (int i = 0; i < nr; ++i)  // some product w_r w_ang < 0
-      NKX_REQUIRE(!((wr[i] < 0 && wang_max > 0) || (wr[i] > 0 && wang_min < 0)),
+    const int b1 = (pruning == 1 ? nr / 3 : nr / 4) + 1, b2 = nr / 2 + 1;
+    for (int i = 0; i < nr; ++i) {
+      const int rank = r[0] < r[nr - 1] ? i : nr - 1 - i;
+      const int k = pruning == 0 ? 0 : (rank < b1 ? 1 : (rank < b2 ? 2 : 0));
+      NKX_REQUIRE(!((wr[i] < 0 && wang_max[k] > 0) || (wr[i] > 0 && wang_min[k] < 0)),
                   NKX_ERR_NEGATIVE_WEIGHT, "Detected negative weights aborting");
-    rr.insert(rr.end(), r.begin(), r.end());
-    rw.insert(rw.end(), wr.begin(), wr.end());
-    roff.push_back(roff.back() + nr);
-    poff.push_back(poff.back() + (int64_t)nr * nang);
-    max_nr = std::max(max_nr, nr);
+      sh_atom.push_back(a);
+      sh_rule.push_back(k);
+      sh_r.push_back(r[i]);
+      sh_w.push_back(wr[i]);
+      poff.push_back(poff.back() + rule_n[k]);
+    }
   }
   const int64_t n = poff.back();
+  const int nshell = (int)sh_atom.size();
   hipStream_t s = c.stream;
   nkx::DevBuf<double> p0, w0, xr, xw, xa, xc;
-  nkx::DevBuf<int> o0, xro;
+  nkx::DevBuf<int> o0, xat, xru, xro, xrn;
   nkx::DevBuf<int64_t> cnt, xpo;
-  xr.upload(rr.data(), rr.size(), s);
-  xw.upload(rw.data(), rw.size(), s);
+  xr.upload(sh_r.data(), sh_r.size(), s);
+  xw.upload(sh_w.data(), sh_w.size(), s);
+  xat.upload(sh_atom.data(), sh_atom.size(), s);
+  xru.upload(sh_rule.data(), sh_rule.size(), s);
+  xpo.upload(poff.data(), poff.size(), s);
+  xro.upload(rule_off.data(), rule_off.size(), s);
+  xrn.upload(rule_n.data(), rule_n.size(), s);
   xa.upload(ang.data(), ang.size(), s);
   xc.upload(xyz, 3 * (size_t)natoms, s);
-  xro.upload(roff.data(), roff.size(), s);
-  xpo.upload(poff.data(), poff.size(), s);
   p0.reserve(3 * n);
   w0.reserve(n);
   o0.reserve(n);
-  nkx::launch_grid_points(s, natoms, nang, max_nr, xpo.p, xro.p, xc.p, xr.p, xw.p, xa.p, p0.p,
-                          w0.p, o0.p);
+  nkx::launch_grid_shells(s, nshell, xat.p, xr.p, xw.p, xru.p, xpo.p, xro.p, xrn.p, xc.p, xa.p,
+                          p0.p, w0.p, o0.p);
   out.centers_host.assign(xyz, xyz + 3 * natoms);
   out.centers.upload(xyz, 3 * (size_t)natoms, s);
   out.natoms = natoms;
@@ -323,15 +348,22 @@ nkx::BasisHost make_basis(int nbf, const int *n, const int *l, const int *m, con
 }  // namespace
 }  // extern "C++"
 
-int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z, int nrad,
-                       int nang_order, int radial_scheme, double ta_alpha, double w_thr,
-                       int radial_sizing, int64_t *npts_out) {
+int nkx_make_flat_grid_pruned(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z,
+                              int nrad, int nang_order, int radial_scheme, double ta_alpha,
+                              double w_thr, int radial_sizing, int pruning, int64_t *npts_out) {
   NKX_CTX_CHECK(ctx);
   return guarded(ctx, [&] {
     build_grid(ctx->c, natoms, xyz, Z, nrad, nang_order, radial_scheme, ta_alpha, w_thr,
-               radial_sizing, ctx->c.grid);
+               radial_sizing, ctx->c.grid, pruning);
     if (npts_out) *npts_out = ctx->c.grid.npts;
   });
+}
+
+int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z, int nrad,
+                       int nang_order, int radial_scheme, double ta_alpha, double w_thr,
+                       int radial_sizing, int64_t *npts_out) {
+  return nkx_make_flat_grid_pruned(ctx, natoms, xyz, Z, nrad, nang_order, radial_scheme,
+                                   ta_alpha, w_thr, radial_sizing, 0, npts_out);
 }
 
 int nkx_molset_create(nkx_ctx *ctx, int nmol, const int *natoms, const double *xyz,

@@ -60,35 +60,36 @@ __global__ void __launch_bounds__(256) becke_kernel(int natoms, const double *__
   w[g] *= pown / norm;
 }
 
-// Atom-centred product grid on the device: atom a (blockIdx.y) owns points
-// poff[a] .. poff[a] + nr_a nang, point (i, j) = centre + r_i u_j with weight
-// wr_i w_j (radial nodes of atom a at rad[roff[a] ..]).  Rounded products and
-// sums, no contraction: the points are bit-identical to a host loop.
-__global__ void __launch_bounds__(256) grid_points_kernel(
-    int nang, const int64_t *__restrict__ poff, const int *__restrict__ roff,
-    const double *__restrict__ xyz, const double *__restrict__ rad_r,
-    const double *__restrict__ rad_w, const double *__restrict__ ang, double *__restrict__ P,
-    double *__restrict__ W, int *__restrict__ O) {
-  const int a = blockIdx.y;
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int nr = roff[a + 1] - roff[a];
-  if (k >= (int64_t)nr * nang) return;
-  const int i = (int)(k / nang), j = (int)(k % nang);
-  const double r = rad_r[roff[a] + i];
-  const int64_t g = poff[a] + k;
-  for (int d = 0; d < 3; ++d) P[3 * g + d] = __dadd_rn(xyz[3 * a + d], __dmul_rn(r, ang[4 * j + d]));
-  W[g] = __dmul_rn(rad_w[roff[a] + i], ang[4 * j + 3]);
-  O[g] = a;
+// Atom-centred product grid on the device, one workgroup per radial shell:
+// shell q of atom sh_atom[q] at radius sh_r[q] (radial weight sh_w[q]) takes
+// angular rule sh_rule[q] (points ang[4 (rule_off[k] + j) ..]) and writes
+// its points from sh_poff[q]: centre + r u_j with weight w_r w_j.  Rounded
+// products and sums, no contraction: bit-identical to a host loop.
+__global__ void __launch_bounds__(256) grid_shells_kernel(
+    const int *__restrict__ sh_atom, const double *__restrict__ sh_r,
+    const double *__restrict__ sh_w, const int *__restrict__ sh_rule,
+    const int64_t *__restrict__ sh_poff, const int *__restrict__ rule_off,
+    const int *__restrict__ rule_n, const double *__restrict__ xyz,
+    const double *__restrict__ ang, double *__restrict__ P, double *__restrict__ W,
+    int *__restrict__ O) {
+  const int q = blockIdx.x, a = sh_atom[q], k = sh_rule[q];
+  const double r = sh_r[q], wr = sh_w[q];
+  const double *u = ang + 4 * (int64_t)rule_off[k];
+  for (int j = threadIdx.x; j < rule_n[k]; j += 256) {
+    const int64_t g = sh_poff[q] + j;
+    for (int d = 0; d < 3; ++d) P[3 * g + d] = __dadd_rn(xyz[3 * a + d], __dmul_rn(r, u[4 * j + d]));
+    W[g] = __dmul_rn(wr, u[4 * j + 3]);
+    O[g] = a;
+  }
 }
 
-void launch_grid_points(hipStream_t s, int natoms, int nang, int max_nr, const int64_t *poff,
-                        const int *roff, const double *xyz, const double *rad_r,
-                        const double *rad_w, const double *ang, double *P, double *W, int *O) {
-  const int64_t per_atom = (int64_t)max_nr * nang;
-  if (natoms <= 0 || per_atom <= 0) return;
-  NKX_REQUIRE(natoms <= 65535, NKX_ERR_UNSUPPORTED, "grid generation: at most 65535 atoms");
-  hipLaunchKernelGGL(grid_points_kernel, dim3((unsigned)((per_atom + 255) / 256), natoms),
-                     dim3(256), 0, s, nang, poff, roff, xyz, rad_r, rad_w, ang, P, W, O);
+void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const double *sh_r,
+                        const double *sh_w, const int *sh_rule, const int64_t *sh_poff,
+                        const int *rule_off, const int *rule_n, const double *xyz,
+                        const double *ang, double *P, double *W, int *O) {
+  if (nshell <= 0) return;
+  hipLaunchKernelGGL(grid_shells_kernel, dim3((unsigned)nshell), dim3(256), 0, s, sh_atom, sh_r,
+                     sh_w, sh_rule, sh_poff, rule_off, rule_n, xyz, ang, P, W, O);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

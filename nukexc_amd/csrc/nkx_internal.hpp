@@ -261,10 +261,11 @@ double radial_scale(int scheme, int Z);
 int periodic_row(int Z);
 
 // ---- kernels (launchers) -------------------------------------------------
-// per-atom radial x angular product points (kernels_basic.hip)
-void launch_grid_points(hipStream_t s, int natoms, int nang, int max_nr, const int64_t *poff,
-                        const int *roff, const double *xyz, const double *rad_r,
-                        const double *rad_w, const double *ang, double *P, double *W, int *O);
+// radial shell x angular rule points, one workgroup per shell (kernels_basic.hip)
+void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const double *sh_r,
+                        const double *sh_w, const int *sh_rule, const int64_t *sh_poff,
+                        const int *rule_off, const int *rule_n, const double *xyz,
+                        const double *ang, double *P, double *W, int *O);
 void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
                   const double *pts, const int *owner, double *w, int capped);
 int64_t compact_points(hipStream_t s, int64_t n, double thr, const double *pts_in,

@@ -642,10 +642,23 @@ void orc_becke(int natoms, const double *centers, long npts, const double *pts,
 // caller (ang: nang rows of x, y, z, w with sum w = 4 pi).  Outputs must hold
 // natoms * nrad_max * nang entries.  Returns the number of points kept
 // (w > w_thr), in generation order, or -1 on a negative weight.
-long orc_make_flat_grid(int natoms, const double *xyz, const int *Z, int nrad,
-                        int nang, const double *ang, int scheme, double alpha,
-                        double w_thr, int capped, int pyscf_sizing,
-                        double *pts_out, double *w_out, int *owner_out) {
+// Angular pruning (grid.hpp:164-166, 256-265; IntegratorXX's Treutler and
+// Robust schemes): counting radial shells from the nucleus outwards, shells
+// 0 .. nr/3 (Treutler) or 0 .. nr/4 (Robust) use the low rule, shells up to
+// nr/2 the medium rule, the rest the full rule.  Pinned by the reference's
+// pruned point counts (latex/Data/convergence_pruning.csv).
+int prune_rule(int pruning, int k, int nr) {  // k: shell rank from the nucleus
+  if (pruning == 0) return 0;
+  const int b1 = (pruning == 1 ? nr / 3 : nr / 4) + 1, b2 = nr / 2 + 1;
+  return k < b1 ? 1 : (k < b2 ? 2 : 0);
+}
+
+// rules: angs[q] with nangs[q] points for q = 0 full, 1 low, 2 medium.
+long orc_make_flat_grid_pruned(int natoms, const double *xyz, const int *Z, int nrad,
+                               const int *nangs, const double *const *angs, int pruning,
+                               int scheme, double alpha, double w_thr, int capped,
+                               int pyscf_sizing, double *pts_out, double *w_out,
+                               int *owner_out) {
   std::vector<double> P, W;
   std::vector<int> O;
   for (int a = 0; a < natoms; ++a) {
@@ -656,7 +669,10 @@ long orc_make_flat_grid(int natoms, const double *xyz, const int *Z, int nrad,
     }
     std::vector<double> r(nr), wr(nr);
     orc_radial(scheme, nr, orc_radial_scale(scheme, Z[a]), alpha, r.data(), wr.data());
-    for (int i = 0; i < nr; ++i)
+    for (int i = 0; i < nr; ++i) {
+      const int q = prune_rule(pruning, r[0] < r[nr - 1] ? i : nr - 1 - i, nr);
+      const int nang = nangs[q];
+      const double *ang = angs[q];
       for (int j = 0; j < nang; ++j) {
         P.push_back(xyz[3 * a + 0] + r[i] * ang[4 * j + 0]);
         P.push_back(xyz[3 * a + 1] + r[i] * ang[4 * j + 1]);
@@ -666,6 +682,7 @@ long orc_make_flat_grid(int natoms, const double *xyz, const int *Z, int nrad,
         W.push_back(ww);
         O.push_back(a);
       }
+    }
   }
   const long n = (long)W.size();
   orc_becke(natoms, xyz, n, P.data(), O.data(), W.data(), capped);
@@ -681,6 +698,16 @@ long orc_make_flat_grid(int natoms, const double *xyz, const int *Z, int nrad,
     }
   }
   return k;
+}
+
+long orc_make_flat_grid(int natoms, const double *xyz, const int *Z, int nrad,
+                        int nang, const double *ang, int scheme, double alpha,
+                        double w_thr, int capped, int pyscf_sizing,
+                        double *pts_out, double *w_out, int *owner_out) {
+  const int nangs[3] = {nang, nang, nang};
+  const double *angs[3] = {ang, ang, ang};
+  return orc_make_flat_grid_pruned(natoms, xyz, Z, nrad, nangs, angs, 0, scheme, alpha, w_thr,
+                                   capped, pyscf_sizing, pts_out, w_out, owner_out);
 }
 
 // STO normalisation N = (2 zeta)^{n+1/2} / sqrt((2n)!)  (stobasis.hpp:200-201)

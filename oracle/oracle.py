@@ -36,6 +36,10 @@ def lib():
                                          ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, _d, _d, _i]
         L.orc_make_flat_grid.restype = ctypes.c_long
+        L.orc_make_flat_grid_pruned.argtypes = [
+            ctypes.c_int, _d, _i, ctypes.c_int, _i, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+            ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _d, _d, _i]
+        L.orc_make_flat_grid_pruned.restype = ctypes.c_long
         L.orc_sto_norm_cutoff.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _d, _d]
         L.orc_solid_harmonic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _d]
@@ -98,18 +102,22 @@ def becke(centers, pts, owner, w, capped=True):
 
 
 def make_flat_grid(xyz, Z, nrad, ang, scheme=0, alpha=0.6, w_thr=1e-30, capped=True,
-                   pyscf_sizing=False):
-    """ang: (nang, 4) array of x, y, z, weight (sum 4 pi)."""
+                   pyscf_sizing=False, pruning=0, ang_low=None, ang_med=None):
+    """ang: (nang, 4) array of x, y, z, weight (sum 4 pi).  pruning 1
+    (Treutler) or 2 (Robust) with the low and medium rules ang_low, ang_med."""
     xyz = _f64(xyz)
     Z = _i32(Z)
-    ang = _f64(ang)
-    nmax = len(Z) * int(nrad * (2.1 if pyscf_sizing else 1) + 1) * len(ang)
+    rules = [_f64(a) for a in (ang, ang if ang_low is None else ang_low,
+                               ang if ang_med is None else ang_med)]
+    nmax = len(Z) * int(nrad * (2.1 if pyscf_sizing else 1) + 1) * len(rules[0])
     P = np.empty((nmax, 3))
     W = np.empty(nmax)
     O = np.empty(nmax, dtype=np.int32)
-    n = lib().orc_make_flat_grid(len(Z), _p(xyz), _p(Z, _i), nrad, len(ang), _p(ang), scheme,
-                                 alpha, w_thr, int(capped), int(pyscf_sizing), _p(P), _p(W),
-                                 _p(O, _i))
+    nangs = (ctypes.c_int * 3)(*[len(a) for a in rules])
+    angs = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in rules])
+    n = lib().orc_make_flat_grid_pruned(len(Z), _p(xyz), _p(Z, _i), nrad, nangs, angs,
+                                        int(pruning), scheme, alpha, w_thr, int(capped),
+                                        int(pyscf_sizing), _p(P), _p(W), _p(O, _i))
     if n < 0:
         raise RuntimeError("Detected negative weights aborting")
     return P[:n].copy(), W[:n].copy(), O[:n].copy()
