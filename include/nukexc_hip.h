@@ -117,6 +117,14 @@ int nkx_radial_scale(int scheme, int Z, double *R);
 int nkx_make_flat_grid_pruned(nkx_ctx *ctx, int natoms, const double *xyz, const int *Z,
                               int nrad, int nang_order, int radial_scheme, double ta_alpha,
                               double w_thr, int radial_sizing, int pruning, int64_t *npts_out);
+/* Lebedev orders a grid of this nang_order uses (host; no GPU needed):
+   orders3[0] the full rule, next_algebraic_order(nang_order)
+   (grid.hpp:186-187); for pruning 1 (Treutler) / 2 (Robust) orders3[1] the
+   inner-shell rule (order 7) and orders3[2] the middle-shell rule (11 for
+   Treutler, for Robust the full rule's order - 6, as
+   convergence_studies/convergence_pruning.cxx:130-134 states), each the
+   smallest tabulated order at or above it. */
+int nkx_pruning_orders(int nang_order, int pruning, int *orders3);
 int nkx_make_flat_grid(nkx_ctx *ctx, int natoms, const double *xyz_host,
                        const int *Z_host, int nrad, int nang_order, int radial_scheme,
                        double ta_alpha, double weight_threshold, int radial_sizing,
@@ -378,7 +386,9 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
 int nkx_set_memory_budget(nkx_ctx *ctx, int64_t bytes);
 int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
-   and the point-batch size (64 or 128; 0 keeps the current value). */
+   and the point-batch size: 64 is the only size (the kernels' tiles are 64
+   points x 64 functions); 0 keeps it, any other value is
+   NKX_ERR_UNSUPPORTED. */
 int nkx_set_screening(nkx_ctx *ctx, double tol, int batch_points);
 /* Statistics of the current plan: number of points, padded points, batches,
    screened (function-block, batch) pairs kept, total pairs. */

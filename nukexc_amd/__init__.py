@@ -11,6 +11,7 @@ from .api import (PBE, TA_M3, TA_M4, Context, FlatGrid, STOBasisSet, XC_result, 
                   fill_grad_collocation, functional_eval, functional_eval_polarized,
                   functional_info,
                   lebedev_rule, make_manual_basis, partition_becke, partition_becke_team,
+                  pruning_orders,
                   radial_rule, radial_scale, solid_harmonic, sto_norm_cutoff)
 from .api import (NeighborList, build_neighbor_list, compute_density_and_sigma_sparse,  # noqa: F401
                   compute_density_sparse, compute_gga_lsda_sparse, compute_lsda_sparse,

@@ -44,6 +44,7 @@ SIGNATURES = {
     "nkx_copy_d2d": [_vp, _vp, _vp, _c_size],
     "nkx_lebedev_npts": [_c_int, _pi, _pi],
     "nkx_lebedev_rule": [_c_int, _pd],
+    "nkx_pruning_orders": [_c_int, _c_int, _pi],
     "nkx_radial_rule": [_c_int, _c_int, _c_double, _c_double, _pd, _pd],
     "nkx_radial_scale": [_c_int, _c_int, _pd],
     "nkx_make_flat_grid": [_vp, _c_int, _pd, _pi, _c_int, _c_int, _c_int, _c_double, _c_double,

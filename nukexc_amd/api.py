@@ -121,6 +121,15 @@ def lebedev_rule(order):
     return out
 
 
+def pruning_orders(nang_order, pruning="none"):
+    """Lebedev orders (full, inner shells, middle shells) that a grid of this
+    nang_order uses under the pruning scheme ("none", "treutler", "robust")."""
+    pr = {"none": 0, "unpruned": 0, "treutler": 1, "robust": 2}[str(pruning).lower()]
+    out = (ctypes.c_int * 3)()
+    check(lib().nkx_pruning_orders(int(nang_order), pr, out))
+    return tuple(out)
+
+
 def radial_rule(scheme, n, R, alpha=TA_M4):
     r, w = np.empty(n), np.empty(n)
     pd = ctypes.POINTER(ctypes.c_double)
@@ -190,6 +199,34 @@ class Context:
     def sync(self):
         self._chk(lib().nkx_sync(self.h))
 
+    # -- argument checks: the C entry points take device pointers without
+    # sizes, so shapes, dtypes and devices are checked here before any call
+    def _nbf(self):
+        n = ctypes.c_int()
+        self._chk(lib().nkx_basis_nbf(self.h, ctypes.byref(n)))
+        return n.value
+
+    def _in_matrix(self, X, nbf, name):
+        """A density matrix argument as a contiguous FP64 tensor on the ctx
+        device, which must be nbf x nbf."""
+        X = _dev_f64(X, self.device).contiguous()
+        if tuple(X.shape) != (nbf, nbf):
+            raise NukexcError(1, f"{name} must be {nbf} x {nbf} (the basis size), "
+                                 f"got {tuple(X.shape)}")
+        return X
+
+    def _check_out(self, V, nbf, name):
+        """A caller-supplied output matrix: nbf x nbf, float64, contiguous, on
+        the ctx device (it is written through its device pointer)."""
+        if not isinstance(V, torch.Tensor):
+            raise NukexcError(1, f"{name} must be a torch tensor on {self.device}")
+        if tuple(V.shape) != (nbf, nbf) or V.dtype != torch.float64:
+            raise NukexcError(1, f"{name} must be a float64 {nbf} x {nbf} tensor, got "
+                                 f"{V.dtype} {tuple(V.shape)}")
+        if V.device != self.device or not V.is_contiguous():
+            raise NukexcError(1, f"{name} must be contiguous on {self.device}")
+        return V
+
     # -- grid -----------------------------------------------------------------
     def make_flat_grid(self, xyz, Z, nrad=50, nang_order=30, weight_threshold=1e-30,
                        ta_alpha=TA_M4, radial="ta", sizing="uniform", pruning="none"):
@@ -257,10 +294,10 @@ class Context:
         Returns XC_result(energy, potential, n_el)."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
-        P = _dev_f64(P, self.device).contiguous()
-        nbf = P.shape[0]
-        V = out if out is not None else torch.empty((nbf, nbf), dtype=torch.float64,
-                                                    device=self.device)
+        nbf = self._nbf()
+        P = self._in_matrix(P, nbf, "P")
+        V = (self._check_out(out, nbf, "out") if out is not None else
+             torch.empty((nbf, nbf), dtype=torch.float64, device=self.device))
         ids, nf = _ids(funcs)
         E, N = ctypes.c_double(), ctypes.c_double()
         self._chk(lib().nkx_vxc(self.h, _ptr(P), nf, ids, prec, ctypes.byref(E), ctypes.byref(N),
@@ -273,9 +310,8 @@ class Context:
         V_beta, (N_alpha, N_beta))."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
-        Pa = _dev_f64(Pa, self.device).contiguous()
-        Pb = _dev_f64(Pb, self.device).contiguous()
-        nbf = Pa.shape[0]
+        nbf = self._nbf()
+        Pa, Pb = self._in_matrix(Pa, nbf, "Pa"), self._in_matrix(Pb, nbf, "Pb")
         Va, Vb = (torch.empty((nbf, nbf), dtype=torch.float64, device=self.device)
                   for _ in range(2))
         ids, nf = _ids(funcs)
@@ -293,7 +329,7 @@ class Context:
         in the mixed form (int8-slice contractions, precision="mixed").
         Returns (E_xc, grad ndarray (natoms, 3))."""
         fn = {"fp64": "nkx_xc_grad", "mixed": "nkx_xc_grad_mixed"}[precision]
-        P = _dev_f64(P, self.device).contiguous()
+        P = self._in_matrix(P, self._nbf(), "P")
         natoms = self.grid.atom_centers.shape[0]
         g = np.zeros(3 * natoms)
         ids, nf = _ids(funcs)
@@ -308,8 +344,8 @@ class Context:
         spin density matrices, in FP64 or the mixed form.  Returns (E_xc,
         grad ndarray (natoms, 3))."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
-        Pa = _dev_f64(Pa, self.device).contiguous()
-        Pb = _dev_f64(Pb, self.device).contiguous()
+        nbf = self._nbf()
+        Pa, Pb = self._in_matrix(Pa, nbf, "Pa"), self._in_matrix(Pb, nbf, "Pb")
         natoms = self.grid.atom_centers.shape[0]
         g = np.zeros(3 * natoms)
         ids, nf = _ids(funcs)
@@ -324,7 +360,7 @@ class Context:
         count for this workload; returns (split, shape, ms per build)."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
-        P = _dev_f64(P, self.device).contiguous()
+        P = self._in_matrix(P, self._nbf(), "P")
         V = torch.empty_like(P)
         ids, nf = _ids(funcs)
         k, g, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
@@ -355,7 +391,9 @@ class Context:
         replay with graph_launch() after changing P in place."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
                 "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
-        assert P.is_contiguous() and V.is_contiguous() and P.dtype == torch.float64
+        nbf = self._nbf()
+        self._check_out(P, nbf, "P")
+        self._check_out(V, nbf, "V")
         ids, nf = _ids(funcs)
         self._graph_keep = (P, V)
         self._chk(lib().nkx_graph_capture(self.h, _ptr(P), nf, ids, prec, _ptr(V)))
@@ -366,8 +404,9 @@ class Context:
         """Capture one open-shell build for the device tensors Pa, Pb (inputs)
         and Va, Vb (outputs); graph_launch() then returns (E_xc, (N_a, N_b))."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
-        for t in (Pa, Pb, Va, Vb):
-            assert t.is_contiguous() and t.dtype == torch.float64
+        nbf = self._nbf()
+        for t, name in ((Pa, "Pa"), (Pb, "Pb"), (Va, "Va"), (Vb, "Vb")):
+            self._check_out(t, nbf, name)
         ids, nf = _ids(funcs)
         self._graph_keep = (Pa, Pb, Va, Vb)
         self._chk(lib().nkx_graph_capture_polarized(self.h, _ptr(Pa), _ptr(Pb), nf, ids, prec,
@@ -421,8 +460,9 @@ class Context:
         matrices.  Returns a list of XC_result."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
         nmol = len(self._molset_nbf)
-        assert len(Ps) == nmol
-        Pd = [_dev_f64(P, self.device).contiguous() for P in Ps]
+        if len(Ps) != nmol:
+            raise NukexcError(1, f"need {nmol} density matrices, got {len(Ps)}")
+        Pd = [self._in_matrix(P, k, f"P[{i}]") for i, (P, k) in enumerate(zip(Ps, self._molset_nbf))]
         Vs = [torch.empty((k, k), dtype=torch.float64, device=self.device)
               for k in self._molset_nbf]
         Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Pd])
@@ -439,8 +479,9 @@ class Context:
         (split, shape, ms per set build)."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
         nmol = len(self._molset_nbf)
-        assert len(Ps) == nmol
-        Pd = [_dev_f64(P, self.device).contiguous() for P in Ps]
+        if len(Ps) != nmol:
+            raise NukexcError(1, f"need {nmol} density matrices, got {len(Ps)}")
+        Pd = [self._in_matrix(P, k, f"P[{i}]") for i, (P, k) in enumerate(zip(Ps, self._molset_nbf))]
         Vs = [torch.empty((k, k), dtype=torch.float64, device=self.device)
               for k in self._molset_nbf]
         Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Pd])
@@ -456,9 +497,11 @@ class Context:
         (inputs) and Vs (outputs); replay with molset_graph_launch()."""
         prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED}[precision]
         nmol = len(self._molset_nbf)
-        assert len(Ps) == nmol and len(Vs) == nmol
-        for P, V in zip(Ps, Vs):
-            assert P.is_contiguous() and V.is_contiguous() and P.dtype == torch.float64
+        if len(Ps) != nmol or len(Vs) != nmol:
+            raise NukexcError(1, f"need {nmol} density and potential matrices")
+        for i, (P, V, k) in enumerate(zip(Ps, Vs, self._molset_nbf)):
+            self._check_out(P, k, f"P[{i}]")
+            self._check_out(V, k, f"V[{i}]")
         self._graph_keep = (list(Ps), list(Vs))
         Pp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Ps])
         Vp = (ctypes.c_void_p * nmol)(*[x.data_ptr() for x in Vs])

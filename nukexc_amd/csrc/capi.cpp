@@ -226,6 +226,26 @@ int nkx_radial_scale(int scheme, int Z, double *R) {
 
 extern "C++" {
 namespace {
+// Angular rules of a grid: 0 the full order; pruned grids (IntegratorXX's
+// Treutler and Robust schemes, grid.hpp:164-166, 256-265) also 1 low (order
+// 7) and 2 medium (order 11 for Treutler, the tabulated full order - 6 for
+// Robust).  The base is the order actually used,
+// next_algebraic_order(nang_order), as
+// convergence_studies/convergence_pruning.cxx:130-134 describes.  Each entry
+// is then looked up as the smallest tabulated order >= it.
+void pruned_rule_orders(int nang_order, int pruning, int orders[3]) {
+  int base = 0;
+  nkx::lebedev_lookup(nang_order, nullptr, &base);
+  const int lo = std::min(7, base);
+  const int me = pruning == 1 ? std::min(11, base) : std::max(base - 6, 1);
+  int a = 0;
+  nkx::lebedev_lookup(lo, nullptr, &a);
+  orders[0] = base;
+  orders[1] = a;
+  nkx::lebedev_lookup(me, nullptr, &a);
+  orders[2] = a;
+}
+
 // make_flat_grid (grid.hpp:172-348) into out: TA / Becke radial x Lebedev per
 // atom, capped Becke partition, compaction of w > w_thr.
 void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nrad,
@@ -237,11 +257,8 @@ void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nr
               "unknown radial sizing");
   NKX_REQUIRE(pruning >= 0 && pruning <= 2, NKX_ERR_INVALID_ARGUMENT,
               "pruning: 0 none, 1 Treutler, 2 Robust");
-  // angular rules: 0 full order; pruned grids (IntegratorXX's Treutler and
-  // Robust schemes, grid.hpp:164-166, 256-265) also 1 low (order 7) and 2
-  // medium (order 11 for Treutler, the full order - 6 for Robust)
-  const int orders[3] = {nang_order, std::min(7, nang_order),
-                         pruning == 1 ? std::min(11, nang_order) : std::max(nang_order - 6, 1)};
+  int orders[3];
+  pruned_rule_orders(nang_order, pruning, orders);
   const int nrules = pruning ? 3 : 1;
   std::vector<double> ang;
   std::vector<int> rule_off, rule_n;
@@ -356,6 +373,15 @@ int nkx_make_flat_grid_pruned(nkx_ctx *ctx, int natoms, const double *xyz, const
     build_grid(ctx->c, natoms, xyz, Z, nrad, nang_order, radial_scheme, ta_alpha, w_thr,
                radial_sizing, ctx->c.grid, pruning);
     if (npts_out) *npts_out = ctx->c.grid.npts;
+  });
+}
+
+int nkx_pruning_orders(int nang_order, int pruning, int *orders3) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(orders3, NKX_ERR_INVALID_ARGUMENT, "orders3 is NULL");
+    NKX_REQUIRE(pruning >= 0 && pruning <= 2, NKX_ERR_INVALID_ARGUMENT,
+                "pruning: 0 none, 1 Treutler, 2 Robust");
+    pruned_rule_orders(nang_order, pruning, orders3);
   });
 }
 

@@ -54,73 +54,6 @@ extern "C" int mfma_probe_bf16(const unsigned short *a_h, const unsigned short *
   return rc;
 }
 
-// Accumulation rounding of v_mfma_f32_16x16x32_bf16: every lane's C is c_h[r];
-// A has a single nonzero bf16, A[0][0] = a, and B a single nonzero, B[0][0] =
-// b, so D[0][0] = C[0][0] + a * b and every other D element = C.
-__global__ void probe_round(const float *c, __bf16 a, __bf16 b, float *d) {
-  const int l = threadIdx.x;
-  v8bf av, bv;
-  for (int i = 0; i < 8; ++i) {
-    av[i] = (__bf16)0.0f;
-    bv[i] = (__bf16)0.0f;
-  }
-  if (l == 0) {  // A[0][0] = a (lane 0 holds row 0, k = 0..7), B[0][0] = b
-    av[0] = a;
-    bv[0] = b;
-  }
-  v4f acc = {c[l * 4 + 0], c[l * 4 + 1], c[l * 4 + 2], c[l * 4 + 3]};
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
-  for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
-}
-
-extern "C" int mfma_probe_round(const float *c_h, float a_f, float b_f, float *d_h) {
-  float *c, *d;
-  if (hipMalloc(&c, 256 * 4) || hipMalloc(&d, 256 * 4)) return 1;
-  (void)hipMemcpy(c, c_h, 256 * 4, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(probe_round, dim3(1), dim3(64), 0, 0, c, (__bf16)a_f, (__bf16)b_f, d);
-  int rc = hipDeviceSynchronize() != hipSuccess;
-  (void)hipMemcpy(d_h, d, 256 * 4, hipMemcpyDeviceToHost);
-  (void)hipFree(c); (void)hipFree(d);
-  return rc;
-}
-
-// n independent v_mfma_f32_16x16x32_bf16 instances, one wavefront each:
-// a, b: 512 bf16 per instance (lane-major, 8 per lane as the kernels load
-// them), c, d: 256 floats per instance (lane * 4 + r).  Used to measure the
-// rounding of the 32-term dot products against exact FP64 sums.
-__global__ void probe_bf16_batch(const __bf16 *a, const __bf16 *b, const float *c, float *d) {
-  const int l = threadIdx.x;
-  const long i = blockIdx.x;
-  v8bf av, bv;
-  for (int e = 0; e < 8; ++e) {
-    av[e] = a[i * 512 + l * 8 + e];
-    bv[e] = b[i * 512 + l * 8 + e];
-  }
-  v4f acc = {c[i * 256 + l * 4 + 0], c[i * 256 + l * 4 + 1], c[i * 256 + l * 4 + 2],
-             c[i * 256 + l * 4 + 3]};
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
-  for (int r = 0; r < 4; ++r) d[i * 256 + l * 4 + r] = acc[r];
-}
-
-extern "C" int mfma_probe_bf16_batch(int n, const unsigned short *a_h, const unsigned short *b_h,
-                                     const float *c_h, float *d_h) {
-  if (n <= 0 || n > 65536) return 2;
-  __bf16 *a, *b;
-  float *c, *d;
-  const size_t na = (size_t)n * 512, nc = (size_t)n * 256;
-  if (hipMalloc(&a, na * 2) || hipMalloc(&b, na * 2) || hipMalloc(&c, nc * 4) ||
-      hipMalloc(&d, nc * 4))
-    return 1;
-  (void)hipMemcpy(a, a_h, na * 2, hipMemcpyHostToDevice);
-  (void)hipMemcpy(b, b_h, na * 2, hipMemcpyHostToDevice);
-  (void)hipMemcpy(c, c_h, nc * 4, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(probe_bf16_batch, dim3(n), dim3(64), 0, 0, a, b, c, d);
-  int rc = hipDeviceSynchronize() != hipSuccess;
-  (void)hipMemcpy(d_h, d, nc * 4, hipMemcpyDeviceToHost);
-  (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); (void)hipFree(d);
-  return rc;
-}
-
 // v_mfma_i32_16x16x64_i8: 16 int8 per lane for A and B (lane-major, as
 // loaded by a kernel), d[lane * 4 + r] = result register r (int32).
 typedef int v4i __attribute__((ext_vector_type(4)));

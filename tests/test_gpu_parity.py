@@ -404,89 +404,6 @@ def test_vxc_mixed_density_shapes(ctx):
         assert np.abs(V - V2).max() <= 1e-12
 
 
-def test_mfma_bf16_accumulate_rounding():
-    """Records how v_mfma_f32_16x16x32_bf16 rounds C + a*b into FP32 (the
-    mixed form bounds FP32 accumulation to 64-element blocks either way)."""
-    probe = ctypes.CDLL(os.path.join(ROOT, "tests", "gpu_probe", "libmfma_probe.so"))
-    probe.mfma_probe_round.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_float,
-                                       ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
-    ulp = 2.0 ** -23
-    c = np.ones(256, np.float32)
-    res = {}
-    for frac in (0.25, 0.5, 0.75, -0.25, -0.75):
-        d = np.zeros(256, np.float32)
-        # a * b = frac ulp(1) exactly: a = frac * 2^-23 (a power of two times
-        # 0.25/0.5/0.75, exact in bf16), b = 1
-        assert probe.mfma_probe_round(c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                      frac * ulp, 1.0,
-                                      d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
-        got = float(d[0 * 4 + 0])  # lane 0, register 0 = D[0][0]
-        res[frac] = (got - 1.0) / ulp
-        assert np.all(d[4:] == 1.0)  # untouched elements
-    print("MFMA bf16 accumulation, (result - 1)/ulp for a*b = frac ulp:", res)
-    # round to nearest even: 0.25 -> 0, 0.75 -> 1, -0.25 -> 0, -0.75 -> -0.5 (ulp below 1 is half)
-    assert res[0.25] in (0.0,) and res[0.75] in (1.0,)
-
-
-def _bf16_bits(x):
-    """float32 -> bf16 bits, round to nearest even."""
-    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
-    return (u >> 16).astype(np.uint16)
-
-
-def _bits_bf16(h):
-    return (h.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-
-
-@pytest.mark.parametrize("with_c", [False, True])
-def test_mfma_bf16_dot_rounding_bias(with_c):
-    """Rounding of v_mfma_f32_16x16x32_bf16 on many random 32-term dot
-    products (values spread over 2^-8 .. 1 with random signs), against exact
-    FP64 sums: the mean signed error in units of ulp(result), oriented along
-    the sign of the result (round-to-nearest of the exact sum gives ~0), and
-    the largest error relative to the largest term |a_k b_k| or |c|.  The
-    measured behaviour (errors set by the largest term, a mean pulled toward
-    zero) is what the mixed form's error budget has to absorb."""
-    probe = ctypes.CDLL(os.path.join(ROOT, "tests", "gpu_probe", "libmfma_probe.so"))
-    u16 = ctypes.POINTER(ctypes.c_uint16)
-    f32 = ctypes.POINTER(ctypes.c_float)
-    probe.mfma_probe_bf16_batch.argtypes = [ctypes.c_int, u16, u16, f32, f32]
-    rng = np.random.default_rng(1)
-    n = 2000
-    mag = lambda shape: 2.0 ** rng.uniform(-8, 0, shape) * rng.choice([-1.0, 1.0], shape)
-    a = _bf16_bits(mag((n, 512)))
-    b = _bf16_bits(mag((n, 512)))
-    c = (mag((n, 256)) * 4.0).astype(np.float32) if with_c else np.zeros((n, 256), np.float32)
-    d = np.zeros((n, 256), np.float32)
-    assert probe.mfma_probe_bf16_batch(n, a.ctypes.data_as(u16), b.ctypes.data_as(u16),
-                                       c.ctypes.data_as(f32), d.ctypes.data_as(f32)) == 0
-    # exact D = A B + C in FP64 (bf16 products are exact, 33-term sums to 1e-16)
-    lane = np.arange(64)
-    A = np.zeros((n, 16, 32))
-    B = np.zeros((n, 32, 16))
-    av, bv = _bits_bf16(a).reshape(n, 64, 8), _bits_bf16(b).reshape(n, 64, 8)
-    for e in range(8):
-        A[:, lane % 16, 8 * (lane // 16) + e] = av[:, :, e]
-        B[:, 8 * (lane // 16) + e, lane % 16] = bv[:, :, e]
-    D = A @ B
-    Cm = np.zeros((n, 16, 16))
-    Dg = np.zeros((n, 16, 16))
-    cv, dv = c.reshape(n, 64, 4).astype(np.float64), d.reshape(n, 64, 4).astype(np.float64)
-    for r in range(4):
-        Cm[:, 4 * (lane // 16) + r, lane % 16] = cv[:, :, r]
-        Dg[:, 4 * (lane // 16) + r, lane % 16] = dv[:, :, r]
-    D += Cm
-    ulp = np.spacing(np.abs(D).astype(np.float32)).astype(np.float64)
-    err = (Dg - D) / ulp * np.sign(D)
-    tmax = np.maximum(np.abs(A[:, :, :, None] * B[:, None, :, :]).max(2), np.abs(Cm))
-    rel = np.abs(Dg - D) / tmax
-    print(f"MFMA bf16 dot (C {'random' if with_c else 'zero'}): mean signed error "
-          f"{err.mean():+.4f} ulp(result), mean |err| {np.abs(err).mean():.4f} ulp(result), "
-          f"max |err| / max term {rel.max():.2e} (2^{np.log2(rel.max()):.1f})")
-    assert rel.max() < 2.0 ** -20
-
-
 def test_mfma_i8_lane_layout():
     """v_mfma_i32_16x16x64_i8: lane l feeds A[l%16][16(l/16) + e] and
     B[16(l/16) + e][l%16] (e = 0..15, byte e of the lane's 16 bytes); result

@@ -33,9 +33,7 @@ def test_pruned_grid_matches_study_and_oracle(ctx, row):
     assert g.weights.shape[0] == npts
     # the library's Lebedev tables (same rules as the oracle's, in the
     # library's point order), so the grids compare point by point
-    lo = min(7, order)
-    me = min(11, order) if scheme == "Treutler" else max(order - 6, 1)
-    full, low, med = (nk.lebedev_rule(o) for o in (order, lo, me))
+    full, low, med = (nk.lebedev_rule(o) for o in nk.pruning_orders(order, scheme))
     pr = {"Unpruned": 0, "Treutler": 1, "Robust": 2}[scheme]
     P0, W0, O0 = O.make_flat_grid(xyz, Z, nrad, full, pruning=pr, ang_low=low, ang_med=med)
     # same points in the same order (the oracle's host compiler may contract
@@ -44,3 +42,27 @@ def test_pruned_grid_matches_study_and_oracle(ctx, row):
     assert np.array_equal(g.point_owner.cpu().numpy(), O0)
     # Becke products in another order on the device: weights to rounding
     assert np.allclose(g.weights.cpu().numpy(), W0, rtol=1e-10, atol=1e-14 * W0.max())
+
+
+@pytest.mark.parametrize("order,npts_med", [(32, 302), (36, 434)])
+def test_robust_middle_shells_use_base_order(ctx, order, npts_med):
+    """nang_order 32 / 36 (tabulated bases 35 / 41): Robust's middle shells
+    take order 29 / 35 rules; the grid equals the oracle's built from those
+    rules, point for point."""
+    xyz, Z = read_xyz(os.path.join(ROOT, "tests", "golden", "readers", "water.xyz"))
+    g = ctx.make_flat_grid(xyz, Z, nrad=30, nang_order=order, pruning="robust")
+    full, low, med = (nk.lebedev_rule(o) for o in nk.pruning_orders(order, "robust"))
+    assert len(med) == npts_med
+    P0, W0, O0 = O.make_flat_grid(xyz, Z, 30, full, pruning=2, ang_low=low, ang_med=med)
+    assert g.weights.shape[0] == len(W0)
+    assert np.array_equal(g.point_owner.cpu().numpy(), O0)
+
+
+def test_robust_negative_weight_order_raises(ctx):
+    """nang_order 30 (base 31): Robust's middle rule is order 25, which has
+    negative weights; the build stops with NKX_ERR_NEGATIVE_WEIGHT as the
+    reference throws "Detected negative weights aborting" (grid.hpp:284-286)."""
+    xyz, Z = read_xyz(os.path.join(ROOT, "tests", "golden", "readers", "water.xyz"))
+    with pytest.raises(nk.NukexcError) as e:
+        ctx.make_flat_grid(xyz, Z, nrad=30, nang_order=30, pruning="robust")
+    assert e.value.code == 6 and "negative weights" in str(e.value)

@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ROOT, lebedev
+from conftest import ROOT, lebedev, next_order
 from nukexc_amd.readers import read_xyz
 from oracle import oracle as O
 
@@ -29,9 +29,13 @@ def _leb(order):
 
 
 def _rules(scheme, order):
-    low = min(7, order)
-    med = min(11, order) if scheme == "Treutler" else max(order - 6, 1)
-    return _leb(order), _leb(low), _leb(med)
+    """IntegratorXX's pruned rules: the base is the tabulated order the full
+    rule uses, and Robust's middle shells take base - 6
+    (convergence_studies/convergence_pruning.cxx:130-134)."""
+    base = next_order(order)
+    low = min(7, base)
+    med = min(11, base) if scheme == "Treutler" else max(base - 6, 1)
+    return _leb(base), _leb(low), _leb(med)
 
 
 def _count(xyz, Z, scheme, nrad, order):
@@ -60,3 +64,25 @@ def test_pruned_point_counts(row):
     assert n - n_unp == npts - ref_unp
     if nrad <= 60:
         assert n == npts and n_unp == ref_unp
+
+
+@pytest.mark.parametrize("order,scheme,want", [
+    (17, "robust", (17, 7, 11)), (29, "robust", (29, 7, 23)), (30, "robust", (31, 7, 25)),
+    (32, "robust", (35, 7, 29)), (33, "robust", (35, 7, 29)), (36, "robust", (41, 7, 35)),
+    (37, "robust", (41, 7, 35)), (18, "robust", (19, 7, 13)), (32, "treutler", (35, 7, 11)),
+    (5, "treutler", (5, 5, 5)), (32, "none", (35, 7, 29))])
+def test_library_pruning_orders(order, scheme, want):
+    """The library's rule orders (host call, no GPU): Robust's middle shells
+    use the tabulated base order - 6, so nang_order 32 and 33 (base 35) give
+    order 29 (302 points), 36 and 37 (base 41) order 35 (434 points), and 30
+    (base 31) the negative-weight order 25 that the reference's study avoids
+    (convergence_studies/convergence_pruning.cxx:130-134)."""
+    import nukexc_amd as nk
+    got = nk.pruning_orders(order, scheme)
+    if scheme == "none":
+        assert got[0] == want[0]
+    else:
+        assert got == want
+    assert got == tuple(next_order(o) for o in got)
+    assert {o: len(_leb(o)) for o in got}.get(29, 302) == 302
+    assert len(_leb(35)) == 434
