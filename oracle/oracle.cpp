@@ -25,9 +25,16 @@
 //   GGA_X_PBE            E, V00 (test_hydrogen.cxx:859-912,
 //                         test_helium.cxx:67-121)
 //   v_rho, v_sigma       central finite differences of e(rho, sigma)
-//   LDA_C_PW(_MOD), GGA_C_PBE   parity unpinned against libxc (not installed);
-//                         checked only against mathematical limits (t -> 0
-//                         gives PW92, derivatives by finite differences).
+//   LDA_C_PW(_MOD)       high-density expansion (exact c0, PW92's published
+//                         c1, no sqrt(rs) term) of all three parameter sets,
+//                         the spin interpolation with the closed-form f''(0),
+//                         the Perdew-Zunger fit, LSD correlation of H (exact
+//                         density) and He (Hartree-Fock density)
+//   GGA_C_PBE            gradient expansion (published beta and gamma =
+//                         (1 - ln 2)/pi^2 from the t^2 and t^4 terms),
+//                         high-density scaling limit, PBE correlation of H
+//                         and He, PW92-mod at zero gradient
+//   libxc's density / sigma / zeta thresholds are restated, not pinned.
 #include <algorithm>
 #include <cmath>
 #include <complex>

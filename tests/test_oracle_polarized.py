@@ -2,9 +2,10 @@
 called by compute_lsda / compute_gga_lsda, xc_integrals.hpp:169-262,
 :543-718).  libxc is not installed, so the checks are exact identities that a
 wrong sign, a dropped term or a misplaced spin index would break:
-spin scaling onto the (separately pinned) unpolarised forms, the PW92
-ferromagnetic and spin-stiffness limits, the PBE gradient expansion, the
-large-gradient limit, finite differences, and dE/dP_s = V_s."""
+spin scaling onto the (separately pinned) unpolarised forms, the PBE
+gradient expansion, the large-gradient limit, finite differences, and
+dE/dP_s = V_s.  The PW92 parameter sets and PBE's beta and gamma are pinned
+in test_oracle_pw92_pbe_c.py."""
 import math
 
 import numpy as np
@@ -69,37 +70,6 @@ def test_lda_x_fully_polarised_closed_form():
     rho = 10 ** RNG.uniform(-4, 2, 50)
     e = _edens([1], rho, 0.0)
     assert np.allclose(e, -0.75 * (6 / math.pi) ** (1 / 3) * rho ** (4 / 3), rtol=1e-14)
-
-
-def _pw_g(rs, A, a1, b1, b2, b3, b4):
-    """PW92 Eq. 10 (Perdew & Wang, PRB 45, 13244)."""
-    q1 = 2 * A * (b1 * np.sqrt(rs) + b2 * rs + b3 * rs ** 1.5 + b4 * rs ** 2)
-    return -2 * A * (1 + a1 * rs) * np.log(1 + 1 / q1)
-
-
-@pytest.mark.parametrize("fid,A1", [(12, 0.015545), (13, 0.01554535)])
-def test_pw92_ferromagnetic_limit(fid, A1):
-    """zeta = 1: f(1) = 1 and the interpolation reduces to the ferromagnetic
-    fit G(rs; A1, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517) (PW92 Table I)."""
-    rho = 10 ** RNG.uniform(-4, 2, 50)
-    rs = (3 / (4 * math.pi * rho)) ** (1 / 3)
-    e = _edens([fid], rho, 0.0)
-    ref = rho * _pw_g(rs, A1, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517)
-    assert np.allclose(e, ref, rtol=1e-12)
-
-
-@pytest.mark.parametrize("fid,A2", [(12, 0.016887), (13, 0.0168869)])
-def test_pw92_spin_stiffness(fid, A2):
-    """d^2 eps / d zeta^2 at zeta = 0 equals the spin stiffness alpha_c =
-    -G(rs; A2, 0.11125, 10.357, 3.6231, 0.88026, 0.49671) > 0 (PW92 Eq. 8
-    expanded to second order, Table I)."""
-    for rho in (1e-3, 0.1, 5.0):
-        rs = (3 / (4 * math.pi * rho)) ** (1 / 3)
-        h = 1e-3
-        eps = [(_edens([fid], rho * (1 + z) / 2, rho * (1 - z) / 2) / rho)[0] for z in (-h, 0.0, h)]
-        d2 = (eps[0] - 2 * eps[1] + eps[2]) / h ** 2
-        ref = -_pw_g(rs, A2, 0.11125, 10.357, 3.6231, 0.88026, 0.49671)
-        assert ref > 0 and abs(d2 - ref) < 2e-5 * ref
 
 
 def test_pbe_c_zero_gradient_is_pw92_mod():
