@@ -153,9 +153,12 @@ def molset_main(args):
     mols, rng = configs.molset(64, 3)
     grid_kw = dict(nrad=75, nang_order=29)
     funcs = (101, 130)
+    order = np.argsort([len(m["Z"]) for m in mols], kind="stable")
+    checked = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    grids = {}
     ctxs, Ps, npts = [], [], 0
     t0 = time.perf_counter()
-    for m in mols:
+    for k, m in enumerate(mols):
         c = nk.Context(0)
         c.set_memory_budget(1 << 30)  # 64 resident contexts
         g = c.make_flat_grid(m["xyz"], m["Z"], **grid_kw)
@@ -164,6 +167,8 @@ def molset_main(args):
         _, P = configs.loewdin_orbitals(S, m["nocc"], rng)
         Pd = torch.as_tensor(P, device="cuda")
         c.vxc(Pd, funcs, precision=args.precision)  # plan and workspace
+        if k in checked:
+            grids[k] = (g.quad_points.cpu().numpy(), g.weights.cpu().numpy(), P)
         ctxs.append(c)
         Ps.append(Pd)
         npts += g.weights.shape[0]
@@ -213,6 +218,9 @@ def molset_main(args):
     log(f"64 sequential builds {seq_default_ms:.1f} ms (default launch), {seq_ms:.1f} ms "
         f"(each autotuned), one set build {set_ms:.1f} ms, "
         f"graph replay {graph_ms:.1f} ms; max |dE| {dE:.2e}, |dN| {dN:.2e}, |dV| {dV:.2e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = molset_cpu_baseline(mols, grids, funcs, ref, res)
     out = {
         "metric": "V_xc builds/s for a 64-molecule PBE set on 1 GPU (one launch sequence vs 64)",
         "value": len(mols) / (graph_ms * 1e-3),
@@ -229,8 +237,45 @@ def molset_main(args):
                    "speedup_vs_sequential": seq_ms / graph_ms,
                    "max_dE_vs_fp64": dE, "max_dN_vs_fp64": dN, "max_dV_vs_fp64": dV},
     }
+    if cpu is not None:
+        out["config"]["vs_oracle"] = cpu["molecules"]
+        out["cpu_baseline"] = cpu["baseline"]
     print(json.dumps(out), flush=True)
     return 0
+
+
+def molset_cpu_baseline(mols, grids, funcs, ref, res):
+    """CPU leg of the molecule-set run: the FP64 CPU restatement of the
+    reference's dense build (oracle.vxc_dense) on the smallest, the median and
+    the largest molecule of the set, timed as the CPU baseline, and the set
+    build's per-molecule errors (FP64 and the run's precision) against it."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    rows, pts_done, t_total = [], 0, 0.0
+    for k, (pts, wts, P) in sorted(grids.items()):
+        b = mols[k]["basis"]
+        basis = dict(n=b.n, l=b.l, m=b.m, zeta=b.zeta, norm=b.norm, centers=b.O)
+        t = time.perf_counter()
+        E0, N0, V0 = O.vxc_dense(basis, pts, wts, funcs, P)
+        t_total += time.perf_counter() - t
+        pts_done += len(wts)
+        row = {"molecule": k, "atoms": int(len(mols[k]["Z"])), "functions": int(b.nbf()),
+               "points": int(len(wts))}
+        for tag, r in (("fp64", ref[k]), ("run", res[k])):
+            row[f"dE_{tag}"] = r.energy - E0
+            row[f"max_dV_{tag}"] = float(np.abs(r.potential.cpu().numpy() - V0).max())
+        log(f"molecule {k} ({row['atoms']} atoms, {row['functions']} functions, "
+            f"{row['points']} points) vs oracle: FP64 dE {row['dE_fp64']:.1e}, dV "
+            f"{row['max_dV_fp64']:.1e}; this run dE {row['dE_run']:.1e}, dV {row['max_dV_run']:.1e}")
+        rows.append(row)
+    return {"molecules": rows,
+            "baseline": {"value": pts_done / t_total, "unit": "grid pts/s", "cores": O.num_threads(),
+                         "kind": "port",
+                         "sample": "the smallest, median and largest molecule of the set, whole "
+                                   "grids, FP64 dense build (collocation, GEMM-route density, "
+                                   "functional, Z, Phi Z^T)"}}
 
 
 def grad_main(args):

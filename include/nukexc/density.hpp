@@ -4,7 +4,9 @@
 // psi = C^T Phi.  C is LayoutLeft (N_bf, N_occ).
 #pragma once
 
+#include "grid.hpp"
 #include "nukexc_config.hpp"
+#include "stobasis.hpp"
 
 namespace Nukexc {
 
@@ -19,6 +21,18 @@ inline DeviceView1D compute_density(const DeviceView2DLeft &collocation_values,
         context().get());
   context().fence();
   return rho;
+}
+
+// compute_density(basis, grid, mo_orbitals, mo_coeff) (density.hpp:55-95):
+// collocation on the grid's points, then rho from (C, f) as above.  The
+// (N_bf, N_pts) collocation matrix is a temporary, as in the reference.
+inline DeviceView1D compute_density(const STOBasisSet &basis, const FlatGrid &grid,
+                                    const DeviceView2DLeft &mo_orbitals,
+                                    const DeviceView1D &mo_coeff) {
+  ExecSpace space;
+  DeviceView2DLeft collocation_values(basis.nbf(), grid.quad_points.size());
+  fill_collocation(space, basis, grid.quad_points, collocation_values);
+  return compute_density(collocation_values, mo_orbitals, mo_coeff);
 }
 
 inline void compute_density_and_sigma(const DeviceView2DLeft &collocation_values,

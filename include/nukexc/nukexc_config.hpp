@@ -58,6 +58,21 @@ inline void initialize(int device = 0) {
   (void)context();
 }
 
+// Execution-space instance (Kokkos::DefaultExecutionSpace in the reference,
+// nukexc_config.hpp:50; `ExecSpace space;` at density.hpp:58 and
+// standalone.cxx).  It names the calling thread's ctx, whose one HIP stream
+// orders every call; work given to a space is asynchronous until fence(), as
+// a Kokkos kernel launched on an instance is.
+class ExecSpace {
+ public:
+  ExecSpace() : ctx_(context().get()) {}
+  nkx_ctx *ctx() const { return ctx_; }
+  void fence() const { check(nkx_sync(ctx_), ctx_); }
+
+ private:
+  nkx_ctx *ctx_;
+};
+
 // ---- points and device arrays ----------------------------------------------
 struct Point {  // ArborX::Point<3, double> in the reference
   double x[3] = {0, 0, 0};

@@ -148,27 +148,59 @@ inline void set_basis(const STOBasisSet &b) {
   have = true;
 }
 
-// fill_collocation: values(mu, g), LayoutLeft (N_bf, N_pts), allocated by the caller.
-inline void fill_collocation(const STOBasisSet &basis, const DeviceArray<Point> &points,
-                             DeviceArray<double> &values) {
+namespace detail {
+inline void need_extents(const STOBasisSet &basis, const DeviceArray<Point> &points,
+                         const DeviceArray<double> &v, const char *what) {
+  if (v.extent(0) != basis.nbf() || v.extent(1) != points.size())
+    throw std::runtime_error(std::string(what) + ": output must be (N_bf, N_pts) = (" +
+                             std::to_string(basis.nbf()) + ", " +
+                             std::to_string(points.size()) + ")");
+}
+}  // namespace detail
+
+// fill_collocation(space, basis, points, values) (stobasis.hpp:494-532):
+// values(mu, g), LayoutLeft (N_bf, N_pts), allocated by the caller; enqueued
+// on the space's stream (asynchronous until space.fence(), as the
+// reference's parallel_for on the instance).
+inline void fill_collocation(ExecSpace &space, const STOBasisSet &basis,
+                             const DeviceArray<Point> &points, DeviceArray<double> &values) {
+  detail::need_extents(basis, points, values, "fill_collocation");
   set_basis(basis);
-  check(nkx_collocation(context().get(), (int64_t)points.size(),
+  check(nkx_collocation(space.ctx(), (int64_t)points.size(),
                         reinterpret_cast<const double *>(points.data()), values.data(), nullptr,
                         nullptr, nullptr),
-        context().get());
-  context().fence();
+        space.ctx());
 }
 
-// fill_grad_collocation: d/dx, d/dy, d/dz, same layout.
+// fill_grad_collocation(space, basis, points, gx, gy, gz)
+// (stobasis.hpp:534-584): d/dx, d/dy, d/dz, same layout and ordering.
+inline void fill_grad_collocation(ExecSpace &space, const STOBasisSet &basis,
+                                  const DeviceArray<Point> &points, DeviceArray<double> &gx,
+                                  DeviceArray<double> &gy, DeviceArray<double> &gz) {
+  detail::need_extents(basis, points, gx, "fill_grad_collocation");
+  detail::need_extents(basis, points, gy, "fill_grad_collocation");
+  detail::need_extents(basis, points, gz, "fill_grad_collocation");
+  set_basis(basis);
+  check(nkx_collocation(space.ctx(), (int64_t)points.size(),
+                        reinterpret_cast<const double *>(points.data()), nullptr, gx.data(),
+                        gy.data(), gz.data()),
+        space.ctx());
+}
+
+// Forms without a space: the thread's default instance, fenced on return.
+inline void fill_collocation(const STOBasisSet &basis, const DeviceArray<Point> &points,
+                             DeviceArray<double> &values) {
+  ExecSpace space;
+  fill_collocation(space, basis, points, values);
+  space.fence();
+}
+
 inline void fill_grad_collocation(const STOBasisSet &basis, const DeviceArray<Point> &points,
                                   DeviceArray<double> &gx, DeviceArray<double> &gy,
                                   DeviceArray<double> &gz) {
-  set_basis(basis);
-  check(nkx_collocation(context().get(), (int64_t)points.size(),
-                        reinterpret_cast<const double *>(points.data()), nullptr, gx.data(),
-                        gy.data(), gz.data()),
-        context().get());
-  context().fence();
+  ExecSpace space;
+  fill_grad_collocation(space, basis, points, gx, gy, gz);
+  space.fence();
 }
 
 }  // namespace Nukexc

@@ -21,7 +21,8 @@
  *    view: element (mu, g) at mu + g * N_bf.
  *  - Functional ids follow libxc numbering: LDA_X = 1, LDA_C_PW = 12,
  *    LDA_C_PW_MOD = 13, LDA_C_VWN = 7, GGA_X_PBE = 101, GGA_C_PBE = 130,
- *    GGA_X_B88 = 106, GGA_C_LYP = 131 (tests/standalone_helpers.hpp:184-225).
+ *    GGA_X_B88 = 106, GGA_C_LYP = 131, GGA_X_PW91 = 109, GGA_C_PW91 = 134
+ *    (tests/standalone_helpers.hpp:184-225).
  */
 #ifndef NUKEXC_HIP_H
 #define NUKEXC_HIP_H
@@ -138,9 +139,13 @@ int nkx_grid_npts(nkx_ctx *ctx, int64_t *npts);
 int nkx_grid_export(nkx_ctx *ctx, double *points_dev, double *weights_dev,
                     int *owner_dev);
 
-/* partition_becke (capped == 0, partitioning.hpp:63-120) and
-   partition_becke_team (capped != 0, :126-213): weights_dev[g] *=
-   P_owner(g) / sum_i P_i on caller arrays.  points_dev is npts x 3. */
+/* partition_becke (mode NKX_BECKE_PLAIN, partitioning.hpp:63-120) and
+   partition_becke_team (NKX_BECKE_CAPPED, :126-213): weights_dev[g] *=
+   P_owner(g) / sum_i P_i on caller arrays.  points_dev is npts x 3.  The
+   capped form skips atoms and factors whose contribution is exactly 0 or 1;
+   NKX_BECKE_CAPPED_UNSCREENED evaluates all of them (same result bit for
+   bit; for checking that).  Any other nonzero mode counts as capped. */
+enum { NKX_BECKE_PLAIN = 0, NKX_BECKE_CAPPED = 1, NKX_BECKE_CAPPED_UNSCREENED = 3 };
 int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers_dev, int64_t npts,
               const double *points_dev, const int *owner_dev, double *weights_dev,
               int capped);

@@ -15,6 +15,7 @@ STATUS_NAMES = {
     4: "NKX_ERR_FUNCTIONAL_FAMILY", 5: "NKX_ERR_NSPIN", 6: "NKX_ERR_NEGATIVE_WEIGHT",
     7: "NKX_ERR_UNKNOWN_FUNCTIONAL", 8: "NKX_ERR_STATE", 9: "NKX_ERR_UNSUPPORTED",
 }
+BECKE_PLAIN, BECKE_CAPPED, BECKE_CAPPED_UNSCREENED = 0, 1, 3
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_BF16 = 2

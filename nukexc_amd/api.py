@@ -25,7 +25,8 @@ except ImportError:  # pragma: no cover
     torch = None
 
 FUNCTIONALS = {"LDA_X": 1, "LDA_C_PW": 12, "LDA_C_PW_MOD": 13, "GGA_X_PBE": 101,
-               "GGA_C_PBE": 130, "GGA_X_B88": 106, "GGA_C_LYP": 131, "LDA_C_VWN": 7}
+               "GGA_C_PBE": 130, "GGA_X_B88": 106, "GGA_C_LYP": 131, "LDA_C_VWN": 7,
+               "GGA_X_PW91": 109, "GGA_C_PW91": 134}
 PBE = (101, 130)
 TA_M4 = 0.6
 TA_M3 = 0.0
@@ -204,6 +205,8 @@ class Context:
     def _nbf(self):
         n = ctypes.c_int()
         self._chk(lib().nkx_basis_nbf(self.h, ctypes.byref(n)))
+        if n.value == 0:
+            raise NukexcError(8, "no basis set on this context (set_basis first)")
         return n.value
 
     def _in_matrix(self, X, nbf, name):
@@ -550,14 +553,20 @@ class Context:
 
 
 # ------------------------------------------------- reference-shaped calls ----
-def partition_becke(ctx, atom_centers, quad_points, point_owner, weights, capped=False):
-    """partition_becke (partitioning.hpp:63-120); in place on weights."""
+def partition_becke(ctx, atom_centers, quad_points, point_owner, weights, capped=False,
+                    screen=True):
+    """partition_becke (partitioning.hpp:63-120); in place on weights.
+    capped=True is partition_becke_team's R_ij cap; screen=False evaluates
+    every factor of the capped form (the screened form must agree bit for
+    bit)."""
     ac = atom_centers.contiguous()
     qp = quad_points.contiguous()
     own = point_owner.to(torch.int32).contiguous()
     assert weights.is_contiguous() and weights.dtype == torch.float64
+    mode = (_capi.BECKE_CAPPED if screen else _capi.BECKE_CAPPED_UNSCREENED) if capped else \
+        _capi.BECKE_PLAIN
     ctx._done(lib().nkx_becke(ctx.h, ac.shape[0], _ptr(ac), qp.shape[0], _ptr(qp), _ptr(own),
-                             _ptr(weights), int(bool(capped))))
+                             _ptr(weights), mode))
     return weights
 
 

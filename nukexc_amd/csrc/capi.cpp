@@ -54,11 +54,13 @@ int functional_info(int id, int *family, int *kind) {
     case 106: *family = NKX_FAMILY_GGA; *kind = 0; return 0;
     case 7: *family = NKX_FAMILY_LDA; *kind = 1; return 0;
     case 131: *family = NKX_FAMILY_GGA; *kind = 1; return 0;
+    case 109: *family = NKX_FAMILY_GGA; *kind = 0; return 0;
+    case 134: *family = NKX_FAMILY_GGA; *kind = 1; return 0;
     default: return 1;
   }
 }
 
-double default_threshold(int id) { return id == 130 ? 1e-12 : 1e-15; }
+double default_threshold(int id) { return id == 130 || id == 134 ? 1e-12 : 1e-15; }
 
 }  // namespace
 
@@ -328,7 +330,9 @@ void build_grid(nkx::Ctx &c, int natoms, const double *xyz, const int *Z, int nr
   out.centers_host.assign(xyz, xyz + 3 * natoms);
   out.centers.upload(xyz, 3 * (size_t)natoms, s);
   out.natoms = natoms;
-  nkx::launch_becke(s, natoms, out.centers.p, n, p0.p, o0.p, w0.p, 1);
+  c.becke_rtab.reserve((size_t)natoms * natoms);
+  nkx::launch_becke(s, natoms, out.centers.p, n, p0.p, o0.p, w0.p, nkx::BECKE_CAPPED,
+                    c.becke_rtab.p);
   cnt.reserve((n + 1023) / 1024 + 1);
   out.pts.reserve(3 * n);
   out.w.reserve(n);
@@ -517,7 +521,12 @@ int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers, int64_t npts, con
   return guarded(ctx, [&] {
     NKX_REQUIRE(natoms > 0 && centers && (npts == 0 || (pts && owner && w)),
                 NKX_ERR_INVALID_ARGUMENT, "becke: bad arguments");
-    nkx::launch_becke(ctx->c.stream, natoms, centers, npts, pts, owner, w, capped ? 1 : 0);
+    const int mode = capped == NKX_BECKE_CAPPED_UNSCREENED
+                         ? nkx::BECKE_CAPPED | nkx::BECKE_NO_SCREEN
+                         : (capped ? nkx::BECKE_CAPPED : 0);
+    ctx->c.becke_rtab.reserve((size_t)natoms * natoms);
+    nkx::launch_becke(ctx->c.stream, natoms, centers, npts, pts, owner, w, mode,
+                      ctx->c.becke_rtab.p);
   });
 }
 

@@ -202,7 +202,7 @@ void radial_rule(int scheme, int n, double R, double alpha, double *r, double *w
   NKX_REQUIRE(n > 0, NKX_ERR_INVALID_ARGUMENT, "radial rule needs n > 0");
   NKX_REQUIRE(scheme == NKX_RADIAL_TREUTLER_AHLRICHS || scheme == NKX_RADIAL_BECKE,
               NKX_ERR_INVALID_ARGUMENT, "unknown radial scheme");
-  const double inv_ln2 = 1.0 / std::log(2.0);
+  const double ln2 = std::log(2.0), inv_ln2 = 1.0 / ln2;
   for (int i = 0; i < n; ++i) {
     const double t = (i + 1) * kPi / (n + 1);
     const double x = std::cos(t);
@@ -210,10 +210,12 @@ void radial_rule(int scheme, int n, double R, double alpha, double *r, double *w
     const double wx = kPi / (n + 1) * st;  // sin^2 t / sqrt(1 - x^2) = sin t
     double rr, jac;
     if (scheme == NKX_RADIAL_TREUTLER_AHLRICHS) {
-      // r = R/ln2 (1+x)^alpha ln(2/(1-x))   (Treutler & Ahlrichs M4)
+      // r = R/ln2 (1+x)^alpha ln(2/(1-x))   (Treutler & Ahlrichs M4), rounded
+      // as ((R / ln2) (1+x)^alpha) ln(2/(1-x)) -- the nodes fix which points
+      // the partition zeroes, so their rounding is kept to one stated order
       const double lg = std::log(2.0 / (1.0 - x));
       const double pa = std::pow(1.0 + x, alpha);
-      rr = R * inv_ln2 * pa * lg;
+      rr = R / ln2 * pa * lg;
       jac = R * inv_ln2 * pa * (alpha * lg / (1.0 + x) + 1.0 / (1.0 - x));
     } else {
       // r = R (1+x)/(1-x)   (Becke)

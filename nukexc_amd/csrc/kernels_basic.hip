@@ -12,50 +12,135 @@ namespace nkx {
 
 // ---------------------------------------------------------------------------
 // Becke partition (partitioning.hpp:44-55, :63-120, :126-213), one thread per
-// point, atoms visited in index order as in the reference.
+// point:
+//   w_g *= P_own / sum_i P_i,  P_i = prod_(j != i) s(mu_ij),
+//   mu_ij = clamp((r_i - r_j) / R~_ij, -1, 1),  s = (1 - p(p(p(mu)))) / 2,
+// with R~ = min(R_ij, 5) in the capped form.  Every product runs over j in
+// index order and the sum over i in index order, as the reference's loops.
 //
-// Capped form, exact screening: with R_ij <= 5 every atom i whose distance
-// exceeds the nearest atom's by >= 5 bohr has mu_{i,nearest} >= 1, so
-// s = 0 and P_i = 0 exactly; every j farther than r_i + 5 has mu_ij <= -1 and
+// Capped form, exact screening: with R~ <= 5 every atom i whose distance
+// exceeds the nearest atom's by >= 5 bohr has mu_(i,nearest) >= 1, so s = 0
+// and P_i = 0 exactly; every j farther than r_i + 5 has mu_ij <= -1 and
 // contributes the factor s(-1) = 1 exactly (p(+-1) = +-1 in FP64).  Skipping
-// those factors and terms leaves every product and sum bit-identical.
+// those factors and terms leaves every product and sum bit-identical (the
+// unscreened mode, BECKE_NO_SCREEN, evaluates all of them and is compared
+// with the screened one bit for bit by the tests).
+//
+// Work per point: distances are compared squared first, so a square root is
+// taken only for atoms near enough to matter (the nearest distance is the
+// root of the smallest square: the rounded root is monotonic).  Candidates i
+// (at most BK_NC per pass, in index order) are held in registers; one sweep
+// over j in index order multiplies every candidate's product by its factor,
+// so each r_j is formed once per pass rather than once per (i, j).  The
+// capped (or plain) inter-atomic distances come from a table (becke_rtab).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double becke_p(double x) { return (1.5 * x) - (0.5 * (x * x * x)); }
-
-__device__ __forceinline__ double bdist(double px, double py, double pz, const double *c) {
-  const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
-  return sqrt(dx * dx + dy * dy + dz * dz);
+// The factor's polynomial and the distances are formed with rounded products
+// and sums in the order written (no contraction into FMAs), the reference's
+// compute_p / compute_s (partitioning.hpp:51-55) with x^3 = (x x) x: near
+// mu = 1, 1 - p(p(p(mu))) keeps no digits below 2^-53, so which points end
+// with exactly zero weight depends on this rounding, and fixing it makes the
+// partition (and the compacted point set) reproducible bit for bit by a host
+// loop with the same order.
+__device__ __forceinline__ double becke_p(double x) {
+#pragma clang fp contract(off)
+  return (1.5 * x) - (0.5 * (x * x * x));
 }
 
+__device__ __forceinline__ double becke_s(double mu) {
+#pragma clang fp contract(off)
+  return 0.5 * (1.0 - becke_p(becke_p(becke_p(mu))));
+}
+
+// squared distance ((dx dx + dy dy) + dz dz), rounded at every step
+__device__ __forceinline__ double bdist2(double px, double py, double pz, const double *c) {
+#pragma clang fp contract(off)
+  const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// R~_ij (capped) or R_ij (plain) for all pairs, row-major natoms x natoms
+__global__ void __launch_bounds__(256) becke_rtab(int natoms, const double *__restrict__ C,
+                                                  int capped, double *__restrict__ rt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)natoms * natoms) return;
+  const int i = (int)(e / natoms), j = (int)(e % natoms);
+  double R = sqrt(bdist2(C[3 * i], C[3 * i + 1], C[3 * i + 2], C + 3 * j));
+  if (capped) R = fmin(R, 5.0);
+  rt[e] = R;
+}
+
+constexpr int BK_NC = 16;
+
 __global__ void __launch_bounds__(256) becke_kernel(int natoms, const double *__restrict__ C,
-                                                    int64_t npts, const double *__restrict__ pts,
+                                                    const double *__restrict__ rt, int64_t npts,
+                                                    const double *__restrict__ pts,
                                                     const int *__restrict__ owner,
-                                                    double *__restrict__ w, int capped) {
+                                                    double *__restrict__ w, int mode) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= npts) return;
+  const bool capped = (mode & BECKE_CAPPED) != 0;
+  const bool screen = capped && !(mode & BECKE_NO_SCREEN);
   const double px = pts[3 * g], py = pts[3 * g + 1], pz = pts[3 * g + 2];
   const int own = owner[g];
-  double rmin = 1e300;
-  if (capped)
-    for (int j = 0; j < natoms; ++j) rmin = fmin(rmin, bdist(px, py, pz, C + 3 * j));
+  double rmin = 0.0, lim_i2 = 0.0;
+  if (screen) {
+    double d2min = 1e300;
+    for (int j = 0; j < natoms; ++j) d2min = fmin(d2min, bdist2(px, py, pz, C + 3 * j));
+    rmin = sqrt(d2min);
+    // beyond this square, r_i - r_min > 5 for sure (margin 1e-3 bohr)
+    lim_i2 = (rmin + 5.001) * (rmin + 5.001);
+  }
   double norm = 0.0, pown = 0.0;
-  for (int i = 0; i < natoms; ++i) {
-    const double ri = bdist(px, py, pz, C + 3 * i);
-    if (capped && ri - rmin >= 5.0) continue;  // P_i == 0 exactly
-    double Pi = 1.0;
-    for (int j = 0; j < natoms; ++j) {
-      if (j == i) continue;
-      const double rj = bdist(px, py, pz, C + 3 * j);
-      if (capped && ri - rj <= -5.0) continue;  // factor == 1 exactly
-      double Rij = bdist(C[3 * i], C[3 * i + 1], C[3 * i + 2], C + 3 * j);
-      if (capped) Rij = fmin(Rij, 5.0);
-      double mu = (ri - rj) / Rij;
-      mu = mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
-      Pi *= 0.5 * (1.0 - becke_p(becke_p(becke_p(mu))));
-      if (Pi == 0.0) break;  // stays exactly zero
+  int next = 0;
+  while (next < natoms) {
+    double ri[BK_NC], Pi[BK_NC];
+    int ii[BK_NC];
+#pragma unroll
+    for (int k = 0; k < BK_NC; ++k) {
+      ri[k] = 0.0;
+      Pi[k] = 1.0;
+      ii[k] = -1;
     }
-    if (i == own) pown = Pi;
-    norm += Pi;
+    int nc = 0;
+    double rmax = 0.0;
+    for (; next < natoms && nc < BK_NC; ++next) {
+      const double d2 = bdist2(px, py, pz, C + 3 * next);
+      if (screen && d2 > lim_i2) continue;
+      const double r = sqrt(d2);
+      if (screen && r - rmin >= 5.0) continue;  // P_i == 0 exactly
+#pragma unroll
+      for (int k = 0; k < BK_NC; ++k)
+        if (k == nc) {
+          ri[k] = r;
+          ii[k] = next;
+        }
+      rmax = fmax(rmax, r);
+      ++nc;
+    }
+    if (nc == 0) break;
+    // beyond this square, r_i - r_j < -5 for every candidate i
+    const double lim_j2 = (rmax + 5.001) * (rmax + 5.001);
+    for (int j = 0; j < natoms; ++j) {
+      const double d2 = bdist2(px, py, pz, C + 3 * j);
+      if (screen && d2 > lim_j2) continue;  // every factor == 1 exactly
+      const double rj = sqrt(d2);
+#pragma unroll
+      for (int k = 0; k < BK_NC; ++k) {
+        const int i = ii[k];
+        if (i < 0 || i == j) continue;
+        // factor == 1 exactly, or the product already exactly zero
+        if (screen && (ri[k] - rj <= -5.0 || Pi[k] == 0.0)) continue;
+        double mu = (ri[k] - rj) / rt[(int64_t)i * natoms + j];
+        mu = mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
+        Pi[k] *= becke_s(mu);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < BK_NC; ++k) {
+      if (ii[k] < 0) continue;
+      if (ii[k] == own) pown = Pi[k];
+      norm += Pi[k];
+    }
   }
   w[g] *= pown / norm;
 }
@@ -64,7 +149,9 @@ __global__ void __launch_bounds__(256) becke_kernel(int natoms, const double *__
 // shell q of atom sh_atom[q] at radius sh_r[q] (radial weight sh_w[q]) takes
 // angular rule sh_rule[q] (points ang[4 (rule_off[k] + j) ..]) and writes
 // its points from sh_poff[q]: centre + r u_j with weight w_r w_j.  Rounded
-// products and sums, no contraction: bit-identical to a host loop.
+// products and sums, no contraction (the pragma: HIP's __dmul_rn is a plain
+// product that may be fused into the following sum): bit-identical to a
+// host loop.
 __global__ void __launch_bounds__(256) grid_shells_kernel(
     const int *__restrict__ sh_atom, const double *__restrict__ sh_r,
     const double *__restrict__ sh_w, const int *__restrict__ sh_rule,
@@ -72,13 +159,14 @@ __global__ void __launch_bounds__(256) grid_shells_kernel(
     const int *__restrict__ rule_n, const double *__restrict__ xyz,
     const double *__restrict__ ang, double *__restrict__ P, double *__restrict__ W,
     int *__restrict__ O) {
+#pragma clang fp contract(off)
   const int q = blockIdx.x, a = sh_atom[q], k = sh_rule[q];
   const double r = sh_r[q], wr = sh_w[q];
   const double *u = ang + 4 * (int64_t)rule_off[k];
   for (int j = threadIdx.x; j < rule_n[k]; j += 256) {
     const int64_t g = sh_poff[q] + j;
-    for (int d = 0; d < 3; ++d) P[3 * g + d] = __dadd_rn(xyz[3 * a + d], __dmul_rn(r, u[4 * j + d]));
-    W[g] = __dmul_rn(wr, u[4 * j + 3]);
+    for (int d = 0; d < 3; ++d) P[3 * g + d] = xyz[3 * a + d] + r * u[4 * j + d];
+    W[g] = wr * u[4 * j + 3];
     O[g] = a;
   }
 }
@@ -94,11 +182,15 @@ void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const dou
 }
 
 void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
-                  const double *pts, const int *owner, double *w, int capped) {
+                  const double *pts, const int *owner, double *w, int mode, double *rtab) {
   if (npts == 0) return;
+  const int64_t npair = (int64_t)natoms * natoms;
+  hipLaunchKernelGGL(becke_rtab, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, s, natoms,
+                     centers, mode & BECKE_CAPPED, rtab);
+  NKX_HIP_CHECK(hipGetLastError());
   const int64_t blocks = (npts + 255) / 256;
-  hipLaunchKernelGGL(becke_kernel, dim3((unsigned)blocks), dim3(256), 0, s, natoms, centers,
-                     npts, pts, owner, w, capped);
+  hipLaunchKernelGGL(becke_kernel, dim3((unsigned)blocks), dim3(256), 0, s, natoms, centers, rtab,
+                     npts, pts, owner, w, mode);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -121,12 +121,13 @@ __global__ void __launch_bounds__(NT) tile_boxes(int64_t n, int per, int64_t nbo
 // Sphere (c, r) meets box: distance from c to the box <= r (rounded
 // products and sums, no contraction, so the test is reproducible on the host).
 __device__ __forceinline__ bool meets(const double *box, const double *c, double r) {
+#pragma clang fp contract(off)
   double d2 = 0.0;
   for (int d = 0; d < 3; ++d) {
     double e = 0.0;
     if (c[d] < box[d]) e = box[d] - c[d];
     else if (c[d] > box[3 + d]) e = c[d] - box[3 + d];
-    d2 = __dadd_rn(d2, __dmul_rn(e, e));
+    d2 = d2 + e * e;
   }
   return sqrt(d2) <= r;
 }

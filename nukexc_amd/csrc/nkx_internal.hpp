@@ -248,6 +248,7 @@ struct Ctx {
   uint64_t mplan_version = 0;    // bumped by every nkx_molset_create
   Workspace ws;
   GraphState graph;
+  DevBuf<double> becke_rtab;     // inter-atomic distances of the last partition
 };
 
 // ---- host helpers (basis.cpp, lebedev.cpp, grid.cpp) --------------------
@@ -266,8 +267,12 @@ void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const dou
                         const double *sh_w, const int *sh_rule, const int64_t *sh_poff,
                         const int *rule_off, const int *rule_n, const double *xyz,
                         const double *ang, double *P, double *W, int *O);
+// Becke partition modes: BECKE_CAPPED (R~ = min(R, 5), partition_becke_team)
+// or plain; BECKE_NO_SCREEN evaluates every factor of the capped form (the
+// screened and unscreened forms agree bit for bit).  rtab: natoms^2 doubles.
+constexpr int BECKE_CAPPED = 1, BECKE_NO_SCREEN = 2;
 void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
-                  const double *pts, const int *owner, double *w, int capped);
+                  const double *pts, const int *owner, double *w, int mode, double *rtab);
 int64_t compact_points(hipStream_t s, int64_t n, double thr, const double *pts_in,
                        const double *w_in, const int *own_in, double *pts_out,
                        double *w_out, int *own_out, int *scratch_flags,

@@ -16,6 +16,22 @@
 //               + (2/3 rho^2 - rho_b^2) s_aa },
 //   w = exp(-c rho^-1/3) / (1 + d rho^-1/3) rho^(-11/3),
 //   del = c rho^-1/3 + d rho^-1/3 / (1 + d rho^-1/3).
+//
+// PW91 exchange (libxc GGA_X_PW91, id 109; Perdew et al., Phys. Rev. B 46,
+// 6671 (1992), Eq. 8): spin scaling of the unpolarised
+//   e = A_x n^(4/3) F(s),  F = 1 + [(c + d e^(-alpha s^2)) s^2 - f s^4]
+//                               / [1 + a s asinh(b s) + f s^4],
+// s = |grad n| / (2 k_F n), a = 0.19645, b = 7.7956, c = 0.2743, d = -0.1508,
+// f = 0.004, alpha = 100 (a, b, c are Becke's beta = 0.0042 in the s variable,
+// c + d the gradient-expansion coefficient 10/81).
+// PW91 correlation (libxc GGA_C_PW91, id 134; same paper, Eqs. 13-15) on
+// PW92 with the PW_MOD parameters: eps_c + H0 + H1,
+//   H0 = phi^3 beta^2/(2 alpha) ln(1 + (2 alpha/beta)(t^2 + A t^4)/(1 + A t^2 + A^2 t^4)),
+//   A  = (2 alpha/beta) / (exp(-2 alpha eps_c / (phi^3 beta^2)) - 1),
+//   H1 = nu (C_xc(rs) - C_xc(0) - 3 C_x/7) phi^3 t^2 exp(-100 phi^4 (k_s^2/k_F^2) t^2),
+// alpha = 0.09, nu = (16/pi)(3 pi^2)^(1/3), beta = nu C_c(0), C_c(0) = 0.004235,
+// C_x = -0.001667, C_xc(rs) the Rasolt-Geldart Pade
+//   (2.568 + 23.266 rs + 0.007389 rs^2) / (1000 (1 + 8.723 rs + 0.472 rs^2 + 0.07389 rs^3)).
 #pragma once
 
 #include "xc_dual.hpp"
@@ -89,6 +105,87 @@ __host__ __device__ inline D5 vwn(D5 ra, D5 rb, double zthr) {
   const D5 z4 = z * z * z * z;
   const D5 eP = vwn_g(x, P), eF = vwn_g(x, F), eA = vwn_g(x, Al);
   return rr * (eP + eA * f * (1.0 - z4) / fpp + (eF - eP) * f * z4);
+}
+
+// PW91 exchange of an unpolarised density n with gradient square g
+__host__ __device__ inline D5 pw91_x_unpol(D5 n, D5 g) {
+  constexpr double a = 0.19645, b = 7.7956, c = 0.2743, d = -0.1508, f = 0.004, al = 100.0;
+  constexpr double cs = 0.026121172985233605;  // 1 / (4 (3 pi^2)^(2/3))
+  const D5 n43 = dpow(n, 4.0 / 3.0);
+  const D5 s2 = cs * g / (n43 * n43);
+  const D5 sv = dsqrt(s2);
+  const D5 s4 = s2 * s2;
+  const D5 F = 1.0 + ((c + d * dexp(-al * s2)) * s2 - f * s4) /
+                         (1.0 + a * sv * dasinh(b * sv) + f * s4);
+  return -0.73855876638202240588 * n43 * F;
+}
+
+// y = (beta/g) t2 (1 + u) / (1 + u + u^2), u = A t2, with the partials in
+// closed form (dy/dt2 = (beta/g)(1 + 2u)/D^2, dy/dA = -(beta/g) t2^2 u (2 + u)/D^2,
+// D = 1 + u + u^2): the quotient differentiated term by term cancels to a
+// relative 1/u^2 where the enhancement saturates.
+__host__ __device__ inline D5 gga_c_y(D5 t2, D5 A, double bg) {
+  D5 y;
+  const double u = A.v * t2.v, D = 1.0 + u + u * u;
+  const double dt = bg * (1.0 + 2.0 * u) / (D * D);
+  const double dA = -bg * t2.v * t2.v * u * (2.0 + u) / (D * D);
+  y.v = bg * t2.v * (1.0 + u) / D;
+  for (int k = 0; k < 5; ++k) y.d[k] = dt * t2.d[k] + dA * A.d[k];
+  return y;
+}
+
+// PW91 correlation energy per particle given eps_c (PW_MOD), phi, rs, t^2
+__host__ __device__ inline D5 pw91_c_h(D5 ec, D5 phi, D5 rs, D5 t2) {
+  constexpr double alpha = 0.09, Cc0 = 4.235e-3, Cxc0 = 2.568e-3, Cx = -0.001667;
+  constexpr double nu = 15.755920349483144;  // (16/pi)(3 pi^2)^(1/3)
+  constexpr double beta = nu * Cc0;
+  constexpr double gam = beta * beta / (2.0 * alpha);
+  // -100 (k_s/k_F)^2 / rs = -100 (4/pi) (4/(9 pi))^(1/3)
+  constexpr double ha = -66.34364396064501;
+  const D5 phi2 = phi * phi, phi3 = phi2 * phi;
+  const D5 A = (beta / gam) / (dexp(-1.0 * ec / (gam * phi3)) - 1.0);
+  const D5 H0 = gam * phi3 * dlog(1.0 + gga_c_y(t2, A, beta / gam));
+  const D5 rs2 = rs * rs;
+  const D5 Cxc = (2.568 + 23.266 * rs + 0.007389 * rs2) /
+                 (1000.0 * (1.0 + 8.723 * rs + 0.472 * rs2 + 0.07389 * rs2 * rs));
+  const D5 H1 = nu * (Cxc - (Cxc0 + 3.0 * Cx / 7.0)) * phi3 * t2 *
+                dexp(ha * rs * phi2 * phi2 * t2);
+  return H0 + H1;
+}
+
+// PW92 energy per particle, PW_MOD parameters (the correlation under PBE
+// and PW91): G0 + z^4 f(z) (G1 - G0 + G2/f''(0)) - f(z) G2/f''(0)
+__host__ __device__ inline D5 pw92_mod_g(D5 rs, double A, double a1, double b1, double b2,
+                                         double b3, double b4) {
+  const D5 s = dsqrt(rs);
+  const D5 den = 2.0 * A * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * dlog(1.0 + 1.0 / den);
+}
+__host__ __device__ inline D5 pw92_mod_eps(D5 rs, D5 z, D5 opz, D5 omz) {
+  const D5 g0 = pw92_mod_g(rs, 0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const D5 g1 = pw92_mod_g(rs, 0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  const D5 g2 = pw92_mod_g(rs, 0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671);
+  const double ifz20 = 1.0 / 1.709920934161365617563962776245;
+  const D5 fz = (dpow(opz, 4.0 / 3.0) + dpow(omz, 4.0 / 3.0) - 2.0) * (1.0 / 0.5198420997897464);
+  const D5 z2 = z * z;
+  const D5 z4 = z2 * z2;
+  return g0 + z4 * fz * (g1 - g0 + g2 * ifz20) - fz * g2 * ifz20;
+}
+
+// PW91 correlation energy density rho (eps_c + H0 + H1); 1 +- zeta as
+// 2 rho_s / rho floored at zthr; stot = sigma_aa + 2 sigma_ab + sigma_bb
+__host__ __device__ inline D5 pw91_c(D5 ra, D5 rb, D5 stot, double zthr) {
+  constexpr double kPi = 3.14159265358979323846;
+  const D5 rr = ra + rb;
+  const D5 r13 = dpow(rr, 1.0 / 3.0);
+  const D5 rs = 0.62035049089940001667 / r13;
+  const D5 opz = dmax(2.0 * ra / rr, zthr), omz = dmax(2.0 * rb / rr, zthr);
+  const D5 ec = pw92_mod_eps(rs, (ra - rb) / rr, opz, omz);
+  const D5 phi = 0.5 * (dpow(opz, 2.0 / 3.0) + dpow(omz, 2.0 / 3.0));
+  // t^2 = stot / (4 phi^2 ks^2 rho^2), ks^2 = 4 kF / pi, kF = (3 pi^2 rho)^(1/3)
+  const D5 kF = 3.09366772628013593097 * r13;
+  const D5 t2 = stot * kPi / (16.0 * phi * phi * kF * rr * rr);
+  return rr * (ec + pw91_c_h(ec, phi, rs, t2));
 }
 
 }  // namespace xcb

@@ -8,7 +8,8 @@
 // Parameters follow libxc: LDA_X (1), LDA_C_PW (12), LDA_C_PW_MOD (13),
 // GGA_X_PBE (101: kappa 0.804, mu = beta pi^2/3, beta 0.06672455060314922),
 // GGA_C_PBE (130: same beta, gamma = (1 - ln 2)/pi^2, PW92 with the PW_MOD
-// parameter set), GGA_X_B88 (106), GGA_C_LYP (131) and LDA_C_VWN (7) (xc_extra.hpp).  Threshold handling: rho < thr contributes nothing; sigma
+// parameter set), GGA_X_B88 (106), GGA_C_LYP (131), LDA_C_VWN (7), GGA_X_PW91
+// (109) and GGA_C_PW91 (134) (xc_extra.hpp).  Threshold handling: rho < thr contributes nothing; sigma
 // is floored at thr^(8/3).
 #pragma once
 
@@ -23,15 +24,22 @@ struct XcPoint {
 };
 
 namespace xcd {
-// B88 (106), LYP (131) or VWN (7) at the unpolarised point: the spin-resolved forms
+// B88 (106), LYP (131), VWN (7), PW91 (109, 134) at the unpolarised point: the spin-resolved forms
 // at rho_a = rho_b = rho / 2, sigma_ss = sigma_ab = sigma / 4, so
 // de/drho = (e_a + e_b) / 2 and de/dsigma = (e_aa + e_ab + e_bb) / 4.
 __host__ __device__ inline void blyp_unpol(int id, double rho, double sigma, XcPoint &o) {
   const D5 ra = dvar(0.5 * rho, 0), rb = dvar(0.5 * rho, 1);
   const D5 saa = dvar(0.25 * sigma, 2), sab = dvar(0.25 * sigma, 3), sbb = dvar(0.25 * sigma, 4);
-  const D5 e = id == 106   ? xcb::b88_channel(ra, saa) + xcb::b88_channel(rb, sbb)
-                : id == 7 ? xcb::vwn(ra, rb, 2.220446049250313e-16)
-                          : xcb::lyp(ra, rb, saa, sab, sbb);
+  D5 e;
+  switch (id) {
+    case 106: e = xcb::b88_channel(ra, saa) + xcb::b88_channel(rb, sbb); break;
+    case 7: e = xcb::vwn(ra, rb, 2.220446049250313e-16); break;
+    case 109:
+      e = 0.5 * (xcb::pw91_x_unpol(2.0 * ra, 4.0 * saa) + xcb::pw91_x_unpol(2.0 * rb, 4.0 * sbb));
+      break;
+    case 134: e = xcb::pw91_c(ra, rb, saa + 2.0 * sab + sbb, 2.220446049250313e-16); break;
+    default: e = xcb::lyp(ra, rb, saa, sab, sbb); break;
+  }
   o.e += e.v;
   o.vrho += 0.5 * (e.d[0] + e.d[1]);
   o.vsigma += 0.25 * (e.d[2] + e.d[3] + e.d[4]);
@@ -142,7 +150,9 @@ __host__ __device__ inline XcPoint xc_eval(int n, const int *ids, const double *
       case 130: xcd::pbe_c(rho, r13, s, o); break;
       case 7:
       case 106:
-      case 131: xcd::blyp_unpol(ids[k], rho, s, o); break;
+      case 109:
+      case 131:
+      case 134: xcd::blyp_unpol(ids[k], rho, s, o); break;
       default: break;
     }
   }

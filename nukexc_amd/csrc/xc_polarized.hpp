@@ -142,6 +142,14 @@ __host__ __device__ inline void xc_eval_polarized(int n, const int *ids, const d
       case 131:
         r = xcb::lyp(ra, rb, saa, sab, sbb);
         break;
+      case 109:  // spin-scaled; a channel at or below thr drops out
+        r = dconst(0.0);
+        if (xa > t) r = r + 0.5 * xcb::pw91_x_unpol(2.0 * ra, 4.0 * saa);
+        if (xb > t) r = r + 0.5 * xcb::pw91_x_unpol(2.0 * rb, 4.0 * sbb);
+        break;
+      case 134:
+        r = xcb::pw91_c(ra, rb, saa + 2.0 * sab + sbb, xcp::kZetaThr);
+        break;
       case 7:
         r = xcb::vwn(ra, rb, xcp::kZetaThr);
         break;

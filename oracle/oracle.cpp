@@ -116,15 +116,15 @@ inline double becke_mu(double ri, double rj, double Rij) {
   double mu = (ri - rj) / Rij;
   return mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
 }
-inline double becke_p(double x) { return (1.5 * x) - (0.5 * std::pow(x, 3)); }
+// x^3 as (x x) x (the reference's std::pow(x, 3) rounds once; near mu = 1 the
+// last bit of p decides whether 1 - p(p(p(mu))) is exactly zero, so the
+// oracle fixes one rounding order, which the device kernel repeats)
+inline double becke_p(double x) { return (1.5 * x) - (0.5 * (x * x * x)); }
 inline double becke_s(double f) { return 0.5 * (1.0 - f); }
 
 inline double dist3(const double *a, const double *b) {
-  double d = 0;
-  d += std::pow(a[0] - b[0], 2);
-  d += std::pow(a[1] - b[1], 2);
-  d += std::pow(a[2] - b[2], 2);
-  return std::sqrt(d);
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,9 +397,76 @@ T vwn_e(T ra, T rb) {
   return n * (eP + eA * f / fpp * (1.0 - z4) + (eF - eP) * f * z4);
 }
 
+// PW91 exchange (GGA_X_PW91, id 109; Perdew et al., PRB 46, 6671 (1992),
+// Eq. 8) of an unpolarised density n with gradient square g:
+//   A_x n^(4/3) [1 + ((c + d exp(-alpha s^2)) s^2 - f s^4) / (1 + a s asinh(b s) + f s^4)],
+// s^2 = g / (4 (3 pi^2)^(2/3) n^(8/3)), with libxc's parameters.
+template <typename T>
+T pw91_x_n(T n, T g) {
+  const double a = 0.19645, b = 7.7956, c = 0.2743, d = -0.1508, f = 0.004, alpha = 100.0;
+  const double Ax = -0.75 * std::cbrt(3.0 / kPi);
+  T n43 = std::pow(n, 4.0 / 3.0);
+  T s2 = g / (4.0 * std::pow(3.0 * kPi * kPi, 2.0 / 3.0) * n43 * n43);
+  T s = std::sqrt(s2);
+  T num = (c + d * std::exp(-alpha * s2)) * s2 - f * s2 * s2;
+  T den = 1.0 + a * s * std::asinh(b * s) + f * s2 * s2;
+  return Ax * n43 * (1.0 + num / den);
+}
+
+// defined with the polarised functionals below
+template <typename T>
+T pw92_pol(T rs, T z, T opz, T omz, bool mod);
+template <typename T>
+T floor_eps(T x);
+inline double pbe_y(double t2, double A, double bg);
+inline std::complex<double> pbe_y(std::complex<double> t2, std::complex<double> A, double bg);
+
+// PW91 correlation (GGA_C_PW91, id 134; same paper, Eqs. 13-15): energy
+// density rho (eps_c + H0 + H1) with eps_c the PW_MOD PW92 energy,
+//   H0 = phi^3 beta^2/(2 alpha) ln(1 + 2 alpha/beta (t^2 + A t^4)/(1 + A t^2 + A^2 t^4)),
+//   A  = 2 alpha/beta / (exp(-2 alpha eps_c/(phi^3 beta^2)) - 1),
+//   H1 = nu (C_xc(rs) - C_xc(0) - 3 C_x / 7) phi^3 t^2 exp(-100 phi^4 (ks/kF)^2 t^2),
+// nu = 16/pi (3 pi^2)^(1/3), beta = nu C_c(0), C_c(0) = 4.235e-3,
+// C_x = -1.667e-3, alpha = 0.09, C_xc the Rasolt-Geldart Pade (PRB 34, 1325
+// (1986)): (2.568 + 23.266 rs + 0.007389 rs^2)/(1000 (1 + 8.723 rs + 0.472 rs^2
+// + 0.07389 rs^3)).  t^2 = |grad rho|^2 / (2 phi ks rho)^2 as for PBE.
+template <typename T>
+T pw91_c_e(T ra, T rb, T stot) {
+  const double alpha = 0.09, Cc0 = 4.235e-3, Cx = -1.667e-3;
+  const double nu = 16.0 / kPi * std::cbrt(3.0 * kPi * kPi);
+  const double beta = nu * Cc0;
+  T rho = ra + rb;
+  T z = (ra - rb) / rho;
+  T opz = 2.0 * ra / rho, omz = 2.0 * rb / rho;
+  T rs = std::pow(3.0 / (4.0 * kPi * rho), 1.0 / 3.0);
+  T ec = pw92_pol(rs, z, opz, omz, true);
+  T phi = 0.5 * (std::pow(floor_eps(opz), 2.0 / 3.0) + std::pow(floor_eps(omz), 2.0 / 3.0));
+  T kF = std::pow(3.0 * kPi * kPi * rho, 1.0 / 3.0);
+  T ks2 = 4.0 * kF / kPi;
+  T t2 = stot / (4.0 * phi * phi * ks2 * rho * rho);
+  T phi3 = phi * phi * phi;
+  T A = (2.0 * alpha / beta) / (std::exp(-2.0 * alpha * ec / (phi3 * beta * beta)) - 1.0);
+  T H0 = phi3 * beta * beta / (2.0 * alpha) *
+         std::log(1.0 + pbe_y(t2, A, 2.0 * alpha / beta));
+  auto cxc = [](T r) {
+    return (2.568 + 23.266 * r + 0.007389 * r * r) /
+           (1000.0 * (1.0 + 8.723 * r + 0.472 * r * r + 0.07389 * r * r * r));
+  };
+  T H1 = nu * (cxc(rs) - cxc(T(0.0)) - 3.0 * Cx / 7.0) * phi3 * t2 *
+         std::exp(-100.0 * phi3 * phi * (ks2 / (kF * kF)) * t2);
+  return rho * (ec + H0 + H1);
+}
+
 template <typename T>
 T blyp_e(int id, T ra, T rb, T saa, T sab, T sbb, bool a_on, bool b_on) {
   if (id == 7) return vwn_e(ra, rb);
+  if (id == 134) return pw91_c_e(ra, rb, saa + 2.0 * sab + sbb);
+  if (id == 109) {
+    T e = 0.0;
+    if (a_on) e += 0.5 * pw91_x_n(T(2.0) * ra, T(4.0) * saa);
+    if (b_on) e += 0.5 * pw91_x_n(T(2.0) * rb, T(4.0) * sbb);
+    return e;
+  }
   if (id == 106) {
     T e = 0.0;
     if (a_on) e += b88_spin(ra, saa);
@@ -429,7 +496,8 @@ XcOut blyp_unpol(int id, double rho, double sigma) {
 // xc_func_info tables.  Parity unpinned (libxc not installed).
 double default_dens_threshold(int id) {
   switch (id) {
-    case 130: return 1e-12;
+    case 130:
+    case 134: return 1e-12;
     default: return 1e-15;
   }
 }
@@ -449,12 +517,16 @@ bool xc_eval_point(int id, double rho, double sigma, double thr, XcOut *o) {
     case 130: *o = gga_c_pbe(rho, s); return true;
     case 7:
     case 106:
-    case 131: *o = blyp_unpol(id, rho, s); return true;
+    case 109:
+    case 131:
+    case 134: *o = blyp_unpol(id, rho, s); return true;
     default: return false;
   }
 }
 
-bool is_gga(int id) { return id == 101 || id == 130 || id == 106 || id == 131; }
+bool is_gga(int id) {
+  return id == 101 || id == 130 || id == 106 || id == 131 || id == 109 || id == 134;
+}
 
 // ---------------------------------------------------------------------------
 // Spin-polarised functionals (libxc XC_POLARIZED, called by compute_lsda /
@@ -562,7 +634,9 @@ T e_polarized(int id, const T *x, double thr) {
       return pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
     case 7:
     case 106:
+    case 109:
     case 131:
+    case 134:
       return blyp_e(id, ra, rb, saa, sab, sbb, std::real(ra) > thr, std::real(rb) > thr);
   }
   return T(0.0);

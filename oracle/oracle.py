@@ -315,3 +315,54 @@ def neighbor_list(boxes, centers, radii):
         nbr.extend(hit.tolist())
         off.append(len(nbr))
     return np.array(off, dtype=np.int32), np.array(nbr, dtype=np.int32)
+
+
+# ---- chunked dense build over BLAS (large parity cases) --------------------
+def vxc_dense(basis, pts, w, ids, P, chunk=16384, want_density=False):
+    """E_xc, N_el, V_xc of compute_gga / compute_lda (xc_integrals.hpp:424-541,
+    :73-166) from a density matrix, evaluated chunk by chunk so that large
+    grids fit in host memory: collocation and the functional from the C
+    restatement above, the contractions as FP64 GEMMs as the reference's
+    KokkosBlas calls (density.hpp:205-212, xc_integrals.hpp:520):
+      X = P Phi,  rho = sum_mu Phi X,  grad rho = 2 sum_mu grad Phi X,
+      Z = w (1/2 v_rho Phi + 2 v_sigma grad rho . grad Phi),  V = Phi Z^T + Z Phi^T.
+    With want_density, also returns rho and sigma per point."""
+    ids = _i32(ids)
+    gga = any(lib().orc_is_gga(int(i)) for i in ids)
+    P = _f64(P)
+    pts = _f64(pts)
+    w = _f64(w)
+    nbf = P.shape[0]
+    E = 0.0
+    N = 0.0
+    V = np.zeros((nbf, nbf))
+    rho_all = np.empty(len(w)) if want_density else None
+    sig_all = np.empty(len(w)) if want_density else None
+    for s in range(0, len(w), chunk):
+        e = slice(s, min(len(w), s + chunk))
+        if gga:
+            phi, gx, gy, gz = collocation(basis, pts[e], grad=True)
+        else:
+            phi = collocation(basis, pts[e], grad=False)
+        X = P @ phi
+        rho = np.einsum("ug,ug->g", phi, X)
+        sig = None
+        if gga:
+            gr = 2.0 * np.stack([np.einsum("ug,ug->g", g, X) for g in (gx, gy, gz)])
+            sig = np.einsum("kg,kg->g", gr, gr)
+        exc, vr, vs = xc_eval(ids, rho, sig)
+        ww = w[e]
+        E += float(ww @ (rho * exc))
+        N += float(ww @ rho)
+        Z = (0.5 * ww * vr) * phi
+        if gga:
+            b = 2.0 * ww * vs
+            Z += (b * gr[0]) * gx + (b * gr[1]) * gy + (b * gr[2]) * gz
+        Vb = phi @ Z.T
+        V += Vb + Vb.T
+        if want_density:
+            rho_all[e] = rho
+            sig_all[e] = sig if gga else 0.0
+    if want_density:
+        return E, N, V, rho_all, sig_all
+    return E, N, V

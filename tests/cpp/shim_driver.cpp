@@ -30,13 +30,50 @@ int main() {
   STOBasisSet basis = make_manual_basis({{1, 0, 0, 1.0, 0., 0., 0.}});
   const size_t npts = grid.weights.size();
 
+  // the reference's argument lists (standalone.cxx:413,424: an ExecSpace
+  // instance first)
+  ExecSpace space;
   DeviceView2DLeft phi(1, npts), gx(1, npts), gy(1, npts), gz(1, npts);
-  fill_collocation(basis, grid.quad_points, phi);
-  fill_grad_collocation(basis, grid.quad_points, gx, gy, gz);
+  fill_collocation(space, basis, grid.quad_points, phi);
+  fill_grad_collocation(space, basis, grid.quad_points, gx, gy, gz);
+  space.fence();
   DeviceView2DLeft C(1, 1);
   DeviceView1D f(1);
   C.from_host(std::vector<double>{1.0});
   f.from_host(std::vector<double>{1.0});
+
+  // compute_density(basis, grid, C, f) (density.hpp:55-95): N_el = 1 and the
+  // same values as from the collocation matrix
+  {
+    auto rho_g = compute_density(basis, grid, C, f).to_host();
+    auto rho_c = compute_density(phi, C, f).to_host();
+    const auto w = grid.weights.to_host();
+    double n = 0.0, dmax = 0.0;
+    for (size_t g = 0; g < npts; ++g) {
+      n += w[g] * rho_g[g];
+      dmax = std::fmax(dmax, std::fabs(rho_g[g] - rho_c[g]));
+    }
+    expect_rel("H compute_density(grid) N", n, 1.0, 1e-10);
+    std::printf("%-28s max |diff| %.2e\n", "grid vs collocation rho", dmax);
+    if (dmax != 0.0) {
+      std::printf("FAIL grid-based compute_density differs\n");
+      return 1;
+    }
+  }
+  // a mis-sized output is refused before any device work
+  {
+    bool bad = false;
+    DeviceView2DLeft wrong(2, npts);
+    try {
+      fill_collocation(space, basis, grid.quad_points, wrong);
+    } catch (const std::runtime_error &) {
+      bad = true;
+    }
+    if (!bad) {
+      std::printf("FAIL fill_collocation accepted a (2, N_pts) output for one function\n");
+      return 1;
+    }
+  }
 
   // test_hydrogen.cxx:815-857
   auto lda = compute_lda(phi, grid.weights, C, f, Functional({1}));

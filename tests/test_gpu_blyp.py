@@ -1,4 +1,5 @@
-"""B88 exchange, LYP and VWN correlation (ids 106, 131, 7) on the GPU
+"""B88 exchange, LYP and VWN correlation (ids 106, 131, 7) and PW91
+exchange and correlation (109, 134) on the GPU
 against the oracle: pointwise (unpolarised and polarised, including one-spin points),
 in the batched V_xc build (FP64 and int8 mixed), and the helium atom built
 end to end from the Thakkar table (reader, grid, build) against the radial
@@ -27,7 +28,8 @@ def dev(a):
     return torch.tensor(a, device="cuda")
 
 
-@pytest.mark.parametrize("ids", [[106], [131], [106, 131], [7], [1, 7]])
+@pytest.mark.parametrize("ids", [[106], [131], [106, 131], [7], [1, 7], [109], [134],
+                                 [109, 134]])
 def test_functional_eval_matches_oracle(ctx, ids):
     rng = np.random.default_rng(11)
     n = 4000
@@ -56,20 +58,20 @@ def test_functional_eval_matches_oracle(ctx, ids):
         assert np.allclose(s1, s0, rtol=1e-9, atol=1e-12 * np.abs(s0).max())
 
 
-def test_blyp_build_water_matches_oracle(ctx):
+@pytest.mark.parametrize("funcs", [(106, 131), (109, 134)], ids=["BLYP", "PW91"])
+def test_blyp_build_water_matches_oracle(ctx, funcs):
     w = configs.workload("water")
     grid = ctx.make_flat_grid(w["xyz"], w["Z"], nrad=40, nang_order=23)
     ctx.set_basis(w["basis"])
     pts, wts = grid.quad_points.cpu().numpy(), grid.weights.cpu().numpy()
     S, C, P = orbitals_on_grid(w, pts, wts)
-    funcs = (106, 131)
     phi, gx, gy, gz = oracle_on_grid(w["basis"], pts, grad=True)
     E0, N0, V0 = O.vxc_from_collocation(phi, wts, funcs, P=P, gx=gx, gy=gy, gz=gz)
     r = ctx.vxc(dev(P), funcs=funcs)
     rm = ctx.vxc(dev(P), funcs=funcs, precision="mixed")
     dV = np.abs(r.potential.cpu().numpy() - V0).max()
     dVm = np.abs(rm.potential.cpu().numpy() - V0).max()
-    print(f"BLYP water: E = {r.energy:.10f}, dE = {r.energy - E0:.1e}, dV = {dV:.1e}; "
+    print(f"{funcs} water: E = {r.energy:.10f}, dE = {r.energy - E0:.1e}, dV = {dV:.1e}; "
           f"mixed dE = {rm.energy - E0:.1e}, dV = {dVm:.1e}")
     assert abs(r.energy - E0) < 1e-8 and dV < 1e-9
     assert abs(rm.energy - E0) < 1e-6 and dVm < 1e-7

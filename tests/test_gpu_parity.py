@@ -102,6 +102,30 @@ def test_becke_matches_oracle(ctx, capped):
     assert err.max() < 1e-14
 
 
+@pytest.mark.parametrize("name,nrad,order,stride", [("water", 75, 29, 1), ("benzene", 99, 41, 1),
+                                                    ("water100", 75, 29, 7)])
+def test_becke_exact_screening_is_bitwise(ctx, name, nrad, order, stride):
+    """The capped partition's exact screening (skipped atoms with P_i = 0,
+    skipped factors s = 1) against every factor evaluated, bit for bit, on
+    the raw product points of the water (75 x 302), benzene (99 x 590) and
+    100-water (75 x 302, every 7th point) grids; plus the oracle's
+    unscreened partition on a sample."""
+    w = configs.workload(name)
+    P, W, Ow = _raw_points(w["xyz"], w["Z"], nrad, order)
+    P, W, Ow = P[::stride], W[::stride], Ow[::stride]
+    print(f"{name}: {len(W)} points, {len(w['Z'])} atoms", flush=True)
+    a = dev(W)
+    b = dev(W)
+    C, Pd, Od = dev(w["xyz"]), dev(P), dev(Ow, torch.int32)
+    nk.partition_becke(ctx, C, Pd, Od, a, capped=True, screen=True)
+    nk.partition_becke(ctx, C, Pd, Od, b, capped=True, screen=False)
+    ga, gb = a.cpu().numpy(), b.cpu().numpy()
+    assert np.array_equal(ga.view(np.int64), gb.view(np.int64))
+    sel = np.random.default_rng(0).choice(len(W), size=min(len(W), 20000), replace=False)
+    ref = O.becke(w["xyz"], P[sel], Ow[sel], W[sel], capped=True)
+    assert (np.abs(ga[sel] - ref) / W[sel]).max() < 1e-14
+
+
 def test_becke_partition_of_unity_gpu(ctx):
     """test_partition.cxx:197-258 on the GPU kernel."""
     centers = np.array([[i, i % 2, i % 3] for i in range(10)], float)
@@ -124,32 +148,61 @@ def test_h2plus_grid_and_overlap_gpu(ctx, nrad, s_ref):
     ctx.set_basis(nk.make_manual_basis([(1, 0, 0, 1.0, 0, 0, 0), (1, 0, 0, 1.0, 1.0, 0, 0)]))
     S = ctx.overlap().cpu().numpy()
     assert abs(S[0, 1] - s_ref) < 5e-13
-    P, W, _ = O.make_flat_grid(H2, [1, 1], nrad, lebedev(59), scheme=0)
-    assert abs(len(W) - grid.weights.shape[0]) <= 16
+    P, W, Ow = O.make_flat_grid(H2, [1, 1], nrad, nk.lebedev_rule(59), scheme=0)
+    _same_grid_but_band(grid, H2, [1, 1], nrad, 59, f"H2+ {nrad} x 59")
     assert abs(grid.weights.sum().item() - W.sum()) < 1e-10 * W.sum()
 
 
-def test_make_flat_grid_matches_oracle(ctx):
-    w = configs.workload("water")
-    grid = ctx.make_flat_grid(w["xyz"], w["Z"], nrad=40, nang_order=29)
-    P, W, Ow = O.make_flat_grid(w["xyz"], w["Z"], 40, lebedev(29), scheme=0)
-    from scipy.spatial import cKDTree
+def _same_grid_but_band(grid, xyz, Z, nrad, order, tag):
+    """The device grid against the oracle's on the library's Lebedev table
+    (same rule, same point order).  The product points are formed in one
+    rounding order on both sides, so they agree bit for bit; the partition
+    agrees to rounding, and the two compacted sets agree except for points
+    whose partition fraction lies in the band where 1 - p(p(p(mu))) has no
+    digits left (the last bit of a distance or quotient decides between an
+    exact zero and a weight ~1e-16 of the raw one).  Every point kept by
+    only one side must carry less than 1e-14 of its raw weight."""
+    ang = nk.lebedev_rule(order)
+    xyz = np.asarray(xyz, float)
+    raw_p, raw_w = [], []
+    for a, z in enumerate(Z):
+        r, wr = O.radial(0, nrad, O.radial_scale(0, int(z)))
+        raw_p.append(xyz[a] + r[:, None, None] * ang[None, :, :3])
+        raw_w.append(wr[:, None] * ang[None, :, 3])
+    raw_p = np.concatenate([x.reshape(-1, 3) for x in raw_p])
+    raw_w = np.concatenate([x.reshape(-1) for x in raw_w])
+    Pa, Wa, Oa = O.make_flat_grid(xyz, Z, nrad, ang, scheme=0, w_thr=-1.0)
+    assert np.array_equal(Pa, raw_p)
+    keep_o = Wa > 1e-30
     gp = grid.quad_points.cpu().numpy()
     gw = grid.weights.cpu().numpy()
     go = grid.point_owner.cpu().numpy()
-    # Same points up to the ordering of the angular rule: match nearest points.
-    # Points kept by only one side carry noise-level weights: their partition
-    # fraction is rounding noise in 1 - p(p(p(mu))) (~1e-16 of the raw weight).
-    noise = 1e-15 * W.max()
-    d, idx = cKDTree(P).query(gp)
-    assert np.all(gw[d > 1e-12] < noise)
-    ok = d <= 1e-12
-    assert np.array_equal(Ow[idx[ok]], go[ok])
-    # partition fractions agree to ~1e-16 absolute (see test_becke_matches_oracle)
-    assert np.all(np.abs(gw[ok] - W[idx[ok]]) <= 1e-11 * W[idx[ok]] + 1e-14 * W.max())
-    d2, _ = cKDTree(gp).query(P)
-    assert np.all(W[d2 > 1e-12] < noise)
-    assert abs(gw.sum() - W.sum()) < 1e-12 * W.sum()
+    # the device points are a subsequence of the raw points, bit for bit
+    pos = {row.tobytes(): i for i, row in enumerate(raw_p)}
+    idx = np.array([pos[row.tobytes()] for row in gp], np.int64)
+    assert np.all(np.diff(idx) > 0)
+    keep_d = np.zeros(len(raw_p), bool)
+    keep_d[idx] = True
+    only_o = keep_o & ~keep_d
+    only_d = keep_d & ~keep_o
+    print(f"{tag}: device {len(gp)} points, oracle {keep_o.sum()}; kept by the oracle only "
+          f"{only_o.sum()}, by the device only {only_d.sum()}")
+    assert np.array_equal(go, Oa[idx])
+    assert np.all(Wa[only_o] < 1e-14 * raw_w[only_o])
+    dev_w = np.zeros(len(raw_p))
+    dev_w[idx] = gw
+    assert np.all(dev_w[only_d] < 1e-14 * raw_w[only_d])
+    both = keep_o & keep_d
+    assert np.all(np.abs(dev_w[both] - Wa[both]) <= 1e-12 * Wa[both] + 1e-15 * raw_w[both])
+
+
+@pytest.mark.parametrize("name,nrad,order", [("water", 40, 29), ("water", 75, 29),
+                                             ("benzene", 99, 41)])
+def test_make_flat_grid_matches_oracle(ctx, name, nrad, order):
+    """make_flat_grid on the device against the oracle's host loop."""
+    w = configs.workload(name)
+    grid = ctx.make_flat_grid(w["xyz"], w["Z"], nrad=nrad, nang_order=order)
+    _same_grid_but_band(grid, w["xyz"], w["Z"], nrad, order, f"{name} {nrad} x {order}")
 
 
 # ------------------------------------------------------------ collocation ----

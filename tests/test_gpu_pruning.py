@@ -54,8 +54,11 @@ def test_robust_middle_shells_use_base_order(ctx, order, npts_med):
     full, low, med = (nk.lebedev_rule(o) for o in nk.pruning_orders(order, "robust"))
     assert len(med) == npts_med
     P0, W0, O0 = O.make_flat_grid(xyz, Z, 30, full, pruning=2, ang_low=low, ang_med=med)
+    own = g.point_owner.cpu().numpy()
+    print("per-atom points: library", np.bincount(own, minlength=3), "oracle",
+          np.bincount(O0, minlength=3))
     assert g.weights.shape[0] == len(W0)
-    assert np.array_equal(g.point_owner.cpu().numpy(), O0)
+    assert np.array_equal(own, O0)
 
 
 def test_robust_negative_weight_order_raises(ctx):
