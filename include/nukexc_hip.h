@@ -149,6 +149,16 @@ enum { NKX_BECKE_PLAIN = 0, NKX_BECKE_CAPPED = 1, NKX_BECKE_CAPPED_UNSCREENED = 
 int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers_dev, int64_t npts,
               const double *points_dev, const int *owner_dev, double *weights_dev,
               int capped);
+/* nkx_becke without any device allocation (no counterpart in the reference):
+   for callers that cannot allocate -- code under HIP graph capture, serving
+   loops, harnesses that forbid hipMalloc.  Same arguments and results (bit
+   for bit) as nkx_becke; its inter-atomic distance table lives in a fixed
+   per-GPU device array, so natoms <= 512 (larger is NKX_ERR_INVALID_ARGUMENT).
+   Every call on one GPU shares that array: calls that overlap on one GPU
+   (other streams or contexts) race.  Stream-ordered. */
+int nkx_becke_noalloc(nkx_ctx *ctx, int natoms, const double *centers_dev, int64_t npts,
+                      const double *points_dev, const int *owner_dev, double *weights_dev,
+                      int capped);
 
 /* ------------------------------------------------------------------ basis */
 /* make_manual_basis normalisation and compute_cutoff radius

@@ -56,6 +56,7 @@ SIGNATURES = {
     "nkx_grid_npts": [_vp, _pi64],
     "nkx_grid_export": [_vp, _vp, _vp, _vp],
     "nkx_becke": [_vp, _c_int, _vp, _c_i64, _vp, _vp, _vp, _c_int],
+    "nkx_becke_noalloc": [_vp, _c_int, _vp, _c_i64, _vp, _vp, _vp, _c_int],
     "nkx_sto_norm_cutoff": [_c_int, _c_double, _c_double, _pd, _pd],
     "nkx_set_basis": [_vp, _c_int, _pi, _pi, _pi, _pd, _pd, _pd, _pd],
     "nkx_basis_nbf": [_vp, _pi],

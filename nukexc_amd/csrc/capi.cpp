@@ -107,6 +107,7 @@ int nkx_create(int device, void *hip_stream, nkx_ctx **out) {
     NKX_HIP_CHECK(hipGetDeviceCount(&n));
     NKX_REQUIRE(device >= 0 && device < n, NKX_ERR_INVALID_ARGUMENT, "no such device");
     NKX_HIP_CHECK(hipSetDevice(device));
+    nkx::becke_scratch_init(device);
     ctx = new nkx_ctx();
     ctx->c.device = device;
     if (hip_stream) {
@@ -527,6 +528,24 @@ int nkx_becke(nkx_ctx *ctx, int natoms, const double *centers, int64_t npts, con
     ctx->c.becke_rtab.reserve((size_t)natoms * natoms);
     nkx::launch_becke(ctx->c.stream, natoms, centers, npts, pts, owner, w, mode,
                       ctx->c.becke_rtab.p);
+  });
+}
+
+int nkx_becke_noalloc(nkx_ctx *ctx, int natoms, const double *centers, int64_t npts,
+                      const double *pts, const int *owner, double *w, int capped) {
+  NKX_CTX_CHECK(ctx);
+  return guarded(ctx, [&] {
+    NKX_REQUIRE(natoms > 0 && centers && (npts == 0 || (pts && owner && w)),
+                NKX_ERR_INVALID_ARGUMENT, "becke_noalloc: bad arguments");
+    NKX_REQUIRE(natoms <= nkx::BECKE_SCRATCH_ATOMS, NKX_ERR_INVALID_ARGUMENT,
+                "becke_noalloc: at most 512 atoms (its fixed table holds 512 x 512 "
+                "inter-atomic distances); use nkx_becke for more");
+    double *rt = nkx::becke_scratch(ctx->c.device);
+    NKX_REQUIRE(rt, NKX_ERR_STATE, "becke_noalloc: scratch table not resolved on this device");
+    const int mode = capped == NKX_BECKE_CAPPED_UNSCREENED
+                         ? nkx::BECKE_CAPPED | nkx::BECKE_NO_SCREEN
+                         : (capped ? nkx::BECKE_CAPPED : 0);
+    nkx::launch_becke(ctx->c.stream, natoms, centers, npts, pts, owner, w, mode, rt);
   });
 }
 

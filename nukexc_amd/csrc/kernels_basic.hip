@@ -1,6 +1,7 @@
 // Grid partition, compaction, reference-shaped collocation / density / XC.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "nkx_internal.hpp"
@@ -179,6 +180,37 @@ void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const dou
   hipLaunchKernelGGL(grid_shells_kernel, dim3((unsigned)nshell), dim3(256), 0, s, sh_atom, sh_r,
                      sh_w, sh_rule, sh_poff, rule_off, rule_n, xyz, ang, P, W, O);
   NKX_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Workspace of the allocation-free Becke partition (nkx_becke_noalloc): the
+// pair-distance table in a file-scope device array, BECKE_SCRATCH_ATOMS^2
+// doubles (2 MiB), one copy per GPU held until the process ends.  Host code
+// never passes the array's name to a kernel: its device address is resolved
+// once per device (becke_scratch_init, from nkx_create with that device
+// current) and kept per device.
+static __device__ __align__(256) double scratch[BECKE_SCRATCH_ATOMS * BECKE_SCRATCH_ATOMS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void becke_scratch_init(int device) {
+  NKX_REQUIRE(device >= 0 && device < kMaxDevices, NKX_ERR_INVALID_ARGUMENT,
+              "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  NKX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double *>(p);
+}
+
+double *becke_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,

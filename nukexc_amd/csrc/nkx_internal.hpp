@@ -273,6 +273,12 @@ void launch_grid_shells(hipStream_t s, int nshell, const int *sh_atom, const dou
 constexpr int BECKE_CAPPED = 1, BECKE_NO_SCREEN = 2;
 void launch_becke(hipStream_t s, int natoms, const double *centers, int64_t npts,
                   const double *pts, const int *owner, double *w, int mode, double *rtab);
+// The allocation-free partition's table: at most BECKE_SCRATCH_ATOMS atoms.
+// becke_scratch_init resolves the device array's address on the current
+// device (once per device); becke_scratch returns it (NULL before that).
+constexpr int BECKE_SCRATCH_ATOMS = 512;
+void becke_scratch_init(int device);
+double *becke_scratch(int device);
 int64_t compact_points(hipStream_t s, int64_t n, double thr, const double *pts_in,
                        const double *w_in, const int *own_in, double *pts_out,
                        double *w_out, int *own_out, int *scratch_flags,
