@@ -40,16 +40,7 @@ def kernel_name(precision, stage, shape):
         return {"density": "density_fused_mixed", "vgemm": "vgemm_mixed",
                 "collocation": "colloc_tiles_mixed"}[stage]
     if stage == "density":
-        shape = shape or 65
-        if shape == 129:
-            return "density_x4_i8"
-        if shape == 67:
-            return "density_x22_i8"
-        if shape in (65, 66, 71, 72):
-            return "density_x2_i8"
-        if shape & 32:
-            return "density_x_i8"
-        return "density_fused_i8"
+        return "density_x2_i8"
     return {"vgemm": "vgemm_i8", "collocation": "colloc_tiles_i8"}[stage]
 
 
@@ -232,7 +223,9 @@ def molset_main(args):
         "config": {"workload": f"molset64: 64 molecules of 3-60 atoms, {npts} points, PBE",
                    "precision": args.precision, "sequential_ms": seq_ms,
                    "sequential_default_launch_ms": seq_default_ms,
-                   "autotuned": {"vgemm_split": tuned[0], "density_group": tuned[1]},
+                   "autotuned": {"vgemm_split": tuned[0], "density_group": tuned[1],
+                                 "points_per_density_workgroup":
+                                     cs.launch_points(args.precision, tuned[1])},
                    "set_build_ms": set_ms, "graph_replay_ms": graph_ms,
                    "speedup_vs_sequential": seq_ms / graph_ms,
                    "max_dE_vs_fp64": dE, "max_dN_vs_fp64": dN, "max_dV_vs_fp64": dV},
@@ -514,7 +507,8 @@ def main():
             fp64_tuned = None
             if not args.no_autotune:  # the FP64 form at its own best launch parameters
                 k, g, ms = ctx.autotune(P, funcs, precision="fp64")
-                fp64_tuned = {"vgemm_split": k, "density_group": g}
+                fp64_tuned = {"vgemm_split": k, "density_group": g,
+                              "points_per_density_workgroup": ctx.launch_points("fp64", g)}
                 log(f"FP64 autotune: V split {k}, density group {g}, {ms:.1f} ms/build")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -533,8 +527,10 @@ def main():
         log(f"launch parameters: V split {k}, density shape {g}")
     elif not args.no_autotune:
         k, g, ms = ctx.autotune(P, funcs, precision=args.precision)
-        tuned = {"vgemm_split": k, "density_group": g, "ms": ms}
-        log(f"autotune: V split {k}, density group {g}, {ms:.1f} ms/build")
+        tuned = {"vgemm_split": k, "density_group": g, "ms": ms,
+                 "points_per_density_workgroup": ctx.launch_points(args.precision, g)}
+        log(f"autotune: V split {k}, density group {g} "
+            f"({tuned['points_per_density_workgroup']} points per workgroup), {ms:.1f} ms/build")
 
     for i in range(args.warmup):
         r = ctx.vxc(P, funcs, precision=args.precision)

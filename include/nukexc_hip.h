@@ -363,15 +363,11 @@ int nkx_molset_graph_capture(nkx_ctx *ctx, const double *const *P_dev, int nfunc
 int nkx_graph_launch(nkx_ctx *ctx, double *E_xc, double *N_el);
 /* Autotuner (no counterpart in the reference): on the ctx grid and basis for
    this P, times one nkx_vxc build per candidate -- first the density
-   workgroup shape (FP64: 1, 2 or 4 point batches per workgroup; int8 form:
-   1 or 2 batches, plus 16 for two function blocks per step, plus 32 for the
-   split form, where contraction, functional and Z are three kernels and
-   the workgroups of one batch group share an XCD; 65 and 66 for the split
-   form with 32 rows per wave; 71 and 72 for 65 on absolute block pairs dealt
-   to XCDs in tiles of 4 x 16 / 8 x 8 (pairs x batches); 67 for two batches
-   x two blocks per workgroup with both operands in LDS; 129 for the split
-   form over quads of function blocks, one batch and eight waves per
-   workgroup; the autotuner times 65, 71, 67 and 129), then the split
+   workgroup shape (FP64: 1, 2 or 4 point batches of 64 points per
+   workgroup; int8 form: 65 and 66 for one and two batches per workgroup,
+   the workgroups of one batch group on one XCD; 71 and 72 for one batch on
+   absolute block pairs dealt to XCDs in tiles of 4 x 16 / 8 x 8 (pairs x
+   batches); the autotuner times 1, 2, 4 or 65, 71), then the split
    count of the V contraction over batches (the automatic choice x 1/4 .. 4)
    -- keeps the fastest for later builds and captures, and returns them
    (either pointer may be NULL) with the best build time.  V receives the
@@ -390,9 +386,8 @@ int nkx_molset_autotune(nkx_ctx *ctx, const double *const *P_dev, int nfunc,
 /* Launch parameters (no counterpart in the reference): set or read the
    values the autotuner picks, e.g. to reuse a tuned pair without timing
    again.  vgemm_split 0 = automatic; density_shape 0 = default, else one of
-   1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50, 65, 66, 67, 71, 72, 129 (int8
-   form); the FP64 form reads the int8 codes modulo 16, the int8 form reads
-   4 as 65. */
+   1, 2, 4 (FP64) or 65, 66, 71, 72 (int8 form); the FP64 form reads 65 and
+   71 as 1 and 66 as 2, the int8 form reads 1, 2 and 4 as 65. */
 int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
 /* Device-memory budget of a build's tile store in bytes (no counterpart in
    the reference): a build whose tiles do not fit runs in chunks of batches.
@@ -400,6 +395,10 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape);
    below 64 MB count as 64 MB.  For many contexts on one GPU. */
 int nkx_set_memory_budget(nkx_ctx *ctx, int64_t bytes);
 int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape);
+/* Grid points per density workgroup (64, 128 or 256: the point-batch size
+   the autotuner chose) that a build of the given precision runs with this
+   density shape (0 = the default shape); no counterpart in the reference. */
+int nkx_launch_points(int precision, int density_shape, int *points);
 /* Screening tolerance on |Phi| for the batch function lists (default 1e-12)
    and the point-batch size: 64 is the only size (the kernels' tiles are 64
    points x 64 functions); 0 keeps it, any other value is

@@ -84,6 +84,7 @@ SIGNATURES = {
     "nkx_set_launch": [_vp, _c_int, _c_int],
     "nkx_set_memory_budget": [_vp, _c_i64],
     "nkx_get_launch": [_vp, _pi, _pi],
+    "nkx_launch_points": [_c_int, _c_int, _pi],
     "nkx_graph_capture": [_vp, _vp, _c_int, _pi, _c_int, _vp],
     "nkx_graph_capture_polarized": [_vp, _vp, _vp, _c_int, _pi, _c_int, _vp, _vp],
     "nkx_graph_launch": [_vp, _pd, _pd],

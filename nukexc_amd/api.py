@@ -373,8 +373,8 @@ class Context:
 
     def set_launch(self, vgemm_split=0, density_shape=0):
         """Set the launch parameters the autotuner picks (0 = automatic /
-        default); density shapes 1, 2, 4 (FP64) or 1, 2, 17, 18, 33, 34, 49, 50,
-        65, 66, 67, 71, 72, 129 (int8 form)."""
+        default); density shapes 1, 2, 4 (FP64) or 65, 66, 71, 72 (int8
+        form)."""
         self._chk(lib().nkx_set_launch(self.h, int(vgemm_split), int(density_shape)))
 
     def set_memory_budget(self, nbytes=0):
@@ -387,6 +387,16 @@ class Context:
         k, g = ctypes.c_int(), ctypes.c_int()
         self._chk(lib().nkx_get_launch(self.h, ctypes.byref(k), ctypes.byref(g)))
         return k.value, g.value
+
+    @staticmethod
+    def launch_points(precision, density_shape):
+        """Grid points per density workgroup (the point-batch size) of a
+        build at this precision and density shape."""
+        prec = {"fp64": _capi.PREC_FP64, "mixed": _capi.PREC_MIXED,
+                "mixed_bf16": _capi.PREC_MIXED_BF16}[precision]
+        n = ctypes.c_int()
+        check(lib().nkx_launch_points(prec, int(density_shape), ctypes.byref(n)))
+        return n.value
 
     # -- HIP graph of one build ----------------------------------------------
     def graph_capture(self, P, V, funcs=PBE, precision="fp64"):

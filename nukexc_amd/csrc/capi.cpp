@@ -985,9 +985,8 @@ int nkx_set_launch(nkx_ctx *ctx, int vgemm_split, int density_shape) {
   return guarded(ctx, [&] {
     NKX_REQUIRE(vgemm_split >= 0, NKX_ERR_INVALID_ARGUMENT, "vgemm_split must be >= 0");
     const int sh = density_shape;
-    NKX_REQUIRE(sh == 0 || sh == 4 || nkx::mixed_density_shape_ok(sh), NKX_ERR_INVALID_ARGUMENT,
-                "density shape must be 0, 1, 2, 4, 17, 18, 33, 34, 49, 50, 65, 66, 67, 71, 72 "
-                "or 129");
+    NKX_REQUIRE(sh == 0 || sh == 1 || sh == 2 || sh == 4 || nkx::mixed_density_shape_ok(sh),
+                NKX_ERR_INVALID_ARGUMENT, "density shape must be 0, 1, 2, 4, 65, 66, 71 or 72");
     ctx->c.opt.vgemm_split = vgemm_split;
     ctx->c.opt.density_group = density_shape;
   });
@@ -1006,6 +1005,21 @@ int nkx_get_launch(nkx_ctx *ctx, int *vgemm_split, int *density_shape) {
   return guarded(ctx, [&] {
     if (vgemm_split) *vgemm_split = ctx->c.opt.vgemm_split;
     if (density_shape) *density_shape = ctx->c.opt.density_group;
+  });
+}
+
+int nkx_launch_points(int precision, int density_shape, int *points) {
+  return guarded(nullptr, [&] {
+    NKX_REQUIRE(points, NKX_ERR_INVALID_ARGUMENT, "points is NULL");
+    NKX_REQUIRE(precision >= NKX_PREC_FP64 && precision <= NKX_PREC_MIXED_BF16,
+                NKX_ERR_INVALID_ARGUMENT, "unknown precision");
+    const int sh = density_shape;
+    int batches = 1;
+    if (precision == NKX_PREC_FP64)  // 1, 2, 4 batches; the int8 codes as the build reads them
+      batches = sh == 66 ? 2 : (sh >= 65 ? 1 : (sh == 1 || sh == 2 ? sh : (sh == 0 ? 2 : 4)));
+    else if (precision == NKX_PREC_MIXED)
+      batches = sh == 66 ? 2 : 1;
+    *points = 64 * batches;
   });
 }
 

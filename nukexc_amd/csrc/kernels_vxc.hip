@@ -509,7 +509,7 @@ void vk_density(hipStream_t s, int nspin, int group, int64_t b0, int64_t b1, int
   if (b1 <= b0) return;
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
-  const int g0 = group & 15;  // the int8 form's shape codes add 16
+  const int g0 = group == 66 ? 2 : (group >= 65 ? 1 : group);  // the int8 form's codes
   const int g = g0 == 1 || g0 == 2 ? g0 : 4;
 #define NKX_DENS(NS, G) launch_density<NS, G>(s, b0, b1, s0, pl, Pd, narr, gga, tiles, xl, coef, blockEN, epoint, rhopt)
   if (nspin == 2) {

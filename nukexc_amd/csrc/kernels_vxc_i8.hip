@@ -123,529 +123,24 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 }
 
 
-// Density contraction, functional and Z.  A workgroup of 4 MB waves handles
-// GBT point batches.  Its steps run over (m group, k): MB consecutive entries
-// m of the union list of live function blocks, k over the same list in
-// ascending order; a step does the tile products P[m][k] Phi_k for every
-// batch of the group in which both blocks are live.  The Pm sections of
-// block k for the group's batches are staged in LDS (two buffers): the next
-// active step's sections are copied into the other buffer while the current
-// one computes, one barrier per step.
-// Wave w works on block m = w / 4 of the group, rows 16 (w % 4) .. +15; its
-// P fragment is loaded into registers one step ahead.
-template <int GBT, int MB>
-__global__ void __launch_bounds__(256 * MB) density_fused_i8(
-    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int gga,
-    signed char *__restrict__ store, const double *__restrict__ pw, XcList xl,
-    double *__restrict__ coef, double *__restrict__ blockEN) {
-  constexpr int NT = 256 * MB, NW = 4 * MB;
-  const int64_t bg = b0 + (int64_t)blockIdx.x * GBT;
-  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int mb = wv >> 2, rw = wv & 3;
-  const int l16 = lane & 15, q = lane >> 4;
-  const int64_t sb_bytes = slot_bytes_i8(gga);
-
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
-  __shared__ double red[NW][4][PT];
-  __shared__ double colsum[GBT][4][PT];
-  __shared__ double cf[GBT][4][PT];
-  __shared__ int ez[FB];
-  __shared__ int tab[GBT][MAXNFB];
-  __shared__ int ublk[MAXNFB];
-  __shared__ int nu_s;
-  static_assert(sizeof(bbuf) >= sizeof(double) * FB * 65, "zt alias");
-  double(*zt)[65] = reinterpret_cast<double(*)[65]>(&bbuf[0][0][0]);
-
-  for (int e = threadIdx.x; e < GBT * nfb; e += NT) tab[e / nfb][e % nfb] = -1;
-  for (int e = threadIdx.x; e < GBT * 4 * PT; e += NT) (&colsum[0][0][0])[e] = 0.0;
-  __syncthreads();
-  for (int sb = 0; sb < nsub; ++sb) {
-    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
-    for (int k = lo + (int)threadIdx.x; k < hi; k += NT) tab[sb][b_fbl[k]] = k;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nfb; base += 64) {
-      const int f = base + lane;
-      bool live = false;
-      if (f < nfb)
-        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
-      const unsigned long long m = __ballot(live);
-      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
-      cnt += __popcll(m);
-    }
-    if (lane == 0) nu_s = cnt;
-  }
-  __syncthreads();
-  const int nu = nu_s;
-  const int nq = gga ? 4 : 1;
-  const int ngrp = (nu + MB - 1) / MB;
-  const int total = ngrp * nu;
-
-  // bit (m * GBT + sb): blocks m (of the group) and k both live in batch sb
-  auto pairmask = [&](int st) -> unsigned {
-    const int mg = st / nu, fk = ublk[st - mg * nu];
-    unsigned pm = 0;
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      const int mi = mg * MB + m;
-      if (mi < nu) {
-        const int fm = ublk[mi];
-#pragma unroll
-        for (int sb = 0; sb < GBT; ++sb)
-          if (sb < nsub && tab[sb][fk] >= 0 && tab[sb][fm] >= 0) pm |= 1u << (m * GBT + sb);
-      }
-    }
-    return pm;
-  };
-  auto next_active = [&](int st, unsigned &pm) -> int {
-    pm = 0;
-    while (st < total) {
-      pm = pairmask(st);
-      if (pm) break;
-      ++st;
-    }
-    return st;
-  };
-  auto colbit = [](unsigned pm, int sb) -> bool {
-    unsigned c = 0;
-#pragma unroll
-    for (int m = 0; m < MB; ++m) c |= pm >> (m * GBT + sb);
-    return c & 1u;
-  };
-  constexpr unsigned ROW = (1u << GBT) - 1u;
-
-  v4i An[4];
-  int eAn = 0;
-  // Pm sections of block k for the live batches -> LDS buffer dbuf (direct
-  // copies), and this wave's P fragment of block pair (m, k) -> registers
-  auto issue = [&](int st, unsigned pm, int dbuf) {
-    const int mg = st / nu, fk = ublk[st - mg * nu];
-    // the P fragment first: its address needs table loads, and waiting for
-    // them must not wait for the section copies as well
-    if ((pm >> (mb * GBT)) & ROW) {
-      const int fm = ublk[mg * MB + mb];
-      const int mol = mt.blk_mol[fm];
-      const signed char *pa =
-          Ps + mt.Ps_off[mol] +
-          ((int64_t)mt.blk_local[fm] * mt.nfb[mol] + mt.blk_local[fk]) * SEC;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
-      eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
-    }
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (!colbit(pm, sb)) continue;
-      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
-                  (wv + 5 * sb) % NW, NW, lane);
-    }
-  };
-
-  unsigned pmc = 0;
-  int cur = next_active(0, pmc);
-  if (cur < total) issue(cur, pmc, 0);
-  int buf = 0;
-  for (int mg = 0; mg < ngrp; ++mg) {
-    double x64[GBT][4][4];
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) x64[sb][j][r] = 0.0;
-    while (cur < total && cur / nu == mg) {
-      wait_vm0();
-      __syncthreads();  // buffer buf complete, buf ^ 1 free
-      v4i A[4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p) A[p] = An[p];
-      const int eAc = eAn;
-      unsigned pmn = 0;
-      const int nxt = next_active(cur + 1, pmn);
-      if (nxt < total) issue(nxt, pmn, buf ^ 1);
-      double sA[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sA[r] = pow2_or_zero((int)(signed char)(eAc >> (8 * r)));
-#pragma unroll
-      for (int sb = 0; sb < GBT; ++sb) {
-        if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
-        const signed char *bb = &bbuf[buf][sb][0];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          v4i B[4];
-#pragma unroll
-          for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-          const int eB = bb[4 * PLANE + 16 * j + l16];
-          const double sB = eB == -127 ? 0.0 : pow2(eB - 19);
-          int hi[4], lo[4];
-          tile_product(A, B, hi, lo);
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            x64[sb][j][r] =
-                fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[r] * sB, x64[sb][j][r]);
-        }
-      }
-      buf ^= 1;
-      cur = nxt;
-      pmc = pmn;
-    }
-    // Epilogue in FP64: column partial sums of Phi X (and grad Phi X) of
-    // every block of the group, summed over waves in a fixed order.
-    const int mi = mg * MB + mb;
-    const int fm = mi < nu ? ublk[mi] : -1;
-    for (int sb = 0; sb < nsub; ++sb) {
-      const int slm = fm >= 0 ? tab[sb][fm] : -1;
-      double part[4][4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) part[j][c] = 0.0;
-      if (slm >= 0) {
-        const signed char *tb = store + (slm - s0) * sb_bytes;
-        const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int g = 16 * j + l16;
-          double sm[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int mu = 16 * rw + 4 * q + r;
-            double x = 0.0;
-#pragma unroll
-            for (int s2 = 0; s2 < GBT; ++s2)
-              if (s2 == sb) x = x64[s2][j][r];
-            sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
-            if (gga) {
-              sm[1] += (double)G[mu * PT + g] * x;
-              sm[2] += (double)G[TILE + mu * PT + g] * x;
-              sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            double v = sm[c];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            part[j][c] = v;
-          }
-        }
-      }
-      if (q == 0) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = part[j][c];
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < 4 * PT; e += NT) {
-        const int c = e / PT, g = e % PT;
-        if (c < nq) {
-          double acc = red[0][c][g];
-#pragma unroll
-          for (int w = 1; w < NW; ++w) acc += red[w][c][g];
-          colsum[sb][c][g] += acc;
-        }
-      }
-      __syncthreads();
-    }
-  }
-
-  // Functional and Z coefficients: wave sb handles the points of batch sb.
-  if (wv < nsub) {
-    const int sb = wv, g = lane;
-    const int64_t b = bg + sb;
-    const int64_t gp = b * PT + g;
-    const double w = pw[gp];
-    const double rho = colsum[sb][0][g];
-    const double gxr = 2.0 * colsum[sb][1][g], gyr = 2.0 * colsum[sb][2][g],
-                 gzr = 2.0 * colsum[sb][3][g];
-    const double sigma = gga ? gxr * gxr + gyr * gyr + gzr * gzr : 0.0;
-    const XcPoint o = xc_eval(xl.n, xl.ids, xl.thr, rho, sigma);
-    const double a = 0.5 * w * o.vrho;
-    const double bs = gga ? 2.0 * w * o.vsigma : 0.0;
-    cf[sb][0][g] = a;
-    cf[sb][1][g] = bs * gxr;
-    cf[sb][2][g] = bs * gyr;
-    cf[sb][3][g] = bs * gzr;
-    coef[4 * gp + 0] = a;
-    coef[4 * gp + 1] = bs * gxr;
-    coef[4 * gp + 2] = bs * gyr;
-    coef[4 * gp + 3] = bs * gzr;
-    double e = w * o.e, n = w * rho;
-    for (int o2 = 32; o2 > 0; o2 >>= 1) {
-      e += __shfl_down(e, o2);
-      n += __shfl_down(n, o2);
-    }
-    if (g == 0) {  // (E, N_alpha, N_beta) per batch, N_beta = 0 here
-      blockEN[3 * b] = e;
-      blockEN[3 * b + 1] = n;
-      blockEN[3 * b + 2] = 0.0;
-    }
-  }
-  __syncthreads();
-
-  // Z = a Phi + b . grad Phi in FP64, per-function exponents, slices (F order).
-  for (int mi = 0; mi < nu; ++mi) {
-    const int fm = ublk[mi];
-    for (int sb = 0; sb < nsub; ++sb) {
-      const int slm = tab[sb][fm];
-      if (slm < 0) continue;
-      signed char *tb = store + (slm - s0) * sb_bytes;
-      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-      for (int e = threadIdx.x; e < TILE; e += NT) {
-        const int f = e / PT, g = e % PT;
-        const double ph = unslice(tb + OFF_F, frag8(f, g), tb[OFF_F + 4 * PLANE + f]);
-        zt[f][g] = gga ? z_value(ph, (double)G[e], (double)G[TILE + e], (double)G[2 * TILE + e],
-                                 cf[sb][0][g], cf[sb][1][g], cf[sb][2][g], cf[sb][3][g])
-                       : cf[sb][0][g] * ph;
-      }
-      __syncthreads();
-      if (threadIdx.x < FB) {
-        double m = 0.0;
-        for (int k = 0; k < PT; ++k) m = fmax(m, fabs(zt[threadIdx.x][k]));
-        ez[threadIdx.x] = row_exp(m);
-      }
-      __syncthreads();
-      if (threadIdx.x < 256) store_section(zt, ez, tb + OFF_Z, false);
-      __syncthreads();
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Split form of the density stage: contraction (density_x_i8), reduction and
-// functional (density_f_i8), Z slices (z_slots_i8).
+// Density stage, three kernels: the contraction (density_x2_i8), per batch
+// the reduction and the functional (density_f_i8), the Z slices (z_slots_i8).
 //
-// density_x_i8: one workgroup of 4 MB waves per (batch group g of GBT
-// batches, m group i of MB blocks of the group's union list).  Workgroup w
-// takes g = 8 (w / 8 / nmg) + w % 8 and i = (w / 8) % nmg, so the m groups of
-// one batch group run on one XCD (workgroups are dealt to the eight XCDs in
-// turn) and share its L2 for the Pm sections.  The step loop is the fused
-// kernel's over k for one m group; the epilogue writes, per live slot
-// (m, batch), the column sums of Phi X (and grad Phi X) over the block's 64
-// functions, summed over the four waves of the block in order, to part.
-template <int GBT, int MB>
-__global__ void __launch_bounds__(256 * MB) density_x_i8(
-    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
-    const signed char *__restrict__ store, double *__restrict__ part) {
-  constexpr int NT = 256 * MB, NW = 4 * MB;
-  const int xcd = blockIdx.x & 7;
-  const int rr = blockIdx.x >> 3;
-  const int mgi = rr % nmg;
-  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * GBT;
-  if (bg >= b1) return;
-  const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int mb = wv >> 2, rw = wv & 3;
-  const int l16 = lane & 15, q = lane >> 4;
-  const int64_t sb_bytes = slot_bytes_i8(gga);
-
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
-  __shared__ double red[NW][4][PT];
-  __shared__ int tab[GBT][MAXNFB];
-  __shared__ int ublk[MAXNFB];
-  __shared__ int nu_s;
-
-  for (int e = threadIdx.x; e < GBT * nfb; e += NT) tab[e / nfb][e % nfb] = -1;
-  __syncthreads();
-  for (int sb = 0; sb < nsub; ++sb) {
-    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
-    for (int k = lo + (int)threadIdx.x; k < hi; k += NT) tab[sb][b_fbl[k]] = k;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nfb; base += 64) {
-      const int f = base + lane;
-      bool live = false;
-      if (f < nfb)
-        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
-      const unsigned long long m = __ballot(live);
-      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
-      cnt += __popcll(m);
-    }
-    if (lane == 0) nu_s = cnt;
-  }
-  __syncthreads();
-  const int nu = nu_s;
-  if (mgi * MB >= nu) return;  // uniform: no block of this m group
-  const int nq = gga ? 4 : 1;
-
-  // bit (m * GBT + sb): block m of the group and block k = ublk[ki] both live in sb
-  auto pairmask = [&](int ki) -> unsigned {
-    const int fk = ublk[ki];
-    unsigned pm = 0;
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      const int mi = mgi * MB + m;
-      if (mi < nu) {
-        const int fm = ublk[mi];
-#pragma unroll
-        for (int sb = 0; sb < GBT; ++sb)
-          if (sb < nsub && tab[sb][fk] >= 0 && tab[sb][fm] >= 0) pm |= 1u << (m * GBT + sb);
-      }
-    }
-    return pm;
-  };
-  auto next_active = [&](int ki, unsigned &pm) -> int {
-    pm = 0;
-    while (ki < nu) {
-      pm = pairmask(ki);
-      if (pm) break;
-      ++ki;
-    }
-    return ki;
-  };
-  auto colbit = [](unsigned pm, int sb) -> bool {
-    unsigned c = 0;
-#pragma unroll
-    for (int m = 0; m < MB; ++m) c |= pm >> (m * GBT + sb);
-    return c & 1u;
-  };
-  constexpr unsigned ROW = (1u << GBT) - 1u;
-
-  v4i An[4];
-  int eAn = 0;
-  auto issue = [&](int ki, unsigned pm, int dbuf) {
-    const int fk = ublk[ki];
-    // the P fragment first (see density_fused_i8)
-    if ((pm >> (mb * GBT)) & ROW) {
-      const int fm = ublk[mgi * MB + mb];
-      const int mol = mt.blk_mol[fm];
-      const signed char *pa =
-          Ps + mt.Ps_off[mol] +
-          ((int64_t)mt.blk_local[fm] * mt.nfb[mol] + mt.blk_local[fk]) * SEC;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) An[p] = ld16(pa + p * PLANE + (rw * 64 + lane) * 16);
-      eAn = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * rw + 4 * q);
-    }
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (!colbit(pm, sb)) continue;
-      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
-                  (wv + 5 * sb) % NW, NW, lane);
-    }
-  };
-
-  double x64[GBT][4][4];
-#pragma unroll
-  for (int sb = 0; sb < GBT; ++sb)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) x64[sb][j][r] = 0.0;
-  unsigned pmc = 0;
-  int cur = next_active(0, pmc);
-  if (cur < nu) issue(cur, pmc, 0);
-  int buf = 0;
-  while (cur < nu) {
-    wait_vm0();
-    __syncthreads();  // buffer buf complete, buf ^ 1 free
-    v4i A[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) A[p] = An[p];
-    const int eAc = eAn;
-    unsigned pmn = 0;
-    const int nxt = next_active(cur + 1, pmn);
-    if (nxt < nu) issue(nxt, pmn, buf ^ 1);
-    double sA[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) sA[r] = pow2_or_zero((int)(signed char)(eAc >> (8 * r)));
-#pragma unroll
-    for (int sb = 0; sb < GBT; ++sb) {
-      if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
-      const signed char *bb = &bbuf[buf][sb][0];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v4i B[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-        const int eB = bb[4 * PLANE + 16 * j + l16];
-        const double sB = eB == -127 ? 0.0 : pow2(eB - 19);
-        int hi[4], lo[4];
-        tile_product(A, B, hi, lo);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          x64[sb][j][r] =
-              fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[r] * sB, x64[sb][j][r]);
-      }
-    }
-    buf ^= 1;
-    cur = nxt;
-    pmc = pmn;
-  }
-
-  // Epilogue: per live slot (m, sb), column sums over the block's functions.
-  const int mi = mgi * MB + mb;
-  const int fm = mi < nu ? ublk[mi] : -1;
-  for (int sb = 0; sb < nsub; ++sb) {
-    const int slm = fm >= 0 ? tab[sb][fm] : -1;
-    double prt[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
-    if (slm >= 0) {
-      const signed char *tb = store + (slm - s0) * sb_bytes;
-      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int g = 16 * j + l16;
-        double sm[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 16 * rw + 4 * q + r;
-          double x = 0.0;
-#pragma unroll
-          for (int s2 = 0; s2 < GBT; ++s2)
-            if (s2 == sb) x = x64[s2][j][r];
-          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
-          if (gga) {
-            sm[1] += (double)G[mu * PT + g] * x;
-            sm[2] += (double)G[TILE + mu * PT + g] * x;
-            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = sm[c];
-          v += __shfl_xor(v, 16);
-          v += __shfl_xor(v, 32);
-          prt[j][c] = v;
-        }
-      }
-    }
-    if (q == 0) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
-    }
-    __syncthreads();
-    // waves 4 m .. 4 m + 3 hold block m: write its slot's partial sums
-    for (int e = threadIdx.x; e < MB * 4 * PT; e += NT) {
-      const int m = e / (4 * PT), c = (e / PT) % 4, g = e % PT;
-      const int mi2 = mgi * MB + m;
-      if (mi2 >= nu) continue;
-      const int sl = tab[sb][ublk[mi2]];
-      if (sl < 0) continue;
-      double acc = 0.0;
-      if (c < nq) acc = red[4 * m][c][g] + red[4 * m + 1][c][g] + red[4 * m + 2][c][g] +
-                        red[4 * m + 3][c][g];
-      part[(sl - s0) * (4 * PT) + c * PT + g] = acc;
-    }
-    __syncthreads();
-  }
-}
-
-// density_x2_i8: the split form's contraction with two row fragments per
-// wave.  One workgroup of four waves per (batch group of GBT batches, pair
-// of union-list blocks m_0 < m_1); wave w holds rows 32 (w % 2) .. +31 of
-// block m_(w / 2), so each B fragment read from LDS feeds two tile products.
-// The active k of the union list and their masks are listed in LDS once;
-// XCD placement, staging and the per-slot output are those of density_x_i8.
+// density_x2_i8: X = P Phi for a pair of function blocks of a batch group.
+// One workgroup of four waves per (batch group of GBT batches, pair of
+// union-list blocks m_0 < m_1); wave w holds rows 32 (w % 2) .. +31 of block
+// m_(w / 2), so each B fragment read from LDS feeds two tile products.  The
+// active k of the union list and their masks are listed in LDS once; the
+// steps run over them in ascending order, each staging the Pm sections of
+// (k, batch) in LDS (two buffers: the next step's copies run while the
+// current one computes, one barrier per step) and each wave's P fragments in
+// registers one step ahead.  Workgroup w takes batch group 8 (w / 8 / nmg) +
+// w % 8 and pair (w / 8) % nmg, so the pairs of one batch group run on one
+// XCD (workgroups are dealt to the eight XCDs in turn) and share its L2 for
+// the Pm sections.  The epilogue writes, per live slot (m, batch), the column
+// sums of Phi X (and grad Phi X) over the block's 64 functions, summed over
+// the waves of the block in order, to part.
 // ABS (one batch per workgroup): the pair is an absolute pair of the
 // molecule's blocks (2 p, 2 p + 1; a workgroup whose two blocks are both dead
 // in its batch exits), and the workgroups are dealt in tiles of tpn pairs x
@@ -916,420 +411,8 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   }
 }
 
-// density_x22_i8: the split form's contraction for a group of two batches
-// and a pair of union-list blocks m_0 < m_1, eight waves: wave w computes
-// batch w / 4, rows 32 (w % 2) .. +31 of block m_((w / 2) % 2).  Both
-// operands are staged in LDS (two buffers of the P sections (m_0, k),
-// (m_1, k) and the Pm sections (k, b_0), (k, b_1)), so each P section read
-// from memory serves both batches and each Pm section both blocks.  Per
-// wave the work and the sums are those of density_x2_i8 for its batch, and
-// the per-slot output is the same.
-__global__ void __launch_bounds__(512) density_x22_i8(
-    MolTab mt, int nfb, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int gga,
-    const signed char *__restrict__ store, double *__restrict__ part) {
-  const int xcd = blockIdx.x & 7;
-  const int rr = blockIdx.x >> 3;
-  const int mgi = rr % nmg;
-  const int64_t bg = b0 + ((int64_t)(rr / nmg) * 8 + xcd) * 2;
-  if (bg >= b1) return;
-  const int nsub = (int)(b1 - bg < 2 ? b1 - bg : 2);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int sb = wv >> 2, mb = (wv >> 1) & 1, rh = wv & 1;
-  const int l16 = lane & 15, q = lane >> 4;
-  const int64_t sb_bytes = slot_bytes_i8(gga);
-
-  // [buffer][P (m_0, k), P (m_1, k), Pm (k, b_0), Pm (k, b_1)]; the column-sum
-  // exchange of the epilogue reuses the first buffer
-  __shared__ __attribute__((aligned(16))) signed char bufA[4][SEC];
-  __shared__ __attribute__((aligned(16))) signed char bufB[4][SEC];
-  __shared__ short tab[2][MAXNFB];  // slot - b_off[batch], -1 if not live
-  __shared__ short ublk[MAXNFB];
-  __shared__ short sk[MAXNFB];
-  __shared__ unsigned char spm[MAXNFB];
-  __shared__ int nu_s, nst_s;
-  static_assert(sizeof(bufA) >= sizeof(double) * 8 * 4 * PT, "red alias");
-  double(*red)[4][PT] = reinterpret_cast<double(*)[4][PT]>(&bufA[0][0]);
-
-  for (int e = threadIdx.x; e < 2 * nfb; e += 512) tab[e / nfb][e % nfb] = -1;
-  __syncthreads();
-  int boff[2] = {0, 0};
-  for (int s2 = 0; s2 < 2; ++s2) {
-    if (s2 >= nsub) break;
-    const int lo = b_off[bg + s2], hi = b_off[bg + s2 + 1];
-    boff[s2] = lo;
-    for (int k = lo + (int)threadIdx.x; k < hi; k += 512) tab[s2][b_fbl[k]] = (short)(k - lo);
-  }
-  __syncthreads();
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nfb; base += 64) {
-      const int f = base + lane;
-      const bool live = f < nfb && (tab[0][f] >= 0 || tab[1][f] >= 0);
-      const unsigned long long m = __ballot(live);
-      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (short)f;
-      cnt += __popcll(m);
-    }
-    if (lane == 0) nu_s = cnt;
-  }
-  __syncthreads();
-  const int nu = nu_s;
-  if (mgi * 2 >= nu) return;  // uniform: no block of this pair
-  const int fm0 = ublk[mgi * 2], fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nu; base += 64) {
-      const int ki = base + lane;
-      unsigned pm = 0;
-      if (ki < nu) {
-        const int fk = ublk[ki];
-        for (int s2 = 0; s2 < 2; ++s2) {
-          if (s2 >= nsub || tab[s2][fk] < 0) continue;
-          if (tab[s2][fm0] >= 0) pm |= 1u << s2;
-          if (fm1 >= 0 && tab[s2][fm1] >= 0) pm |= 1u << (2 + s2);
-        }
-      }
-      const unsigned long long m = __ballot(pm != 0);
-      if (pm) {
-        const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        sk[i] = (short)ki;
-        spm[i] = (unsigned char)pm;
-      }
-      cnt += __popcll(m);
-    }
-    if (lane == 0) nst_s = cnt;
-  }
-  __syncthreads();
-  const int nst = nst_s;
-  const int nq = gga ? 4 : 1;
-  const int fm = mb ? fm1 : fm0;
-  const signed char *prow[2] = {nullptr, nullptr};
-  int mbase[2] = {0, 0};  // first block of each row block's molecule
-#pragma unroll
-  for (int m2 = 0; m2 < 2; ++m2) {
-    const int f = m2 ? fm1 : fm0;
-    if (f >= 0) {
-      const int mol = mt.blk_mol[f];
-      mbase[m2] = f - mt.blk_local[f];
-      prow[m2] = Ps + mt.Ps_off[mol] + (int64_t)(f - mbase[m2]) * mt.nfb[mol] * SEC;
-    }
-  }
-  // P sections of the live blocks and Pm sections of the live batches of step
-  // i -> buffer dst; 8 waves, pieces spread over them
-  auto issue = [&](int i, signed char (*dst)[SEC]) {
-    const int fk = ublk[sk[i]];
-    const unsigned pm = spm[i];
-#pragma unroll
-    for (int m2 = 0; m2 < 2; ++m2)
-      if ((pm >> (2 * m2)) & 3u)
-        dma_section(prow[m2] + (int64_t)(fk - mbase[m2]) * SEC, &dst[m2][0],
-                    (wv + 3 * m2) & 7, 8, lane);
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2)
-      if (((pm >> s2) | (pm >> (2 + s2))) & 1u)
-        dma_section(store + (boff[s2] + tab[s2][fk] - s0) * sb_bytes + OFF_PM,
-                    &dst[2 + s2][0], (wv + 6 + 3 * s2) & 7, 8, lane);
-  };
-
-  double x64[2][4][4];
-#pragma unroll
-  for (int f = 0; f < 2; ++f)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) x64[f][j][r] = 0.0;
-  // One step: wait for its operands, start the next step's copies into the
-  // other buffer, then the tile products.  The two buffers are separate
-  // arrays, so the compiler sees that the copies and the reads touch
-  // different memory and does not wait for the copies.
-  auto step = [&](int i, signed char (*cur)[SEC], signed char (*nxt)[SEC]) {
-    wait_vm0();
-    __syncthreads();  // buffer cur complete, nxt free
-    const unsigned pmc = spm[i];
-    const bool act = (pmc >> (2 * mb + sb)) & 1u;
-    // every operand of this step from LDS into registers before the next
-    // step's copies are issued: reads after a copy into LDS make the
-    // compiler wait for that copy
-    v4i A[2][4], B[4][4];
-    int ea[2] = {0, 0}, eb[4] = {0, 0, 0, 0};
-    if (act) {
-      const signed char *ab = &cur[mb][0];
-      const signed char *bb = &cur[2 + sb][0];
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p) A[f][p] = ld16(ab + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
-        ea[f] = *reinterpret_cast<const int *>(ab + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p) B[j][p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-        eb[j] = bb[4 * PLANE + 16 * j + l16];
-      }
-    }
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the reads are done
-    if (i + 1 < nst) issue(i + 1, nxt);
-    if (act) {
-      double sA[2][4];
-#pragma unroll
-      for (int f = 0; f < 2; ++f)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ea[f] >> (8 * r)));
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double sB = pow2(eb[j] - 19);
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-          int hi4[4], lo4[4];
-          tile_product(A[f], B[j], hi4, lo4);
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            x64[f][j][r] =
-                fma(fma((double)lo4[r], 0x1p-14, (double)hi4[r]), sA[f][r] * sB, x64[f][j][r]);
-        }
-      }
-    }
-  };
-  if (nst > 0) issue(0, bufA);
-  for (int i = 0; i < nst; i += 2) {
-    step(i, bufA, bufB);
-    if (i + 1 < nst) step(i + 1, bufB, bufA);
-  }
-  __syncthreads();  // every wave is done with the buffers (red aliases them)
-
-  // Epilogue: per live slot (m, batch), column sums over the block's functions.
-  const int slm = (sb < nsub && fm >= 0 && tab[sb][fm] >= 0) ? boff[sb] + tab[sb][fm] : -1;
-  double prt[4][4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
-  if (slm >= 0) {
-    const signed char *tb = store + (slm - s0) * sb_bytes;
-    const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int g = 16 * j + l16;
-      double sm[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int f = 0; f < 2; ++f)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 32 * rh + 16 * f + 4 * q + r;
-          const double x = x64[f][j][r];
-          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
-          if (gga) {
-            sm[1] += (double)G[mu * PT + g] * x;
-            sm[2] += (double)G[TILE + mu * PT + g] * x;
-            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-          }
-        }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        double v = sm[c];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        prt[j][c] = v;
-      }
-    }
-  }
-  if (q == 0) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
-  }
-  __syncthreads();
-  {
-    // thread -> (batch, block, column group c, point); waves 4 s + 2 m + {0, 1}
-    const int s2 = threadIdx.x >> 8, m2 = (threadIdx.x >> 7) & 1, c = (threadIdx.x >> 6) & 1,
-              g = threadIdx.x & 63;
-    const int fmx = m2 ? fm1 : fm0;
-    const int sl = (s2 < nsub && fmx >= 0 && tab[s2][fmx] >= 0) ? boff[s2] + tab[s2][fmx] : -1;
-    if (sl >= 0) {
-      const int w0 = 4 * s2 + 2 * m2;
-#pragma unroll
-      for (int cc = 0; cc < 4; cc += 2) {
-        const int c2 = c + cc;
-        const double acc = c2 < nq ? red[w0][c2][g] + red[w0 + 1][c2][g] : 0.0;
-        part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
-      }
-    }
-  }
-}
-
-// density_x4_i8: the split form's contraction for one batch and a quad of
-// function blocks 4i .. 4i + 3 of the batch's molecule, chosen by block
-// index rather than by position in the live list, so that workgroups of
-// different batches with the same quad read the same P sections.  Eight
-// waves; wave w holds rows 32 (w % 2) .. +31 of block 4i + w / 2 and idles
-// when that block is not live in the batch.  Workgroup w takes quad
-// (w / 8) / nbq and batch b0 + 8 ((w / 8) % nbq) + w % 8: the workgroups that
-// run together on one XCD share the quad, whose row of P sections stays in
-// that XCD's L2, and stream the Pm sections of their own batches.  The steps
-// are the batch's live blocks k in ascending order (its CSR range).
-__global__ void __launch_bounds__(512) density_x4_i8(
-    MolTab mt, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
-    const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int64_t nbq, int gga,
-    const signed char *__restrict__ store, double *__restrict__ part) {
-  const int xcd = blockIdx.x & 7;
-  const int64_t rr = blockIdx.x >> 3;
-  const int quad = (int)(rr / nbq);
-  const int64_t b = b0 + (rr % nbq) * 8 + xcd;
-  if (b >= b1) return;
-  const int lo = b_off[b], hi = b_off[b + 1];
-  if (hi <= lo) return;
-  const int f0 = b_fbl[lo];
-  const int mol = mt.blk_mol[f0];
-  const int mbase = f0 - mt.blk_local[f0];  // the molecule's first (global) block
-  const int nbm = mt.nfb[mol];
-  if (4 * quad >= nbm) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int u = wv >> 1, rh = wv & 1;
-  const int l16 = lane & 15, q = lane >> 4;
-  const int64_t sb_bytes = slot_bytes_i8(gga);
-
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][SEC];
-  __shared__ double red[8][4][PT];
-  __shared__ int qslot[4];
-  if (threadIdx.x < 4) {  // slot of each quad block in this batch, -1 if not live
-    const int fb = mbase + 4 * quad + (int)threadIdx.x;
-    int sl = -1;
-    if (4 * quad + (int)threadIdx.x < nbm) {
-      int a = lo, c = hi;  // b_fbl ascending in [lo, hi)
-      while (a < c) {
-        const int mid = (a + c) >> 1;
-        if (b_fbl[mid] < fb) a = mid + 1;
-        else c = mid;
-      }
-      if (a < hi && b_fbl[a] == fb) sl = a;
-    }
-    qslot[threadIdx.x] = sl;
-  }
-  __syncthreads();
-  if (qslot[0] < 0 && qslot[1] < 0 && qslot[2] < 0 && qslot[3] < 0) return;  // uniform
-  const int myslot = qslot[u];
-  const int nq = gga ? 4 : 1;
-  const signed char *prow =
-      Ps + mt.Ps_off[mol] + (int64_t)(4 * quad + u) * nbm * SEC;  // sections (m, k), k local
-
-  struct AFrag {
-    v4i v[2][4];
-    int e[2];
-  };
-  auto issue = [&](int i, int dbuf, AFrag &an) {
-    dma_section(store + (i - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][0], wv, 8, lane);
-    if (myslot >= 0) {
-      const signed char *pa = prow + (int64_t)(b_fbl[i] - mbase) * SEC;
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          an.v[f][p] = ld16(pa + p * PLANE + ((2 * rh + f) * 64 + lane) * 16);
-        an.e[f] = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * (2 * rh + f) + 4 * q);
-      }
-    }
-  };
-  double x64[2][4][4];
-#pragma unroll
-  for (int f = 0; f < 2; ++f)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) x64[f][j][r] = 0.0;
-  // rows or points stored as zero have zero slices: their scale does not matter
-  auto step = [&](int i, int buf, const AFrag &ac, AFrag &an) {
-    wait_vm0();
-    __syncthreads();  // buffer buf complete, buf ^ 1 free
-    if (i + 1 < hi) issue(i + 1, buf ^ 1, an);
-    if (myslot < 0) return;
-    double sA[2][4];
-#pragma unroll
-    for (int f = 0; f < 2; ++f)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ac.e[f] >> (8 * r)));
-    const signed char *bb = &bbuf[buf][0];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      v4i B[4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-      const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        int hi4[4], lo4[4];
-        tile_product(ac.v[f], B, hi4, lo4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          x64[f][j][r] =
-              fma(fma((double)lo4[r], 0x1p-14, (double)hi4[r]), sA[f][r] * sB, x64[f][j][r]);
-      }
-    }
-  };
-  AFrag a0, a1;
-  issue(lo, 0, a0);
-  for (int i = lo; i < hi; i += 2) {
-    step(i, 0, a0, a1);
-    if (i + 1 < hi) step(i + 1, 1, a1, a0);
-  }
-
-  // Epilogue: column sums over each live quad block's functions.
-  double prt[4][4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
-  if (myslot >= 0) {
-    const signed char *tb = store + (myslot - s0) * sb_bytes;
-    const float *G = reinterpret_cast<const float *>(tb + OFF_G);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int g = 16 * j + l16;
-      double sm[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int f = 0; f < 2; ++f)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 32 * rh + 16 * f + 4 * q + r;
-          const double x = x64[f][j][r];
-          sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
-          if (gga) {
-            sm[1] += (double)G[mu * PT + g] * x;
-            sm[2] += (double)G[TILE + mu * PT + g] * x;
-            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-          }
-        }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        double v = sm[c];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        prt[j][c] = v;
-      }
-    }
-  }
-  if (q == 0) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
-  }
-  __syncthreads();
-  {
-    const int uu = threadIdx.x >> 7, c = (threadIdx.x >> 6) & 1, g = threadIdx.x & 63;
-    const int sl = qslot[uu];
-    if (sl >= 0) {
-#pragma unroll
-      for (int cc = 0; cc < 4; cc += 2) {
-        const int c2 = c + cc;
-        const double acc = c2 < nq ? red[2 * uu][c2][g] + red[2 * uu + 1][c2][g] : 0.0;
-        part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
-      }
-    }
-  }
-}
-
-// Per batch: column sums of the live slots in block order (the fused
-// kernel's order), functional, Z coefficients, (E, N) partial sums.
+// Per batch: column sums of the live slots in ascending block order,
+// functional, Z coefficients, (E, N) partial sums.
 __global__ void __launch_bounds__(256) density_f_i8(
     int64_t s0, int64_t b0, const int *__restrict__ b_off, int gga,
     const double *__restrict__ part, const double *__restrict__ pw, XcList xl,
@@ -1697,55 +780,6 @@ void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-template <int GBT, int MB>
-static void launch_density_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
-                              const signed char *Ps, int gga, signed char *store,
-                              const XcList &xl, double *coef, double *blockEN) {
-  const unsigned nwg = (unsigned)((b1 - b0 + GBT - 1) / GBT);
-  hipLaunchKernelGGL((density_fused_i8<GBT, MB>), dim3(nwg), dim3(256 * MB), 0, s, pl.tab(),
-                     pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, gga, store, pl.pw.p, xl,
-                     coef, blockEN);
-}
-
-template <int GBT, int MB>
-static void launch_density_x_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
-                                const Plan &pl, const signed char *Ps, int gga,
-                                const signed char *store, double *part) {
-  const int64_t ngroups = (b1 - b0 + GBT - 1) / GBT;
-  const int64_t ngq = (ngroups + 7) / 8;
-  const int umax = std::min(pl.nfb, GBT * pl.max_mol_nfb);
-  const int nmg = (umax + MB - 1) / MB;
-  const int64_t nwg = ngq * 8 * nmg;
-  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
-  hipLaunchKernelGGL((density_x_i8<GBT, MB>), dim3((unsigned)nwg), dim3(256 * MB), 0, s,
-                     pl.tab(), pl.nfb, Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store,
-                     part);
-}
-
-static void launch_density_x22_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
-                                  const Plan &pl, const signed char *Ps, int gga,
-                                  const signed char *store, double *part) {
-  const int64_t ngroups = (b1 - b0 + 1) / 2;
-  const int64_t ngq = (ngroups + 7) / 8;
-  const int umax = std::min(pl.nfb, 2 * pl.max_mol_nfb);
-  const int nmg = (umax + 1) / 2;
-  const int64_t nwg = ngq * 8 * nmg;
-  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
-  hipLaunchKernelGGL(density_x22_i8, dim3((unsigned)nwg), dim3(512), 0, s, pl.tab(), pl.nfb,
-                     Ps, pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nmg, gga, store, part);
-}
-
-static void launch_density_x4_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
-                                 const Plan &pl, const signed char *Ps, int gga,
-                                 const signed char *store, double *part) {
-  const int64_t nbq = (b1 - b0 + 7) / 8;
-  const int nquad = (pl.max_mol_nfb + 3) / 4;
-  const int64_t nwg = nbq * 8 * nquad;
-  NKX_REQUIRE(nwg < (int64_t)1 << 31, NKX_ERR_UNSUPPORTED, "density grid too large");
-  hipLaunchKernelGGL(density_x4_i8, dim3((unsigned)nwg), dim3(512), 0, s, pl.tab(), Ps,
-                     pl.b_off.p, pl.b_fbl.p, s0, b0, b1, nbq, gga, store, part);
-}
-
 template <int GBT>
 static void launch_density_x2_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t s0,
                                  const Plan &pl, const signed char *Ps, int gga,
@@ -1774,22 +808,11 @@ static void launch_density_x2a_i8(hipStream_t s, int64_t b0, int64_t b1, int64_t
                      store, part);
 }
 
-// shape: batches per workgroup (1, 2) + 16 x (function blocks per step - 1),
-// + 32 for the split form (contraction, functional, Z as three kernels),
-// 65 / 66: the split form with two row fragments per wave (1 / 2 batches),
-// 71 / 72: 65 on absolute block pairs in XCD tiles of 4 x 16 / 8 x 8
-// (pairs x batches),
-// 67: the split form for two batches x two blocks with both operands in LDS,
-// 129: the split form over block quads (one batch, eight waves);
-// other values take 65
+// shape: 65 / 66 one / two batches per workgroup, 71 / 72 one batch per
+// workgroup on absolute block pairs dealt to the XCDs in tiles of 4 x 16 /
+// 8 x 8 (pairs x batches); other values take 65
 bool mixed_density_shape_ok(int shape) {
-  switch (shape) {
-    case 1: case 2: case 17: case 18: case 33: case 34: case 49: case 50:
-    case 65: case 66: case 67: case 71: case 72: case 129:
-      return true;
-    default:
-      return false;
-  }
+  return shape == 65 || shape == 66 || shape == 71 || shape == 72;
 }
 
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
@@ -1799,49 +822,23 @@ void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, c
   NKX_REQUIRE(pl.nfb <= MAXNFB, NKX_ERR_UNSUPPORTED,
               "more than 1024 function blocks (65536 functions)");
   if (!mixed_density_shape_ok(shape)) shape = 65;
-  if (shape & 224) {
-    NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "split density form without a partial buffer");
-    if (shape == 67) {
-      launch_density_x22_i8(s, b0, b1, s0, pl, Ps, gga, store, part);
-    } else if (shape == 129) {
-      launch_density_x4_i8(s, b0, b1, s0, pl, Ps, gga, store, part);
-    } else if (shape & 64) {
-      switch (shape) {
-        case 66: launch_density_x2_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-        case 71: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 4, 16); break;
-        case 72: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 8, 8); break;
-        default: launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-      }
-    } else {
-    switch (shape & 31) {
-      case 1: launch_density_x_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-      case 17: launch_density_x_i8<1, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-      case 18: launch_density_x_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-      default: launch_density_x_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
-    }
-    }
-    NKX_HIP_CHECK(hipGetLastError());
-    return;  // functional and Z: vkm_density_finish
-  }
+  NKX_REQUIRE(part != nullptr, NKX_ERR_STATE, "density contraction without a partial buffer");
   switch (shape) {
-    case 1: launch_density_i8<1, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
-    case 2: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
-    case 17: launch_density_i8<1, 2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
-    case 18: launch_density_i8<2, 2>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
-    default: launch_density_i8<2, 1>(s, b0, b1, s0, pl, Ps, gga, store, xl, coef, blockEN); break;
+    case 66: launch_density_x2_i8<2>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
+    case 71: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 4, 16); break;
+    case 72: launch_density_x2a_i8(s, b0, b1, s0, pl, Ps, gga, store, part, 8, 8); break;
+    default: launch_density_x2_i8<1>(s, b0, b1, s0, pl, Ps, gga, store, part); break;
   }
   NKX_HIP_CHECK(hipGetLastError());
 }
 
-// Second half of the split form (shape with bit 32, 64 or 128): per-batch
-// functional from the slot partial sums, then the Z slices of every slot.
-// Nothing for the fused shapes.
+// Second half of the density stage: per-batch functional from the slot
+// partial sums, then the Z slices of every slot.
 void vkm_density_finish(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0,
                         const Plan &pl, int gga, signed char *store, const XcList &xl,
                         double *coef, double *blockEN, const double *part) {
   if (b1 <= b0) return;
-  if (!mixed_density_shape_ok(shape)) shape = 65;
-  if (!(shape & 224)) return;
+  (void)shape;
   hipLaunchKernelGGL(density_f_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0,
                      pl.b_off.p, gga, part, pl.pw.p, xl, coef, blockEN, nullptr);
   NKX_HIP_CHECK(hipGetLastError());

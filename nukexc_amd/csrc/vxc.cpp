@@ -534,9 +534,6 @@ BuildSetup prepare_build(Workspace &ws, const Plan &pl, int nspin, bool gga, int
   b.mixed = precision == NKX_PREC_MIXED || b.bf16;
   NKX_REQUIRE(!b.bf16 || nspin == 1, NKX_ERR_UNSUPPORTED,
               "the spin-polarised build is available in the FP64 and int8 forms");
-  // the polarised int8 build needs the split density form (contraction per
-  // spin, then one polarised functional)
-  if (b.mixed && !b.bf16 && nspin == 2 && !(b.group & 224)) b.group = 65;
   NKX_REQUIRE(!b.bf16 || nmol == 1, NKX_ERR_UNSUPPORTED,
               "the split-bf16 form builds one molecule at a time");
   const size_t slot_bytes = b.bf16    ? (size_t)bf16_slot_bytes(gga ? 1 : 0)
@@ -728,13 +725,12 @@ double autotune_on(Ctx &c, const Plan &pl, int precision, const std::function<vo
     c.opt.vgemm_split = base;
     build();  // warm caches and workspace
     int best_g = bf16 ? 1 : (precision == NKX_PREC_MIXED ? 65 : 2);
-    // density shapes: batches per workgroup (int8 form: the split-form
-    // shapes 65 (32 rows per wave), 71 (65 in XCD tiles of absolute block
-    // pairs), 67 (two batches x two blocks, both operands in LDS) and 129
-    // (block quads); the fused shapes and 33, 34, 49, 50, 66, 72 are never
-    // the fastest on large grids and are not timed)
+    // density shapes: point batches (64, 128 or 256 points) per workgroup
+    // in the FP64 form; in the int8 form 65 (one batch) and 71 (one batch,
+    // absolute block pairs in XCD tiles) -- 66 and 72 (two batches, 8 x 8
+    // tiles) are never the fastest on large grids and are not timed
     const std::vector<int> shapes =
-        precision == NKX_PREC_MIXED ? std::vector<int>{65, 71, 67, 129}
+        precision == NKX_PREC_MIXED ? std::vector<int>{65, 71}
                                     : std::vector<int>{1, 2, 4};
     if (!bf16)
       for (int g : shapes) {

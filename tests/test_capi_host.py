@@ -123,3 +123,12 @@ def test_function_atoms_map_and_rejection():
     moved = dataclasses.replace(w["basis"], O=w["basis"].O + 1e-6)
     with pytest.raises(ValueError):
         function_atoms(moved, w["xyz"])
+
+
+def test_launch_points_reports_the_batch_size():
+    """nkx_launch_points: grid points per density workgroup for a precision
+    and density shape (the point-batch size the autotuner picks): FP64 1, 2,
+    4 batches of 64 points (default 2), int8 65 / 71 / 72 one batch, 66 two."""
+    from nukexc_amd.api import Context
+    assert [Context.launch_points("fp64", s) for s in (0, 1, 2, 4)] == [128, 64, 128, 256]
+    assert [Context.launch_points("mixed", s) for s in (0, 65, 66, 71, 72)] == [64, 64, 128, 64, 64]
