@@ -2,6 +2,8 @@
 // integer MFMA accumulation (slice form and tile layout in slices_i8.hpp):
 // collocation into slices, P gather, the density contraction forms, the
 // per-batch functional, Z slices and the V contraction.
+#include <type_traits>
+
 #include "nkx_internal.hpp"
 #include "sto_eval.hpp"
 #include "vxc_kernels.hpp"
@@ -282,8 +284,8 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb) {
       if (!(((pm >> sb) | (pm >> (GBT + sb))) & 1u)) continue;
-      dma_section(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
-                  (wv + sb) & 3, 4, lane);
+      dma_section<4>(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+                     (wv + sb) & 3, lane);
     }
     if ((pm >> (mb * GBT)) & ROW) {
       const signed char *pa = prow + (int64_t)(fk - mbase) * SEC;
@@ -591,14 +593,20 @@ __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restr
 // Z_I Phi_J^T over the batches where I and J are live, for the four tiles
 // (I_a, J_b) of a super-pair (blocks I_0 < I_1, J_0 < J_1 of one molecule;
 // a missing second block is -1; on the diagonal I_1 x J_0 is the transpose
-// of I_0 x J_1 and is skipped).  One workgroup of eight waves per
+// of I_0 x J_1 and has no pair index).  One workgroup of eight waves per
 // (super-pair, split); its batches b = lo + split + nsplit i are taken in
 // windows of 512 candidates, compacted in ascending order into an LDS list
 // of the four slots.  The Z and F sections of the J slots (shared by all
 // waves) are staged in LDS, two buffers: the next batch's are copied in
-// while the current one computes.  Wave w owns rows 16 (w % 4) .. +15 of
-// block I_(w / 4) (its F and Z fragments loaded one batch ahead) and all 128
-// columns of J_0, J_1.  Each tile is added into vpart at its pair index.
+// while the current one computes.  Wave w owns rows 32 h .. 32 h + 31 of
+// block I_a and the 64 columns of block J_b (a = w / 4, h = (w / 2) % 2,
+// b = w % 2), so that every J fragment read from LDS feeds two 16-row tile
+// products.  A batch runs in two phases, F_I Z_J^T and then Z_I F_J^T; the
+// wave's I fragments live in registers, one set each: Z_I is loaded when the
+// batch starts (used in the second phase) and F_I of the next batch between
+// the phases (F_I is dead by then).  Each element sums its batches in
+// ascending order, F_I Z_J^T before Z_I F_J^T within a batch, as in the
+// one-pair kernel.  Each tile is added into vpart at its pair index.
 __global__ void __launch_bounds__(512) vgemm_i8(
     int nsp, int nsplit, const int4 *__restrict__ sp_blk, const int4 *__restrict__ sp_pidx,
     int npairs, const int *__restrict__ tile_slot, MolTab mt, int64_t b0, int64_t b1,
@@ -609,10 +617,11 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   const int sp = blockIdx.x % nsp;
   const int split = blockIdx.x / nsp;
   const int4 blk = sp_blk[sp];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int half = wv >> 2, rw = wv & 3;
+  // wave-uniform values through readfirstlane, so that the branches on them
+  // are scalar branches
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wa = wv >> 2, wh = (wv >> 1) & 1, wb = wv & 1;
   const int l16 = lane & 15, q = lane >> 4;
-  const bool diag = blk.x == blk.z;
   const int64_t sb_bytes = slot_bytes_i8(gga);
   // J sections, two buffers (Z_J0 F_J0 Z_J1 F_J1 each), declared apart so
   // that a copy into one is known not to touch the other
@@ -620,13 +629,16 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   __shared__ __attribute__((aligned(16))) signed char bufB[4][SEC];
   __shared__ int ls[4][512];  // slots of I_0, I_1, J_0, J_1 per listed batch
   __shared__ int wcnt[8];
+  // this wave's tile and its pair index (-1: no work for this wave)
+  const int4 pix = sp_pidx[sp];
+  const int pp = wa ? (wb ? pix.w : pix.z) : (wb ? pix.y : pix.x);
   double v64[2][4][4];
 #pragma unroll
-  for (int jb = 0; jb < 2; ++jb)
+  for (int g = 0; g < 2; ++g)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) v64[jb][j][r] = 0.0;
+      for (int r = 0; r < 4; ++r) v64[g][j][r] = 0.0;
   // batches of the super-pair's molecule inside [b0, b1)
   const int mol = mt.blk_mol[blk.x];
   const int64_t mb0 = mt.b0[mol], mnb = mt.nb[mol];
@@ -637,83 +649,108 @@ __global__ void __launch_bounds__(512) vgemm_i8(
   for (int u = 0; u < 4; ++u) tso[u] = bl[u] >= 0 ? (int64_t)mt.blk_local[bl[u]] * mnb : -1;
   const int64_t lo = b0 > mb0 ? b0 : mb0, hi = b1 < mb0 + mnb ? b1 : mb0 + mnb;
 
-  // this wave's F and Z fragments of its I slot and their row exponents
-  // (two register sets, used in turn)
-  struct AFrag {
-    v4i F[4], Z[4];
-    int eF, eZ;
+  // this wave's I fragments (two 16-row groups) and row exponents
+  v4i Fr[2][4], Zr[2][4];
+  int eF[2] = {0, 0}, eZ[2] = {0, 0};
+  auto f_section = [&](int s) { return store + (s - s0) * sb_bytes + OFF_F; };
+  auto z_section = [&](int s) { return zbase + (s - s0) * zstride; };
+  auto load_I = [&](const signed char *sec, v4i (*dst)[4], int *ex) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int rg = 2 * wh + g;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) dst[g][pl] = ld16(sec + pl * PLANE + (rg * 64 + lane) * 16);
+      ex[g] = *reinterpret_cast<const int *>(sec + 4 * PLANE + 16 * rg + 4 * q);
+    }
   };
-  // J sections -> LDS buffer dst (direct copies); the I fragments -> an
-  auto issue = [&](int i, signed char (*dst)[SEC], AFrag &an) {
+  // whether the wave's tile has work in listed batch i
+  auto active = [&](int i) {
+    return pp >= 0 && __builtin_amdgcn_readfirstlane(ls[wa][i]) >= 0 &&
+           __builtin_amdgcn_readfirstlane(ls[2 + wb][i]) >= 0;
+  };
+  // F_I of batch i; where the wave's tile is idle in batch i (or past the
+  // list) the loads read slot s0 instead, whose values go unused, so that
+  // every path issues the same loads and the waits after them stay exact
+  auto next_F = [&](int i, int n) {
+    const int sI = i < n && active(i) ? __builtin_amdgcn_readfirstlane(ls[wa][i]) : (int)s0;
+    load_I(f_section(sI), Fr, eF);
+  };
+  // J sections of listed batch i -> LDS buffer dst (direct copies)
+  auto issue_J = [&](int i, signed char (*dst)[SEC]) {
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb) {
-      const int sJ = ls[2 + jb][i];
+      const int sJ = __builtin_amdgcn_readfirstlane(ls[2 + jb][i]);
       if (sJ < 0) continue;
-      const signed char *tJ = store + (sJ - s0) * sb_bytes;
-      dma_section(zbase + (sJ - s0) * zstride, &dst[2 * jb][0], (wv + 4 * jb) & 7, 8, lane);
-      dma_section(tJ + OFF_F, &dst[2 * jb + 1][0], (wv + 4 * jb + 2) & 7, 8, lane);
-    }
-    const int sI = ls[half][i];
-    if (sI >= 0) {
-      const signed char *tI = store + (sI - s0) * sb_bytes;
-      const signed char *zI = zbase + (sI - s0) * zstride;
-#pragma unroll
-      for (int pl = 0; pl < 4; ++pl) {
-        an.F[pl] = ld16(tI + OFF_F + pl * PLANE + (rw * 64 + lane) * 16);
-        an.Z[pl] = ld16(zI + pl * PLANE + (rw * 64 + lane) * 16);
-      }
-      an.eF = *reinterpret_cast<const int *>(tI + OFF_F + 4 * PLANE + 16 * rw + 4 * q);
-      an.eZ = *reinterpret_cast<const int *>(zI + 4 * PLANE + 16 * rw + 4 * q);
+      dma_section<8>(z_section(sJ), &dst[2 * jb][0], (wv + 4 * jb) & 7, lane);
+      dma_section<8>(f_section(sJ), &dst[2 * jb + 1][0], (wv + 4 * jb + 2) & 7, lane);
     }
   };
-  // one batch of the list: wait for its operands, start the next batch's
-  // copies into the other buffer and register set, then the tile products
-  auto step = [&](int i, int n, signed char (*cur)[SEC], signed char (*nxt)[SEC],
-                  const AFrag &ac, AFrag &an) {
-    wait_vm0();
-    __syncthreads();  // buffer buf complete, buf ^ 1 free
-    const bool mine = ls[half][i] >= 0;
-    const bool jl[2] = {ls[2][i] >= 0 && !(diag && half == 1), ls[3][i] >= 0};
-    if (i + 1 < n) issue(i + 1, nxt, an);
-    if (!mine) return;
-    // rows and columns stored as zero have zero slices: no zero test on the
-    // scales
-    double sF[4], sZ[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      sF[r] = pow2((int)(signed char)(ac.eF >> (8 * r)));
-      sZ[r] = pow2((int)(signed char)(ac.eZ >> (8 * r)));
-    }
+  // the first batch of a window: J sections through VGPRs and LDS stores
+  // (with a direct copy into bufA ahead of the batch loop as well, the
+  // compiler's wait insertion stops telling the copies into bufB apart from
+  // the reads of bufA and waits for them before every batch's first LDS
+  // read), and F_I
+  auto fill = [&](int n, signed char (*dst)[SEC]) {
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb) {
-      if (!jl[jb]) continue;
-      const signed char *bz = &cur[2 * jb][0];
-      const signed char *bf = &cur[2 * jb + 1][0];
+      const int sJ = __builtin_amdgcn_readfirstlane(ls[2 + jb][0]);
+      if (sJ < 0) continue;
+      const signed char *src[2] = {z_section(sJ), f_section(sJ)};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v4i BZ[4], BF[4];
+      for (int u = 0; u < 2; ++u) {
+        v4i t[3];
 #pragma unroll
-        for (int pl = 0; pl < 4; ++pl) {
-          BZ[pl] = ld16(bz + pl * PLANE + (j * 64 + lane) * 16);
-          BF[pl] = ld16(bf + pl * PLANE + (j * 64 + lane) * 16);
+        for (int k = 0; k < 3; ++k) {
+          const int o = 16 * threadIdx.x + k * 16 * 512;
+          if (o < SEC) t[k] = ld16(src[u] + o);
         }
-        const double sZJ = pow2((int)bz[4 * PLANE + 16 * j + l16] - 19);
-        const double sFJ = pow2((int)bf[4 * PLANE + 16 * j + l16] - 19);
-        int h1[4], o1[4], h2[4], o2[4];
-        tile_product(ac.F, BZ, h1, o1);
-        tile_product(ac.Z, BF, h2, o2);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          v64[jb][j][r] =
-              fma(fma((double)o1[r], 0x1p-14, (double)h1[r]), sF[r] * sZJ, v64[jb][j][r]);
-          v64[jb][j][r] =
-              fma(fma((double)o2[r], 0x1p-14, (double)h2[r]), sZ[r] * sFJ, v64[jb][j][r]);
+        for (int k = 0; k < 3; ++k) {
+          const int o = 16 * threadIdx.x + k * 16 * 512;
+          if (o < SEC) *reinterpret_cast<v4i *>(&dst[2 * jb + u][o]) = t[k];
         }
       }
     }
+    next_F(0, n);
+  };
+  // one phase of a batch: A = F_I (p = 0) or Z_I (p = 1) against the Z or F
+  // section of J_b
+  auto phase = [&](int p, signed char (*cur)[SEC]) {
+    const signed char *sec = &cur[2 * wb + p][0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v4i B[4];
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) B[pl] = ld16(sec + pl * PLANE + (j * 64 + lane) * 16);
+      // rows and columns stored as zero have zero slices: no zero test on
+      // the scales
+      const double sc = pow2((int)sec[4 * PLANE + 16 * j + l16] - 19);
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        int h[4], o[4];
+        tile_product(p ? Zr[g] : Fr[g], B, h, o);
+        const int er = p ? eZ[g] : eF[g];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          v64[g][j][r] = fma(fma((double)o[r], 0x1p-14, (double)h[r]),
+                             pow2((int)(signed char)(er >> (8 * r))) * sc, v64[g][j][r]);
+      }
+    }
+  };
+  // one listed batch: wait for its operands, load Z_I, start the next
+  // batch's J copies, then the two phases with F_I of the next batch loaded
+  // between them
+  auto step = [&](int i, int n, signed char (*cur)[SEC], signed char (*nxt)[SEC]) {
+    wait_vm0();
+    __syncthreads();  // buffer cur complete, nxt free
+    const bool act = active(i);
+    if (act) load_I(z_section(__builtin_amdgcn_readfirstlane(ls[wa][i])), Zr, eZ);
+    if (i + 1 < n) issue_J(i + 1, nxt);
+    if (act) phase(0, cur);
+    next_F(i + 1, n);
+    if (act) phase(1, cur);
   };
 
-  AFrag a0, a1;
   for (int64_t c0 = lo + split; c0 < hi; c0 += 512 * (int64_t)nsplit) {
     const int64_t b = c0 + (int64_t)nsplit * threadIdx.x;
     int sl[4] = {-1, -1, -1, -1};
@@ -732,6 +769,7 @@ __global__ void __launch_bounds__(512) vgemm_i8(
       if (w < wv) off += wcnt[w];
       n += wcnt[w];
     }
+    n = __builtin_amdgcn_readfirstlane(n);
     if (act) {
       const int i = off + __popcll(m & ((1ull << lane) - 1ull));
 #pragma unroll
@@ -739,26 +777,23 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     }
     __syncthreads();
     if (n == 0) continue;
-    issue(0, bufA, a0);
+    fill(n, bufA);
     for (int i = 0; i < n; i += 2) {
-      step(i, n, bufA, bufB, a0, a1);
-      if (i + 1 < n) step(i + 1, n, bufB, bufA, a1, a0);
+      step(i, n, bufA, bufB);
+      if (i + 1 < n) step(i + 1, n, bufB, bufA);
     }
   }
-  const int4 pix = sp_pidx[sp];
-  const int pp[2] = {half ? pix.z : pix.x, half ? pix.w : pix.y};
+  if (pp < 0) return;
+  double *out = vpart + ((int64_t)split * npairs + pp) * (FB * FB);
 #pragma unroll
-  for (int jb = 0; jb < 2; ++jb) {
-    if (pp[jb] < 0) continue;
-    double *out = vpart + ((int64_t)split * npairs + pp[jb]) * (FB * FB);
+  for (int g = 0; g < 2; ++g)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int mu = 16 * rw + 4 * q + r, nu = 16 * j + l16;
-        out[mu * FB + nu] += v64[jb][j][r];
+        const int mu = 32 * wh + 16 * g + 4 * q + r, nu = 16 * j + l16;
+        out[mu * FB + nu] += v64[g][j][r];
       }
-  }
 }
 
 // ---------------------------------------------------------------------------

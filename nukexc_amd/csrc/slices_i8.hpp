@@ -155,17 +155,22 @@ __device__ inline void store_section(const double (*v)[65], const int *ex, signe
 }
 
 // Direct global -> LDS copy of one section (4 planes + the 256-byte exponent
-// area) by the waves w0, w0 + nw, ... of a workgroup: 16 pieces of 1 KB
-// (global_load_lds_dwordx4: lane l writes its 16 bytes at dst + 16 l) and
-// one of 256 bytes (dword form).  The copies are tracked by vmcnt only; the
-// consumer waits with wait_vm0() before the barrier that publishes the LDS.
-__device__ __forceinline__ void dma_section(const signed char *src, signed char *dst_lds,
-                                            int w0, int nw, int lane) {
-  for (int pc = w0; pc < 17; pc += nw) {
+// area) by the NW waves of a workgroup, this one taking pieces w0, w0 + NW,
+// ...: 16 pieces of 1 KB (global_load_lds_dwordx4: lane l writes its 16 bytes
+// at dst + 16 l) and one of 256 bytes (dword form).  The fixed trip count
+// keeps the copies straight-line code, so the compiler's waits after them
+// stay exact.  The copies are tracked by vmcnt only; the consumer waits with
+// wait_vm0() before the barrier that publishes the LDS.
+template <int NW>
+__device__ __forceinline__ void dma_section(const signed char *src, signed char *dst_lds, int w0,
+                                            int lane) {
+#pragma unroll
+  for (int k = 0; k < (17 + NW - 1) / NW; ++k) {
+    const int pc = w0 + k * NW;
     if (pc < 16)
       __builtin_amdgcn_global_load_lds(src + pc * 1024 + lane * 16, dst_lds + pc * 1024, 16, 0,
                                        0);
-    else
+    else if (pc == 16)
       __builtin_amdgcn_global_load_lds(src + 16384 + lane * 4, dst_lds + 16384, 4, 0, 0);
   }
 }
