@@ -27,7 +27,10 @@ namespace xcd {
 // B88 (106), LYP (131), VWN (7), PW91 (109, 134) at the unpolarised point: the spin-resolved forms
 // at rho_a = rho_b = rho / 2, sigma_ss = sigma_ab = sigma / 4, so
 // de/drho = (e_a + e_b) / 2 and de/dsigma = (e_aa + e_ab + e_bb) / 4.
-__host__ __device__ inline void blyp_unpol(int id, double rho, double sigma, XcPoint &o) {
+// Not inlined: its dual-number code would otherwise set the register count
+// of every kernel that evaluates the functional list, PBE included.
+__host__ __device__ inline __attribute__((noinline)) void blyp_unpol(int id, double rho,
+                                                                    double sigma, XcPoint &o) {
   const D5 ra = dvar(0.5 * rho, 0), rb = dvar(0.5 * rho, 1);
   const D5 saa = dvar(0.25 * sigma, 2), sab = dvar(0.25 * sigma, 3), sbb = dvar(0.25 * sigma, 4);
   D5 e;
