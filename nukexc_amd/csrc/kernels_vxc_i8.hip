@@ -142,7 +142,8 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 // XCD (workgroups are dealt to the eight XCDs in turn) and share its L2 for
 // the Pm sections.  The epilogue writes, per live slot (m, batch), the column
 // sums of Phi X (and grad Phi X) over the block's 64 functions, summed over
-// the waves of the block in order, to part.
+// the waves of the block in order, to part; Phi comes from the slot's Pm
+// section (one 4-byte load per plane gives a lane its four rows at a point).
 // ABS (one batch per workgroup): the pair is an absolute pair of the
 // molecule's blocks (2 p, 2 p + 1; a workgroup whose two blocks are both dead
 // in its batch exits), and the workgroups are dealt in tiles of tpn pairs x
@@ -364,9 +365,18 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int g = 16 * j + l16;
+        // Phi from the Pm section (rows = points, exponent per point): the
+        // lane's four rows mu = 32 rh + 16 f + 4 q + r at point g are four
+        // consecutive bytes of each plane, one 4-byte load per plane
+        const int eP = tb[OFF_PM + 4 * PLANE + g];
         double sm[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < 2; ++f) {
+          const int off = frag8(g, 32 * rh + 16 * f + 4 * q);
+          int pw4[4];
+#pragma unroll
+          for (int pl = 0; pl < 4; ++pl)
+            pw4[pl] = *reinterpret_cast<const int *>(tb + OFF_PM + pl * PLANE + off);
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int mu = 32 * rh + 16 * f + 4 * q + r;
@@ -374,13 +384,22 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
             for (int s2 = 0; s2 < GBT; ++s2)
               if (s2 == sb) x = x64[s2][f][j][r];
-            sm[0] += unslice(tb + OFF_F, frag8(mu, g), tb[OFF_F + 4 * PLANE + mu]) * x;
+            double ph = 0.0;
+            if (eP != -127) {
+              const double v = (double)(signed char)(pw4[0] >> (8 * r)) +
+                               ldexp((double)(signed char)(pw4[1] >> (8 * r)), -7) +
+                               ldexp((double)(signed char)(pw4[2] >> (8 * r)), -14) +
+                               ldexp((double)(signed char)(pw4[3] >> (8 * r)), -21);
+              ph = ldexp(v, eP - 6);
+            }
+            sm[0] += ph * x;
             if (gga) {
               sm[1] += (double)G[mu * PT + g] * x;
               sm[2] += (double)G[TILE + mu * PT + g] * x;
               sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
             }
           }
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           double v = sm[c];
