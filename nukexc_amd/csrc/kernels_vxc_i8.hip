@@ -31,13 +31,13 @@ __global__ void __launch_bounds__(256) colloc_tiles_i8(
   const int64_t s = s0 + blockIdx.x;
   const int b = slot_b[s];
   const int fb = b_fbl[s];
-  __shared__ FuncRec fr[FB];
   __shared__ double v[FB][65];
   __shared__ int eF[FB], eP[PT];
-  if (threadIdx.x < FB) fr[threadIdx.x] = F[fb * FB + threadIdx.x];
-  __syncthreads();
   const int g = threadIdx.x & (PT - 1);
-  const int f0 = 16 * (threadIdx.x / PT);
+  // the wave's functions are wave-uniform: their records are scalar loads and
+  // the branches on them scalar branches
+  const int f0 = 16 * __builtin_amdgcn_readfirstlane(threadIdx.x / PT);
+  const FuncRec *fr = F + (int64_t)fb * FB;
   const int64_t gp = (int64_t)b * PT + g;
   const double x = px[gp], y = py[gp], z = pz[gp];
   signed char *base = store + (int64_t)blockIdx.x * slot_bytes_i8(gga);
@@ -85,15 +85,24 @@ __global__ void __launch_bounds__(256) colloc_tiles_i8(
     }
   }
   __syncthreads();
-  if (threadIdx.x < FB) {
-    double m = 0.0;
-    for (int k = 0; k < PT; ++k) m = fmax(m, fabs(v[threadIdx.x][k]));
-    eF[threadIdx.x] = row_exp(m);
-  } else if (threadIdx.x < FB + PT) {
-    const int gg = threadIdx.x - FB;
-    double m = 0.0;
-    for (int k = 0; k < FB; ++k) m = fmax(m, fabs(v[k][gg]));
-    eP[gg] = row_exp(m);
+  {
+    // row maxima along functions (F) and along points (Pm): four threads per
+    // row, 16 elements each, then the four partial maxima by shuffles
+    const int rr = threadIdx.x >> 2, qt = threadIdx.x & 3;
+    double m = 0.0, mp = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      m = fmax(m, fabs(v[rr][16 * qt + k]));
+      mp = fmax(mp, fabs(v[16 * qt + k][rr]));
+    }
+    m = fmax(m, __shfl_xor(m, 1));
+    m = fmax(m, __shfl_xor(m, 2));
+    mp = fmax(mp, __shfl_xor(mp, 1));
+    mp = fmax(mp, __shfl_xor(mp, 2));
+    if (qt == 0) {
+      eF[rr] = row_exp(m);
+      eP[rr] = row_exp(mp);
+    }
   }
   __syncthreads();
   store_section(v, eF, base + OFF_F, false);
@@ -142,8 +151,8 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
 // XCD (workgroups are dealt to the eight XCDs in turn) and share its L2 for
 // the Pm sections.  The epilogue writes, per live slot (m, batch), the column
 // sums of Phi X (and grad Phi X) over the block's 64 functions, summed over
-// the waves of the block in order, to part; Phi comes from the slot's Pm
-// section (one 4-byte load per plane gives a lane its four rows at a point).
+// the block's functions in order, to part; Phi comes from the slot's Pm
+// section.
 // ABS (one batch per workgroup): the pair is an absolute pair of the
 // molecule's blocks (2 p, 2 p + 1; a workgroup whose two blocks are both dead
 // in its batch exits), and the workgroups are dealt in tiles of tpn pairs x
@@ -352,83 +361,78 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   }
 
   // Epilogue: per live slot (m, sb), column sums over the block's functions.
+  // The two waves of block m put their X rows into LDS (over the step
+  // buffers); then every thread takes one point g and 16 consecutive
+  // functions, so that the loads of grad Phi (rows of 64 points) and of the
+  // Pm slices (16 functions of a point: one 16-byte load per plane) are whole
+  // rows, and the four partial sums of a point are added in order.
+  double (*Xs)[65] = reinterpret_cast<double (*)[65]>(&bbuf[0][0][0]);
+  static_assert(sizeof(bbuf) >= sizeof(double) * 64 * 65, "X tile must fit the step buffers");
+  __syncthreads();  // the steps are done with the buffers
   for (int sb = 0; sb < nsub; ++sb) {
-    const int slm = fm >= 0 ? tab[sb][fm] : -1;
-    double prt[4][4];
+#pragma unroll 1
+    for (int m = 0; m < 2; ++m) {
+      const int fmx = m ? fm1 : fm0;
+      const int sl = fmx >= 0 ? tab[sb][fmx] : -1;
+      if (sl < 0) continue;  // uniform
+      if (mb == m) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+        for (int f = 0; f < 2; ++f)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) prt[j][c] = 0.0;
-    if (slm >= 0) {
-      const signed char *tb = store + (slm - s0) * sb_bytes;
-      const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+          for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int g = 16 * j + l16;
-        // Phi from the Pm section (rows = points, exponent per point): the
-        // lane's four rows mu = 32 rh + 16 f + 4 q + r at point g are four
-        // consecutive bytes of each plane, one 4-byte load per plane
+            for (int r = 0; r < 4; ++r) {
+              double x = 0.0;
+#pragma unroll
+              for (int s2 = 0; s2 < GBT; ++s2)
+                if (s2 == sb) x = x64[s2][f][j][r];
+              Xs[32 * rh + 16 * f + 4 * q + r][16 * j + l16] = x;
+            }
+      }
+      __syncthreads();
+      {
+        const int g = threadIdx.x & 63, qq = threadIdx.x >> 6;
+        const signed char *tb = store + (sl - s0) * sb_bytes;
+        const float *G = reinterpret_cast<const float *>(tb + OFF_G);
         const int eP = tb[OFF_PM + 4 * PLANE + g];
+        const int off = frag8(g, 16 * qq);
+        v4i pw4[4];
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) pw4[pl] = ld16(tb + OFF_PM + pl * PLANE + off);
+        const signed char *c0 = reinterpret_cast<const signed char *>(&pw4[0]);
+        const signed char *c1 = reinterpret_cast<const signed char *>(&pw4[1]);
+        const signed char *c2 = reinterpret_cast<const signed char *>(&pw4[2]);
+        const signed char *c3 = reinterpret_cast<const signed char *>(&pw4[3]);
         double sm[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int f = 0; f < 2; ++f) {
-          const int off = frag8(g, 32 * rh + 16 * f + 4 * q);
-          int pw4[4];
-#pragma unroll
-          for (int pl = 0; pl < 4; ++pl)
-            pw4[pl] = *reinterpret_cast<const int *>(tb + OFF_PM + pl * PLANE + off);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int mu = 32 * rh + 16 * f + 4 * q + r;
-            double x = 0.0;
-#pragma unroll
-            for (int s2 = 0; s2 < GBT; ++s2)
-              if (s2 == sb) x = x64[s2][f][j][r];
-            double ph = 0.0;
-            if (eP != -127) {
-              const double v = (double)(signed char)(pw4[0] >> (8 * r)) +
-                               ldexp((double)(signed char)(pw4[1] >> (8 * r)), -7) +
-                               ldexp((double)(signed char)(pw4[2] >> (8 * r)), -14) +
-                               ldexp((double)(signed char)(pw4[3] >> (8 * r)), -21);
-              ph = ldexp(v, eP - 6);
-            }
-            sm[0] += ph * x;
-            if (gga) {
-              sm[1] += (double)G[mu * PT + g] * x;
-              sm[2] += (double)G[TILE + mu * PT + g] * x;
-              sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-            }
+        for (int e = 0; e < 16; ++e) {
+          const int mu = 16 * qq + e;
+          const double x = Xs[mu][g];
+          double ph = 0.0;
+          if (eP != -127) {
+            const double v = (double)c0[e] + ldexp((double)c1[e], -7) +
+                             ldexp((double)c2[e], -14) + ldexp((double)c3[e], -21);
+            ph = ldexp(v, eP - 6);
+          }
+          sm[0] += ph * x;
+          if (gga) {
+            sm[1] += (double)G[mu * PT + g] * x;
+            sm[2] += (double)G[TILE + mu * PT + g] * x;
+            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
           }
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = sm[c];
-          v += __shfl_xor(v, 16);
-          v += __shfl_xor(v, 32);
-          prt[j][c] = v;
-        }
+        for (int c = 0; c < 4; ++c) red[qq][c][g] = sm[c];
       }
-    }
-    if (q == 0) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        for (int c = 0; c < 4; ++c) red[wv][c][16 * j + l16] = prt[j][c];
-    }
-    __syncthreads();
-    {
-      const int m = threadIdx.x >> 7, c = (threadIdx.x >> 6) & 1, g = threadIdx.x & 63;
-      const int fmx = m ? fm1 : fm0;
-      const int sl = fmx >= 0 ? tab[sb][fmx] : -1;
-      if (sl >= 0) {
-#pragma unroll
-        for (int cc = 0; cc < 4; cc += 2) {
-          const int c2 = c + cc;
-          const double acc = c2 < nq ? red[2 * m][c2][g] + red[2 * m + 1][c2][g] : 0.0;
-          part[(sl - s0) * (4 * PT) + c2 * PT + g] = acc;
-        }
+      __syncthreads();
+      {
+        const int c = threadIdx.x >> 6, g = threadIdx.x & 63;
+        const double acc =
+            c < nq ? ((red[0][c][g] + red[1][c][g]) + red[2][c][g]) + red[3][c][g] : 0.0;
+        part[(sl - s0) * (4 * PT) + c * PT + g] = acc;
       }
+      __syncthreads();  // red and the X tile free for the next slot
     }
-    __syncthreads();
   }
 }
 
