@@ -186,12 +186,18 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   // largest molecule)
   if (!ABS && GBT == 1 && 2 * mgi >= b_off[bg + 1] - b_off[bg]) return;
   const int nsub = (int)(b1 - bg < GBT ? b1 - bg : GBT);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // wave-uniform values (the wave index, and the step lists and slot table
+  // read from LDS) go through readfirstlane, so that the branches on them
+  // are scalar branches
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int mb = wv >> 1, rh = wv & 1;
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
 
-  __shared__ __attribute__((aligned(16))) signed char bbuf[2][GBT][SEC];
+  // Pm sections of a step, two buffers, declared apart so that a copy into
+  // one is known not to touch the other
+  __shared__ __attribute__((aligned(16))) signed char bufA[GBT][SEC];
+  __shared__ __attribute__((aligned(16))) signed char bufB[GBT][SEC];
   __shared__ double red[4][4][PT];
   __shared__ int tab[GBT][MAXNFB];
   __shared__ int ublk[MAXNFB];
@@ -220,7 +226,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     if (lane == 0) nu_s = cnt;
   }
   __syncthreads();
-  const int nu = nu_s;
+  const int nu = __builtin_amdgcn_readfirstlane(nu_s);
   int fm0, fm1;
   if (ABS) {  // blocks 2 mgi, 2 mgi + 1 of the batch's molecule, if live
     const int lo = b_off[bg];
@@ -240,8 +246,8 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     fm1 = c;
   } else {
     if (mgi * 2 >= nu) return;  // uniform: no block of this pair
-    fm0 = ublk[mgi * 2];
-    fm1 = mgi * 2 + 1 < nu ? ublk[mgi * 2 + 1] : -1;
+    fm0 = __builtin_amdgcn_readfirstlane(ublk[mgi * 2]);
+    fm1 = mgi * 2 + 1 < nu ? __builtin_amdgcn_readfirstlane(ublk[mgi * 2 + 1]) : -1;
   }
   if (wv == 0) {
     int cnt = 0;
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     if (lane == 0) nst_s = cnt;
   }
   __syncthreads();
-  const int nst = nst_s;
+  const int nst = __builtin_amdgcn_readfirstlane(nst_s);
   const int nq = gga ? 4 : 1;
   const int fm = mb ? fm1 : fm0;
   constexpr unsigned ROW = (1u << GBT) - 1u;
@@ -288,13 +294,14 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     v4i v[2][4];
     int e[2];
   };
-  auto issue = [&](int i, int dbuf, AFrag &an) {
-    const int fk = ublk[sk[i]];
-    const unsigned pm = spm[i];
+  auto issue = [&](int i, signed char (*dst)[SEC], AFrag &an) {
+    const int fk = __builtin_amdgcn_readfirstlane(ublk[sk[i]]);
+    const unsigned pm = __builtin_amdgcn_readfirstlane(spm[i]);
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb) {
       if (!(((pm >> sb) | (pm >> (GBT + sb))) & 1u)) continue;
-      dma_section<4>(store + (tab[sb][fk] - s0) * sb_bytes + OFF_PM, &bbuf[dbuf][sb][0],
+      dma_section<4>(store + (__builtin_amdgcn_readfirstlane(tab[sb][fk]) - s0) * sb_bytes + OFF_PM,
+                     &dst[sb][0],
                      (wv + sb) & 3, lane);
     }
     if ((pm >> (mb * GBT)) & ROW) {
@@ -321,11 +328,12 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   // One step: wait for its operands, start the next step's copies into the
   // other buffer and register set, then the tile products.  Rows or points
   // stored as zero have zero slices, so their scale does not matter.
-  auto step = [&](int i, int buf, const AFrag &ac, AFrag &an) {
+  auto step = [&](int i, signed char (*cur)[SEC], signed char (*nxt)[SEC], const AFrag &ac,
+                  AFrag &an) {
     wait_vm0();
-    __syncthreads();  // buffer buf complete, buf ^ 1 free
-    const unsigned pmc = spm[i];
-    if (i + 1 < nst) issue(i + 1, buf ^ 1, an);
+    __syncthreads();  // buffer cur complete, nxt free
+    const unsigned pmc = __builtin_amdgcn_readfirstlane(spm[i]);
+    if (i + 1 < nst) issue(i + 1, nxt, an);
     double sA[2][4];
 #pragma unroll
     for (int f = 0; f < 2; ++f)
@@ -334,7 +342,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb) {
       if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
-      const signed char *bb = &bbuf[buf][sb][0];
+      const signed char *bb = &cur[sb][0];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         v4i B[4];
@@ -354,10 +362,10 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     }
   };
   AFrag a0, a1;
-  if (nst > 0) issue(0, 0, a0);
+  if (nst > 0) issue(0, bufA, a0);
   for (int i = 0; i < nst; i += 2) {
-    step(i, 0, a0, a1);
-    if (i + 1 < nst) step(i + 1, 1, a1, a0);
+    step(i, bufA, bufB, a0, a1);
+    if (i + 1 < nst) step(i + 1, bufB, bufA, a1, a0);
   }
 
   // Epilogue: per live slot (m, sb), column sums over the block's functions.
@@ -366,14 +374,17 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   // functions, so that the loads of grad Phi (rows of 64 points) and of the
   // Pm slices (16 functions of a point: one 16-byte load per plane) are whole
   // rows, and the four partial sums of a point are added in order.
-  double (*Xs)[65] = reinterpret_cast<double (*)[65]>(&bbuf[0][0][0]);
-  static_assert(sizeof(bbuf) >= sizeof(double) * 64 * 65, "X tile must fit the step buffers");
+  // rows 0..31 of the X tile over bufA, rows 32..63 over bufB
+  double (*XsA)[65] = reinterpret_cast<double (*)[65]>(&bufA[0][0]);
+  double (*XsB)[65] = reinterpret_cast<double (*)[65]>(&bufB[0][0]);
+  static_assert(sizeof(bufA) >= sizeof(double) * 32 * 65, "X tile must fit the step buffers");
+  auto Xs = [&](int mu) -> double * { return mu < 32 ? XsA[mu] : XsB[mu - 32]; };
   __syncthreads();  // the steps are done with the buffers
   for (int sb = 0; sb < nsub; ++sb) {
 #pragma unroll 1
     for (int m = 0; m < 2; ++m) {
       const int fmx = m ? fm1 : fm0;
-      const int sl = fmx >= 0 ? tab[sb][fmx] : -1;
+      const int sl = fmx >= 0 ? __builtin_amdgcn_readfirstlane(tab[sb][fmx]) : -1;
       if (sl < 0) continue;  // uniform
       if (mb == m) {
 #pragma unroll
@@ -386,7 +397,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
               for (int s2 = 0; s2 < GBT; ++s2)
                 if (s2 == sb) x = x64[s2][f][j][r];
-              Xs[32 * rh + 16 * f + 4 * q + r][16 * j + l16] = x;
+              Xs(32 * rh + 16 * f + 4 * q + r)[16 * j + l16] = x;
             }
       }
       __syncthreads();
@@ -407,7 +418,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
           const int mu = 16 * qq + e;
-          const double x = Xs[mu][g];
+          const double x = Xs(mu)[g];
           double ph = 0.0;
           if (eP != -127) {
             const double v = (double)c0[e] + ldexp((double)c1[e], -7) +
