@@ -110,29 +110,111 @@ __global__ void __launch_bounds__(256) colloc_tiles_i8(
 }
 
 // P (FP64, original order) -> one section per 64 x 64 block pair (fm, fk),
-// permuted and zero-padded; one workgroup per block pair.
+// permuted and zero-padded; one workgroup per block pair.  The row exponents
+// are shared by the KG blocks fk of a group (fk / KG): each row's maximum is
+// taken over the group's columns, so that the contraction can sum the exact
+// int32 slice products of a group's steps before one FP64 combine.
 __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
                                                    const int *__restrict__ fperm,
                                                    const double *__restrict__ P,
                                                    signed char *__restrict__ Ps) {
   const int fm = blockIdx.x / nfb, fk = blockIdx.x % nfb;
   __shared__ double v[FB][65];
+  __shared__ double rmax[4][FB];
   __shared__ int ex[FB];
   for (int e = threadIdx.x; e < TILE; e += 256) {
     const int r = e / FB, c = e % FB;
     const int i = fm * FB + r, j = fk * FB + c;
     v[r][c] = (i < nbf && j < nbf) ? P[(int64_t)fperm[i] * nbf + fperm[j]] : 0.0;
   }
+  {
+    // row maxima over the columns of the group's blocks: thread (row, part)
+    const int r = threadIdx.x & (FB - 1), part = threadIdx.x >> 6;
+    const int i = fm * FB + r;
+    const int c0 = (fk / KG) * KG * FB;
+    const int c1 = min(c0 + KG * FB, nbf);
+    double m = 0.0;
+    if (i < nbf)
+      for (int j = c0 + part; j < c1; j += 4)
+        m = fmax(m, fabs(P[(int64_t)fperm[i] * nbf + fperm[j]]));
+    rmax[part][r] = m;
+  }
   __syncthreads();
   if (threadIdx.x < FB) {
-    double m = 0.0;
-    for (int k = 0; k < FB; ++k) m = fmax(m, fabs(v[threadIdx.x][k]));
+    const double m = fmax(fmax(rmax[0][threadIdx.x], rmax[1][threadIdx.x]),
+                          fmax(rmax[2][threadIdx.x], rmax[3][threadIdx.x]));
     ex[threadIdx.x] = row_exp(m);
   }
   __syncthreads();
   store_section(v, ex, Ps + (int64_t)blockIdx.x * SEC, false);
 }
 
+
+// Per batch, the Pm sections of the blocks of one exponent group (local block
+// index / KG) are re-sliced to a common exponent per point (the group's
+// largest): one workgroup of 256 threads per batch; thread t takes point row
+// t % 64 and the 16 functions 16 (t / 64) .. +15 of every section of the
+// group (16 contiguous bytes per plane).  An element's four digits give the
+// exact integer V = d0 2^21 + d1 2^14 + d2 2^7 + d3; V 2^-s is rounded (half
+// to even, so that the sums over points see no bias) and split again into
+// balanced base-128 digits.  A batch's slots come in ascending block order,
+// so a group is a run of consecutive slots.
+__global__ void __launch_bounds__(256) pm_group_i8(int64_t s0, int64_t b0,
+                                                   const int *__restrict__ b_off,
+                                                   const int *__restrict__ b_fbl, MolTab mt,
+                                                   int gga, signed char *__restrict__ store) {
+  const int64_t b = b0 + blockIdx.x;
+  const int lo = b_off[b], hi = b_off[b + 1];
+  const int pr = threadIdx.x & 63, ch = threadIdx.x >> 6;
+  const int off = frag8(pr, 16 * ch);
+  const int64_t sb = slot_bytes_i8(gga);
+  int k = lo;
+  while (k < hi) {
+    const int g = mt.blk_local[b_fbl[k]] / KG;
+    int ke = k, em = -127;
+    for (; ke < hi && mt.blk_local[b_fbl[ke]] / KG == g; ++ke)
+      em = max(em, (int)store[(ke - s0) * sb + OFF_PM + 4 * PLANE + pr]);
+    for (int kk = k; kk < ke; ++kk) {
+      signed char *sec = store + (kk - s0) * sb + OFF_PM;
+      const int sh = em - (int)sec[4 * PLANE + pr];
+      if (sh <= 0) continue;
+      v4i w[4];
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) w[pl] = ld16(sec + pl * PLANE + off);
+      const signed char *c0 = reinterpret_cast<const signed char *>(&w[0]);
+      const signed char *c1 = reinterpret_cast<const signed char *>(&w[1]);
+      const signed char *c2 = reinterpret_cast<const signed char *>(&w[2]);
+      const signed char *c3 = reinterpret_cast<const signed char *>(&w[3]);
+      __attribute__((aligned(16))) signed char out[4][16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        int V = (int)c0[e] * (1 << 21) + (int)c1[e] * (1 << 14) + (int)c2[e] * (1 << 7) +
+                (int)c3[e];
+        if (sh > 28) {
+          V = 0;
+        } else {
+          const int qv = V >> sh, rv = V - qv * (1 << sh), half = 1 << (sh - 1);
+          V = qv + ((rv > half || (rv == half && (qv & 1))) ? 1 : 0);
+        }
+#pragma unroll
+        for (int pl = 3; pl > 0; --pl) {
+          const int d = ((V + 64) & 127) - 64;
+          out[pl][e] = (signed char)d;
+          V = (V - d) >> 7;
+        }
+        out[0][e] = (signed char)V;
+      }
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl)
+        *reinterpret_cast<v4i *>(sec + pl * PLANE + off) = *reinterpret_cast<const v4i *>(out[pl]);
+    }
+    __syncthreads();  // every thread has read the old exponents of the group
+    if (ch == 0)
+      for (int kk = k; kk < ke; ++kk)
+        store[(kk - s0) * sb + OFF_PM + 4 * PLANE + pr] = (signed char)em;
+    k = ke;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Density stage, three kernels: the contraction (density_x2_i8), per batch
@@ -165,6 +247,7 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int64_t b1, int nmg, int tpn,
     int tbn, int gga, const signed char *__restrict__ store, double *__restrict__ part) {
   static_assert(!ABS || GBT == 1, "absolute pairs take one batch per workgroup");
+  static_assert(GBT <= 3, "step masks take bits 0 .. 2 GBT - 1, bit 7 ends a group");
   const int xcd = blockIdx.x & 7;
   const int rr = blockIdx.x >> 3;
   int mgi = 0;
@@ -201,8 +284,12 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   __shared__ double red[4][4][PT];
   __shared__ int tab[GBT][MAXNFB];
   __shared__ int ublk[MAXNFB];
-  __shared__ short sk[MAXNFB];            // active steps: union index of k
-  __shared__ unsigned char spm[MAXNFB];   // and mask, bit (m * GBT + sb)
+  // active steps, one record each: k | mask << 16 (mask bit m * GBT + sb:
+  // block m and batch sb take part; bit 7: last step of an exponent group),
+  // then the slot of (k, sb) per batch (-1 where k is dead in sb)
+  constexpr int SR = GBT == 1 ? 2 : 4;
+  typedef int Rec __attribute__((ext_vector_type(SR)));
+  __shared__ __attribute__((aligned(16))) int srec[MAXNFB][SR];
   __shared__ int nu_s, nst_s;
 
   for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
@@ -266,10 +353,26 @@ __global__ void __launch_bounds__(256) density_x2_i8(
       const unsigned long long m = __ballot(pm != 0);
       if (pm) {
         const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        sk[i] = (short)ki;
-        spm[i] = (unsigned char)pm;
+        const int fk = ublk[ki];
+        srec[i][0] = fk | (int)(pm << 16);
+#pragma unroll
+        for (int sb = 0; sb < SR - 1; ++sb)
+          srec[i][1 + sb] = sb < nsub ? tab[sb < GBT ? sb : 0][fk] : -1;
       }
       cnt += __popcll(m);
+    }
+    // bit 7 of a step's mask: the last step of its exponent group (blocks
+    // with the same local index / KG of one molecule; the steps are in
+    // ascending block order, so a group is a run of steps)
+    for (int i = lane; i < cnt; i += 64) {
+      const int f = srec[i][0] & 0xFFFF;
+      const int key = f - mt.blk_local[f] % KG;
+      bool last = i + 1 == cnt;
+      if (!last) {
+        const int f2 = srec[i + 1][0] & 0xFFFF;
+        last = f2 - mt.blk_local[f2] % KG != key;
+      }
+      if (last) srec[i][0] |= 0x80 << 16;
     }
     if (lane == 0) nst_s = cnt;
   }
@@ -294,13 +397,20 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     v4i v[2][4];
     int e[2];
   };
-  auto issue = [&](int i, signed char (*dst)[SEC], AFrag &an) {
-    const int fk = __builtin_amdgcn_readfirstlane(ublk[sk[i]]);
-    const unsigned pm = __builtin_amdgcn_readfirstlane(spm[i]);
+  // a step's record, read one step before its copies are issued (so that no
+  // LDS look-up sits between a barrier and the copies); past the end: any
+  // record in bounds, never used
+  auto rec = [&](int i) -> Rec {
+    return *reinterpret_cast<const Rec *>(&srec[i < nst ? i : 0][0]);
+  };
+  auto issue = [&](const Rec &rn, signed char (*dst)[SEC], AFrag &an) {
+    const int r0 = __builtin_amdgcn_readfirstlane(rn[0]);
+    const int fk = r0 & 0xFFFF;
+    const unsigned pm = ((unsigned)r0 >> 16) & 0xFFu;
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb) {
       if (!(((pm >> sb) | (pm >> (GBT + sb))) & 1u)) continue;
-      dma_section<4>(store + (__builtin_amdgcn_readfirstlane(tab[sb][fk]) - s0) * sb_bytes + OFF_PM,
+      dma_section<4>(store + (__builtin_amdgcn_readfirstlane(rn[1 + sb]) - s0) * sb_bytes + OFF_PM,
                      &dst[sb][0],
                      (wv + sb) & 3, lane);
     }
@@ -316,29 +426,44 @@ __global__ void __launch_bounds__(256) density_x2_i8(
     }
   };
 
+  // x64: FP64 sums over the groups done; hacc, lacc: exact int32 sums of the
+  // current group's tile products (hi = 128 d0 + d1, lo = 128 d2 + d3), whose
+  // row and point exponents (eAg from the P fragments, eBg from the Pm
+  // sections) are the same for every step of the group
   double x64[GBT][2][4][4];
+  v4i hacc[GBT][2][4], lacc[GBT][2][4];
+  int eAg[2] = {0, 0};
+  int eBg[GBT][4];
 #pragma unroll
   for (int sb = 0; sb < GBT; ++sb)
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j) {
+        hacc[sb][f][j] = v4i{0, 0, 0, 0};
+        lacc[sb][f][j] = v4i{0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) x64[sb][f][j][r] = 0.0;
+      }
+#pragma unroll
+  for (int sb = 0; sb < GBT; ++sb)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) eBg[sb][j] = 0;
   // One step: wait for its operands, start the next step's copies into the
-  // other buffer and register set, then the tile products.  Rows or points
-  // stored as zero have zero slices, so their scale does not matter.
-  auto step = [&](int i, signed char (*cur)[SEC], signed char (*nxt)[SEC], const AFrag &ac,
-                  AFrag &an) {
+  // other buffer and register set, then the tile products into the group's
+  // int32 sums; after a group's last step, the sums are scaled into x64 once.
+  // Rows or points stored as zero have zero slices, so their scale does not
+  // matter.
+  auto step = [&](int i, const Rec &rc, const Rec &rn, signed char (*cur)[SEC],
+                  signed char (*nxt)[SEC], const AFrag &ac, AFrag &an) {
     wait_vm0();
     __syncthreads();  // buffer cur complete, nxt free
-    const unsigned pmc = __builtin_amdgcn_readfirstlane(spm[i]);
-    if (i + 1 < nst) issue(i + 1, nxt, an);
-    double sA[2][4];
-#pragma unroll
-    for (int f = 0; f < 2; ++f)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sA[f][r] = pow2((int)(signed char)(ac.e[f] >> (8 * r)));
+    const unsigned pmc = ((unsigned)__builtin_amdgcn_readfirstlane(rc[0]) >> 16) & 0xFFu;
+    if (i + 1 < nst) issue(rn, nxt, an);
+    if ((pmc >> (mb * GBT)) & ROW) {
+      eAg[0] = ac.e[0];
+      eAg[1] = ac.e[1];
+    }
 #pragma unroll
     for (int sb = 0; sb < GBT; ++sb) {
       if (!((pmc >> (mb * GBT + sb)) & 1u)) continue;
@@ -348,24 +473,39 @@ __global__ void __launch_bounds__(256) density_x2_i8(
         v4i B[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) B[p] = ld16(bb + p * PLANE + (j * 64 + lane) * 16);
-        const double sB = pow2((int)bb[4 * PLANE + 16 * j + l16] - 19);
+        eBg[sb][j] = bb[4 * PLANE + 16 * j + l16];
 #pragma unroll
-        for (int f = 0; f < 2; ++f) {
-          int hi[4], lo[4];
-          tile_product(ac.v[f], B, hi, lo);
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            x64[sb][f][j][r] = fma(fma((double)lo[r], 0x1p-14, (double)hi[r]), sA[f][r] * sB,
-                                   x64[sb][f][j][r]);
-        }
+        for (int f = 0; f < 2; ++f) tile_product_acc(ac.v[f], B, hacc[sb][f][j], lacc[sb][f][j]);
       }
+    }
+    if (pmc & 0x80u) {
+#pragma unroll
+      for (int sb = 0; sb < GBT; ++sb)
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              x64[sb][f][j][r] =
+                  fma(fma((double)lacc[sb][f][j][r], 0x1p-14, (double)hacc[sb][f][j][r]),
+                      pow2((int)(signed char)(eAg[f] >> (8 * r)) + eBg[sb][j] - 19),
+                      x64[sb][f][j][r]);
+            hacc[sb][f][j] = v4i{0, 0, 0, 0};
+            lacc[sb][f][j] = v4i{0, 0, 0, 0};
+          }
     }
   };
   AFrag a0, a1;
-  if (nst > 0) issue(0, bufA, a0);
+  Rec rA = rec(0), rB = rec(1);
+  if (nst > 0) issue(rA, bufA, a0);
   for (int i = 0; i < nst; i += 2) {
-    step(i, bufA, bufB, a0, a1);
-    if (i + 1 < nst) step(i + 1, bufB, bufA, a1, a0);
+    step(i, rA, rB, bufA, bufB, a0, a1);
+    rA = rec(i + 2);
+    if (i + 1 < nst) {
+      step(i + 1, rB, rA, bufB, bufA, a1, a0);
+      rB = rec(i + 3);
+    }
   }
 
   // Epilogue: per live slot (m, sb), column sums over the block's functions.
@@ -402,35 +542,44 @@ __global__ void __launch_bounds__(256) density_x2_i8(
       }
       __syncthreads();
       {
+        // every global load of the slot is issued before the first use
+        // (a branch between them would make each wait for the one before)
         const int g = threadIdx.x & 63, qq = threadIdx.x >> 6;
         const signed char *tb = store + (sl - s0) * sb_bytes;
-        const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+        const float *G = reinterpret_cast<const float *>(tb + OFF_G) + 16 * qq * PT + g;
         const int eP = tb[OFF_PM + 4 * PLANE + g];
         const int off = frag8(g, 16 * qq);
         v4i pw4[4];
 #pragma unroll
         for (int pl = 0; pl < 4; ++pl) pw4[pl] = ld16(tb + OFF_PM + pl * PLANE + off);
+        float gv[3][16];
+        if (gga) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) gv[c][e] = G[c * TILE + e * PT];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) gv[c][e] = 0.0f;
+        }
         const signed char *c0 = reinterpret_cast<const signed char *>(&pw4[0]);
         const signed char *c1 = reinterpret_cast<const signed char *>(&pw4[1]);
         const signed char *c2 = reinterpret_cast<const signed char *>(&pw4[2]);
         const signed char *c3 = reinterpret_cast<const signed char *>(&pw4[3]);
+        // 2^(eP - 6) scales the slices exactly; 0 for a point stored as zero
+        const double sP = pow2_or_zero(eP == -127 ? -127 : eP - 6);
         double sm[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-          const int mu = 16 * qq + e;
-          const double x = Xs(mu)[g];
-          double ph = 0.0;
-          if (eP != -127) {
-            const double v = (double)c0[e] + ldexp((double)c1[e], -7) +
-                             ldexp((double)c2[e], -14) + ldexp((double)c3[e], -21);
-            ph = ldexp(v, eP - 6);
-          }
-          sm[0] += ph * x;
-          if (gga) {
-            sm[1] += (double)G[mu * PT + g] * x;
-            sm[2] += (double)G[TILE + mu * PT + g] * x;
-            sm[3] += (double)G[2 * TILE + mu * PT + g] * x;
-          }
+          const double x = Xs(16 * qq + e)[g];
+          const double v = (double)c0[e] + ldexp((double)c1[e], -7) +
+                           ldexp((double)c2[e], -14) + ldexp((double)c3[e], -21);
+          sm[0] += (v * sP) * x;
+          sm[1] += (double)gv[0][e] * x;
+          sm[2] += (double)gv[1][e] * x;
+          sm[3] += (double)gv[2][e] * x;
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) red[qq][c][g] = sm[c];
@@ -571,32 +720,41 @@ __global__ void __launch_bounds__(256) z_slots_i8(int64_t s0, const int *__restr
   for (int p = 0; p < 4; ++p) fp[p] = ld16(tb + OFF_F + p * PLANE + t * 16);
   const int eF = tb[OFF_F + 4 * PLANE + row];
   const float *G = reinterpret_cast<const float *>(tb + OFF_G) + row * PT + k0;
+  // every load is issued before the first use (a branch between them would
+  // make each wait for the one before)
+  float4 gq[3][4];
+  if (gga) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4)
+        gq[c][q4] = *reinterpret_cast<const float4 *>(G + c * TILE + 4 * q4);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) gq[c][q4] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   __syncthreads();
   const signed char *c0 = reinterpret_cast<const signed char *>(&fp[0]);
   const signed char *c1 = reinterpret_cast<const signed char *>(&fp[1]);
   const signed char *c2 = reinterpret_cast<const signed char *>(&fp[2]);
   const signed char *c3 = reinterpret_cast<const signed char *>(&fp[3]);
+  // 2^(eF - 6) scales the slices exactly; 0 for a row stored as zero
+  const double sF = pow2_or_zero(eF == -127 ? -127 : eF - 6);
   double z[16];
   double m = 0.0;
 #pragma unroll
   for (int q4 = 0; q4 < 4; ++q4) {
-    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx, gz = gx;
-    if (gga) {
-      gx = *reinterpret_cast<const float4 *>(G + 4 * q4);
-      gy = *reinterpret_cast<const float4 *>(G + TILE + 4 * q4);
-      gz = *reinterpret_cast<const float4 *>(G + 2 * TILE + 4 * q4);
-    }
-    const float gxa[4] = {gx.x, gx.y, gx.z, gx.w}, gya[4] = {gy.x, gy.y, gy.z, gy.w},
-                gza[4] = {gz.x, gz.y, gz.z, gz.w};
+    const float gxa[4] = {gq[0][q4].x, gq[0][q4].y, gq[0][q4].z, gq[0][q4].w},
+                gya[4] = {gq[1][q4].x, gq[1][q4].y, gq[1][q4].z, gq[1][q4].w},
+                gza[4] = {gq[2][q4].x, gq[2][q4].y, gq[2][q4].z, gq[2][q4].w};
 #pragma unroll
     for (int e4 = 0; e4 < 4; ++e4) {
       const int e = 4 * q4 + e4, g = k0 + e;
-      double ph = 0.0;
-      if (eF != -127) {
-        const double v = (double)c0[e] + ldexp((double)c1[e], -7) + ldexp((double)c2[e], -14) +
-                         ldexp((double)c3[e], -21);
-        ph = ldexp(v, eF - 6);
-      }
+      const double v = (double)c0[e] + ldexp((double)c1[e], -7) + ldexp((double)c2[e], -14) +
+                       ldexp((double)c3[e], -21);
+      const double ph = v * sF;
       const double zz = gga ? z_value(ph, (double)gxa[e4], (double)gya[e4], (double)gza[e4],
                                       cf[g][0], cf[g][1], cf[g][2], cf[g][3])
                             : cf[g][0] * ph;
@@ -839,6 +997,14 @@ void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const doubl
 }
 
 int64_t mixed_p_bytes(int nfb) { return (int64_t)nfb * nfb * SEC; }
+
+void vkm_pm_group(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int gga,
+                  signed char *store) {
+  if (b1 <= b0) return;
+  hipLaunchKernelGGL(pm_group_i8, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, s0, b0, pl.b_off.p,
+                     pl.b_fbl.p, pl.tab(), gga, store);
+  NKX_HIP_CHECK(hipGetLastError());
+}
 
 void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
                 signed char *store) {

@@ -53,6 +53,10 @@ constexpr int PLANE = TILE;                  // bytes per slice plane
 constexpr int SEC = 4 * PLANE + 256;         // section: 4 planes + exponents (padded)
 constexpr int OFF_F = 0, OFF_PM = SEC, OFF_Z = 2 * SEC, OFF_G = 3 * SEC;
 constexpr int MAXNFB = 1024;
+// function blocks per exponent group: the P sections of blocks (fm, fk) with
+// the same fk / KG share row exponents, and a batch's Pm sections of blocks
+// with the same local index / KG share point exponents
+constexpr int KG = 4;
 
 __host__ __device__ inline int64_t slot_bytes_i8(int gga) {
   return 3 * (int64_t)SEC + (gga ? 3 * (int64_t)TILE * 4 : 0);
@@ -114,6 +118,30 @@ __device__ __forceinline__ void tile_product(const v4i *A, const v4i *B, int hi[
   d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[1], d2, 0, 0, 0);
   d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[0], d2, 0, 0, 0);
   v4i d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[3], z, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[2], d3, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[1], d3, 0, 0, 0);
+  d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[3], B[0], d3, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    hi[r] = d0[r] * 128 + d1[r];
+    lo[r] = d2[r] * 128 + d3[r];
+  }
+}
+
+// The same ten slice products added to running sums hi, lo of earlier tile
+// products with the same row and column exponents: the d1 and d3 chains start
+// from hi and lo, so hi = 128 d0 + d1 and lo = 128 d2 + d3 stay exact int32
+// sums.  Over n products |hi| <= n (128 + 2) 2^18 and |lo| <= n (3 * 128 + 4)
+// 2^18, both below 2^31 for n <= 16.
+__device__ __forceinline__ void tile_product_acc(const v4i *A, const v4i *B, v4i &hi, v4i &lo) {
+  const v4i z = {0, 0, 0, 0};
+  const v4i d0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[0], z, 0, 0, 0);
+  v4i d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[1], hi, 0, 0, 0);
+  d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[0], d1, 0, 0, 0);
+  v4i d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[2], z, 0, 0, 0);
+  d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[1], d2, 0, 0, 0);
+  d2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[0], d2, 0, 0, 0);
+  v4i d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[0], B[3], lo, 0, 0, 0);
   d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[1], B[2], d3, 0, 0, 0);
   d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[2], B[1], d3, 0, 0, 0);
   d3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[3], B[0], d3, 0, 0, 0);

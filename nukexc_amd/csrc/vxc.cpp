@@ -609,9 +609,10 @@ void enqueue_build(Ctx &c, Workspace &ws, const Plan &pl, const BuildSetup &b,
     int tt = tm.begin(kStageColloc, s);
     if (bf16)
       vkb_colloc(s, s0, s1, pl, ig, bstore);
-    else if (mixed)
+    else if (mixed) {
       vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
-    else
+      vkm_pm_group(s, b0, b1, s0, pl, ig, ws.mstore.p);
+    } else
       vk_colloc(s, s0, s1, pl, narr, ig, ws.tiles64.p);
     tm.end(tt, s);
     tt = tm.begin(kStageDensity, s);
@@ -980,6 +981,7 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
       // with w e per point), Y slices (the first spin's over the Z sections),
       // then the basis-centre term per spin
       vkm_colloc(s, s0, s1, pl, ig, ws.mstore.p);
+      vkm_pm_group(s, b0, b1, s0, pl, ig, ws.mstore.p);
       for (int sp = 0; sp < nspin; ++sp)
         vkm_density(s, 65, b0, b1, s0, pl, ws.Ps.p + sp * psb, ig, ws.mstore.p, xl, ws.coef.p,
                     ws.blockEN.p, ws.dpart.p + sp * dsz);

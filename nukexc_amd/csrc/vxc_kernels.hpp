@@ -54,6 +54,9 @@ void vkm_gather_p(hipStream_t s, int nbf, int nfb, const int *fperm, const doubl
                   signed char *Ps);
 void vkm_colloc(hipStream_t s, int64_t s0, int64_t s1, const Plan &pl, int gga,
                 signed char *store);
+// Pm sections of a batch's exponent groups re-sliced to shared point exponents
+void vkm_pm_group(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl, int gga,
+                  signed char *store);
 // density shape codes of the int8 form (other values run as 65)
 bool mixed_density_shape_ok(int shape);
 void vkm_density(hipStream_t s, int shape, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
