@@ -906,26 +906,34 @@ __global__ void __launch_bounds__(512) vgemm_i8(
     next_F(0, n);
   };
   // one phase of a batch: A = F_I (p = 0) or Z_I (p = 1) against the Z or F
-  // section of J_b
+  // section of J_b; the B fragment of column fragment j + 1 is read from LDS
+  // while the products of j run (two B register sets)
   auto phase = [&](int p, signed char (*cur)[SEC]) {
     const signed char *sec = &cur[2 * wb + p][0];
+    // rows and columns stored as zero have zero slices: no zero test on the
+    // scales
+    double sc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sc[j] = pow2((int)sec[4 * PLANE + 16 * j + l16] - 19);
+    v4i B[2][4];
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) B[0][pl] = ld16(sec + pl * PLANE + lane * 16);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      v4i B[4];
+      if (j < 3) {
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) B[pl] = ld16(sec + pl * PLANE + (j * 64 + lane) * 16);
-      // rows and columns stored as zero have zero slices: no zero test on
-      // the scales
-      const double sc = pow2((int)sec[4 * PLANE + 16 * j + l16] - 19);
+        for (int pl = 0; pl < 4; ++pl)
+          B[(j + 1) & 1][pl] = ld16(sec + pl * PLANE + ((j + 1) * 64 + lane) * 16);
+      }
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
         int h[4], o[4];
-        tile_product(p ? Zr[g] : Fr[g], B, h, o);
+        tile_product(p ? Zr[g] : Fr[g], B[j & 1], h, o);
         const int er = p ? eZ[g] : eF[g];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           v64[g][j][r] = fma(fma((double)o[r], 0x1p-14, (double)h[r]),
-                             pow2((int)(signed char)(er >> (8 * r))) * sc, v64[g][j][r]);
+                             pow2((int)(signed char)(er >> (8 * r))) * sc[j], v64[g][j][r]);
       }
     }
   };
