@@ -91,10 +91,14 @@ __global__ void __launch_bounds__(256) y_slots_i8(int64_t s0, const int *__restr
 // X and W accumulate over the live blocks K: wave w takes rows 16 w .. +15 of
 // I (its P fragment of pair (I, K) in registers), every point of the batch
 // (four 16-point fragments of the K slot's Pm and Y sections), the ten slice
-// products per tile and an FP64 combine per K.  The epilogue is the FP64
-// form's: STO gradient (and Hessian for GGA) in FP64 per (mu, g), per-function
-// sums over the batch's points into cpart, per-point sums over functions into
-// tpt (with the sign of the point-motion term).
+// products per tile and an FP64 combine per K.  The steps (I, K) run in that
+// order; the K slot's Pm and Y sections are copied into LDS once per step for
+// the four waves (two buffers: the next step's copies are issued while the
+// current step computes; a second register set for the P fragments would
+// cost the occupancy).  The epilogue, after the last K of
+// an I, is the FP64 form's: STO gradient (and Hessian for GGA) in FP64 per
+// (mu, g), per-function sums over the batch's points into cpart, per-point
+// sums over functions into tpt (with the sign of the point-motion term).
 __global__ void __launch_bounds__(256) grad_basis_i8(
     MolTab mt, const signed char *__restrict__ Ps, const int *__restrict__ b_off,
     const int *__restrict__ b_fbl, int64_t s0, int64_t b0, int gga,
@@ -105,46 +109,137 @@ __global__ void __launch_bounds__(256) grad_basis_i8(
     int accum, double *__restrict__ cpart, double *__restrict__ tpt) {
   const int b = (int)(b0 + blockIdx.x);
   const int lo = b_off[b], hi = b_off[b + 1];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, q = lane >> 4;
   const int64_t sb_bytes = slot_bytes_i8(gga);
 
+  // Pm and Y sections of a step's K slot, two buffers, declared apart so that
+  // a copy into one is known not to touch the other
+  __shared__ __attribute__((aligned(16))) signed char bufA[2][SEC];
+  __shared__ __attribute__((aligned(16))) signed char bufB[2][SEC];
   __shared__ double cfs[4][PT];
   __shared__ double red[4][3][PT];
   __shared__ double tsum[3][PT];
+  __shared__ int lbl[MAXNFB];  // local block index of the batch's live blocks
   for (int e = threadIdx.x; e < 4 * PT; e += 256) {
     const int c = e / PT, g = e % PT;
     cfs[c][g] = coef[4 * ((int64_t)b * PT + g) + c];
   }
   for (int e = threadIdx.x; e < 3 * PT; e += 256) (&tsum[0][0])[e] = 0.0;
+  const int n = hi - lo;
+  for (int k = threadIdx.x; k < n; k += 256) lbl[k] = mt.blk_local[b_fbl[lo + k]];
   __syncthreads();
-
-  for (int mi = lo; mi < hi; ++mi) {
-    const int fm = b_fbl[mi];
-    const int mol = mt.blk_mol[fm];
-    const signed char *prow =
-        Ps + mt.Ps_off[mol] + (int64_t)mt.blk_local[fm] * mt.nfb[mol] * SEC;
+  if (n > 0) {
+    // the batch's blocks are blocks of one molecule
+    const int mol = mt.blk_mol[b_fbl[lo]];
+    const signed char *pbase = Ps + mt.Ps_off[mol];
+    const int nfbm = mt.nfb[mol];
+    const int nsteps = n * n;
+    // the K slot's sections of a step -> LDS buffer dst (direct copies)
+    auto issue = [&](int st, signed char (*dst)[SEC]) {
+      const int64_t sk = lo + st % n - s0;
+      dma_section<4>(store + sk * sb_bytes + OFF_PM, &dst[0][0], wv, lane);
+      dma_section<4>(ybase + sk * ystride, &dst[1][0], (wv + 2) & 3, lane);
+    };
     double X[4][4], W[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) X[j][r] = W[j][r] = 0.0;
-    for (int ki = lo; ki < hi; ++ki) {
-      const int fk = b_fbl[ki];
-      const signed char *pa = prow + (int64_t)mt.blk_local[fk] * SEC;
+    // Epilogue of block I = b_fbl[mi]: c_mu(g) for the 16 (mu, g) pairs of
+    // this lane; result register r of lane l holds row 4 (l / 16) + r, point
+    // l % 16 of a tile.
+    auto epilogue = [&](int mi) {
+      const int fm = b_fbl[mi];
+      double cm[4][3], tp[4][3];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 3; ++k) cm[r][k] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 3; ++k) tp[j][k] = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = 16 * wv + 4 * q + r;
+        const FuncRec rec = F[fm * FB + mu];
+        if (rec.norm == 0.0) continue;  // padding function
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int g = 16 * j + l16;
+          const int64_t gp = (int64_t)b * PT + g;
+          double c[3];
+          if (gga) {
+            double v, gr[3], h[6];
+            sto_value_grad_hess(rec, T, px[gp], py[gp], pz[gp], v, gr, h);
+            const double bx = cfs[1][g], by = cfs[2][g], bz = cfs[3][g];
+            const double x = X[j][r], w = W[j][r];
+            c[0] = -2.0 * (gr[0] * w + (h[0] * bx + h[1] * by + h[2] * bz) * x);
+            c[1] = -2.0 * (gr[1] * w + (h[1] * bx + h[3] * by + h[4] * bz) * x);
+            c[2] = -2.0 * (gr[2] * w + (h[2] * bx + h[4] * by + h[5] * bz) * x);
+          } else {
+            double v, gx, gy, gz;
+            sto_value_grad<double>(rec, T, px[gp], py[gp], pz[gp], v, gx, gy, gz);
+            const double w = W[j][r];
+            c[0] = -2.0 * gx * w;
+            c[1] = -2.0 * gy * w;
+            c[2] = -2.0 * gz * w;
+          }
+          for (int k = 0; k < 3; ++k) {
+            cm[r][k] += c[k];
+            tp[j][k] += c[k];
+          }
+        }
+      }
+      // per function: sum over the batch's points (j in registers, l16 lanes)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 3; ++k) {
+          double v = cm[r][k];
+          v += __shfl_xor(v, 1);
+          v += __shfl_xor(v, 2);
+          v += __shfl_xor(v, 4);
+          v += __shfl_xor(v, 8);
+          if (l16 == 0) cpart[(mi - s0) * GC + (16 * wv + 4 * q + r) * 3 + k] = v;
+        }
+      // per point: sum over this block's functions (q lanes, then waves)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 3; ++k) {
+          double v = tp[j][k];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          if (q == 0) red[wv][k][16 * j + l16] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x < 3 * PT) {
+        const int k = threadIdx.x / PT, g = threadIdx.x % PT;
+        tsum[k][g] += red[0][k][g] + red[1][k][g] + red[2][k][g] + red[3][k][g];
+      }
+      __syncthreads();
+    };
+    // one step (I, K): its P fragment, wait for its sections, issue the next
+    // step's copies, then the tile products; after the last K of an I, its
+    // epilogue
+    auto step = [&](int st, signed char (*cur)[SEC], signed char (*nxt)[SEC]) {
+      const int ii = st / n, kk = st - ii * n;
+      const signed char *pa =
+          pbase + ((int64_t)__builtin_amdgcn_readfirstlane(lbl[ii]) * nfbm +
+                   __builtin_amdgcn_readfirstlane(lbl[kk])) * SEC;
       v4i A[4];
 #pragma unroll
       for (int p = 0; p < 4; ++p) A[p] = ld16(pa + p * PLANE + (wv * 64 + lane) * 16);
       const int eA = *reinterpret_cast<const int *>(pa + 4 * PLANE + 16 * wv + 4 * q);
+      wait_vm0();
+      __syncthreads();  // buffer cur complete, nxt free
+      if (st + 1 < nsteps) issue(st + 1, nxt);
       double sA[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) sA[r] = pow2_or_zero((int)(signed char)(eA >> (8 * r)));
-      const signed char *tb = store + (ki - s0) * sb_bytes;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
 #pragma unroll
         for (int op = 0; op < 2; ++op) {
-          const signed char *sec = op ? ybase + (ki - s0) * ystride : tb + OFF_PM;
+          const signed char *sec = &cur[op][0];
           v4i B[4];
 #pragma unroll
           for (int p = 0; p < 4; ++p) B[p] = ld16(sec + p * PLANE + (j * 64 + lane) * 16);
@@ -162,74 +257,19 @@ __global__ void __launch_bounds__(256) grad_basis_i8(
           }
         }
       }
-    }
-    // Epilogue: c_mu(g) for the 16 (mu, g) pairs of this lane; result
-    // register r of lane l holds row 4 (l / 16) + r, point l % 16 of a tile.
-    double cm[4][3], tp[4][3];
+      if (kk == n - 1) {  // uniform: the last K of block I
+        epilogue(lo + ii);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      for (int k = 0; k < 3; ++k) cm[r][k] = 0.0;
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 3; ++k) tp[j][k] = 0.0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int mu = 16 * wv + 4 * q + r;
-      const FuncRec rec = F[fm * FB + mu];
-      if (rec.norm == 0.0) continue;  // padding function
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int g = 16 * j + l16;
-        const int64_t gp = (int64_t)b * PT + g;
-        double c[3];
-        if (gga) {
-          double v, gr[3], h[6];
-          sto_value_grad_hess(rec, T, px[gp], py[gp], pz[gp], v, gr, h);
-          const double bx = cfs[1][g], by = cfs[2][g], bz = cfs[3][g];
-          const double x = X[j][r], w = W[j][r];
-          c[0] = -2.0 * (gr[0] * w + (h[0] * bx + h[1] * by + h[2] * bz) * x);
-          c[1] = -2.0 * (gr[1] * w + (h[1] * bx + h[3] * by + h[4] * bz) * x);
-          c[2] = -2.0 * (gr[2] * w + (h[2] * bx + h[4] * by + h[5] * bz) * x);
-        } else {
-          double v, gx, gy, gz;
-          sto_value_grad<double>(rec, T, px[gp], py[gp], pz[gp], v, gx, gy, gz);
-          const double w = W[j][r];
-          c[0] = -2.0 * gx * w;
-          c[1] = -2.0 * gy * w;
-          c[2] = -2.0 * gz * w;
-        }
-        for (int k = 0; k < 3; ++k) {
-          cm[r][k] += c[k];
-          tp[j][k] += c[k];
-        }
+          for (int r = 0; r < 4; ++r) X[j][r] = W[j][r] = 0.0;
       }
+    };
+    issue(0, bufA);
+    for (int st = 0; st < nsteps; st += 2) {
+      step(st, bufA, bufB);
+      if (st + 1 < nsteps) step(st + 1, bufB, bufA);
     }
-    // per function: sum over the batch's points (j in registers, l16 lanes)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      for (int k = 0; k < 3; ++k) {
-        double v = cm[r][k];
-        v += __shfl_xor(v, 1);
-        v += __shfl_xor(v, 2);
-        v += __shfl_xor(v, 4);
-        v += __shfl_xor(v, 8);
-        if (l16 == 0) cpart[(mi - s0) * GC + (16 * wv + 4 * q + r) * 3 + k] = v;
-      }
-    // per point: sum over this block's functions (q lanes, then waves)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 3; ++k) {
-        double v = tp[j][k];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        if (q == 0) red[wv][k][16 * j + l16] = v;
-      }
-    __syncthreads();
-    if (threadIdx.x < 3 * PT) {
-      const int k = threadIdx.x / PT, g = threadIdx.x % PT;
-      tsum[k][g] += red[0][k][g] + red[1][k][g] + red[2][k][g] + red[3][k][g];
-    }
-    __syncthreads();
   }
   if (threadIdx.x < 3 * PT) {
     const int k = threadIdx.x / PT, g = threadIdx.x % PT;
