@@ -47,24 +47,36 @@ __global__ void __launch_bounds__(256) y_slots_i8(int64_t s0, const int *__restr
 #pragma unroll
   for (int p = 0; p < 4; ++p) fp[p] = ld16(tb + OFF_PM + p * PLANE + t * 16);
   const int eP = tb[OFF_PM + 4 * PLANE + g];
+  // every load is issued before the first use (a branch between them would
+  // make each wait for the one before)
   const float *G = reinterpret_cast<const float *>(tb + OFF_G);
+  float gv[3][16];
+  if (gga) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) gv[c][e] = G[c * TILE + (k0 + e) * PT + g];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) gv[c][e] = 0.0f;
+  }
   const signed char *c0 = reinterpret_cast<const signed char *>(&fp[0]);
   const signed char *c1 = reinterpret_cast<const signed char *>(&fp[1]);
   const signed char *c2 = reinterpret_cast<const signed char *>(&fp[2]);
   const signed char *c3 = reinterpret_cast<const signed char *>(&fp[3]);
+  // 2^(eP - 6) scales the slices exactly; 0 for a point stored as zero
+  const double sP = pow2_or_zero(eP == -127 ? -127 : eP - 6);
   double y[16];
   double m = 0.0;
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const int f = k0 + e;
-    double ph = 0.0;
-    if (eP != -127) {
-      const double v = (double)c0[e] + ldexp((double)c1[e], -7) + ldexp((double)c2[e], -14) +
-                       ldexp((double)c3[e], -21);
-      ph = ldexp(v, eP - 6);
-    }
-    const double yy = gga ? z_value(ph, (double)G[f * PT + g], (double)G[TILE + f * PT + g],
-                                    (double)G[2 * TILE + f * PT + g], a2, bx, by, bz)
+    const double v = (double)c0[e] + ldexp((double)c1[e], -7) + ldexp((double)c2[e], -14) +
+                     ldexp((double)c3[e], -21);
+    const double ph = v * sP;
+    const double yy = gga ? z_value(ph, (double)gv[0][e], (double)gv[1][e], (double)gv[2][e],
+                                    a2, bx, by, bz)
                           : a2 * ph;
     y[e] = yy;
     m = fmax(m, fabs(yy));
