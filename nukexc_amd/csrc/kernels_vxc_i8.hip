@@ -105,8 +105,8 @@ __global__ void __launch_bounds__(256) colloc_tiles_i8(
     }
   }
   __syncthreads();
-  store_section(v, eF, base + OFF_F, false);
-  store_section(v, eP, base + OFF_PM, true);
+  store_section<false>(v, eF, base + OFF_F);
+  store_section<true>(v, eP, base + OFF_PM);
 }
 
 // P (FP64, original order) -> one section per 64 x 64 block pair (fm, fk),
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) gather_p_i8(int nbf, int nfb,
     ex[threadIdx.x] = row_exp(m);
   }
   __syncthreads();
-  store_section(v, ex, Ps + (int64_t)blockIdx.x * SEC, false);
+  store_section<false>(v, ex, Ps + (int64_t)blockIdx.x * SEC);
 }
 
 

@@ -159,20 +159,26 @@ __device__ __forceinline__ double z_value(double ph, double gx, double gy, doubl
   return fma(bz, gz, fma(by, gy, fma(bx, gx, a * ph)));
 }
 
-// Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65) with
-// per-row exponents ex[row] into a section at dst (fragment order); one
-// thread per (row fragment, lane) item, 256 threads.
-__device__ inline void store_section(const double (*v)[65], const int *ex, signed char *dst,
-                              bool transpose) {
+// Slice a 64 x 64 FP64 tile held in LDS (v[row][k], row stride 65; TR: the
+// tile read transposed, v[k][row]) with per-row exponents ex[row] into a
+// section at dst (fragment order); one thread per (row fragment, lane) item,
+// 256 threads.  Inlined and specialised on TR: as a called function with a
+// run-time flag every element paid for the select and the call for its
+// register saves.  A row stored as zero slices 0 with exponent 0 (no branch).
+template <bool TR>
+__device__ __forceinline__ void store_section(const double (*v)[65], const int *ex,
+                                              signed char *dst) {
   const int t = threadIdx.x;
   const int rt = t >> 6, l = t & 63;
   const int row = 16 * rt + (l & 15), k0 = 16 * (l >> 4);
+  const int er = ex[row];
+  const bool zero = er == -127;
   __attribute__((aligned(16))) signed char out[4][16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const double x = transpose ? v[k0 + e][row] : v[row][k0 + e];
+    const double x = TR ? v[k0 + e][row] : v[row][k0 + e];
     signed char s[4];
-    slice4(x, ex[row], s);
+    slice4(zero ? 0.0 : x, zero ? 0 : er, s);
 #pragma unroll
     for (int p = 0; p < 4; ++p) out[p][e] = s[p];
   }
