@@ -282,8 +282,10 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   __shared__ __attribute__((aligned(16))) signed char bufA[GBT][SEC];
   __shared__ __attribute__((aligned(16))) signed char bufB[GBT][SEC];
   __shared__ double red[4][4][PT];
-  __shared__ int tab[GBT][MAXNFB];
-  __shared__ int ublk[MAXNFB];
+  // slot table and union list of the batch group (one batch: not needed)
+  __shared__ int tab[GBT == 1 ? 1 : GBT][GBT == 1 ? 1 : MAXNFB];
+  __shared__ int ublk[GBT == 1 ? 1 : MAXNFB];
+  __shared__ int pos_s[2];
   // active steps, one record each: k | mask << 16 (mask bit m * GBT + sb:
   // block m and batch sb take part; bit 7: last step of an exponent group),
   // then the slot of (k, sb) per batch (-1 where k is dead in sb)
@@ -292,91 +294,141 @@ __global__ void __launch_bounds__(256) density_x2_i8(
   __shared__ __attribute__((aligned(16))) int srec[MAXNFB][SR];
   __shared__ int nu_s, nst_s;
 
-  for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
-  __syncthreads();
-  for (int sb = 0; sb < nsub; ++sb) {
-    const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
-    for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nfb; base += 64) {
-      const int f = base + lane;
-      bool live = false;
-      if (f < nfb)
-        for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
-      const unsigned long long m = __ballot(live);
-      if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
-      cnt += __popcll(m);
-    }
-    if (lane == 0) nu_s = cnt;
-  }
-  __syncthreads();
-  const int nu = __builtin_amdgcn_readfirstlane(nu_s);
   int fm0, fm1;
-  if (ABS) {  // blocks 2 mgi, 2 mgi + 1 of the batch's molecule, if live
-    const int lo = b_off[bg];
-    if (lo == b_off[bg + 1]) return;  // uniform: empty batch
-    const int f0 = b_fbl[lo];
-    const int blk0 = f0 - mt.blk_local[f0], nfm = mt.nfb[mt.blk_mol[f0]];
-    int a = 2 * mgi < nfm ? blk0 + 2 * mgi : -1;
-    int c = 2 * mgi + 1 < nfm ? blk0 + 2 * mgi + 1 : -1;
-    if (a >= 0 && tab[0][a] < 0) a = -1;
-    if (c >= 0 && tab[0][c] < 0) c = -1;
-    if (a < 0) {
-      a = c;
-      c = -1;
-    }
-    if (a < 0) return;  // uniform: both blocks dead here
-    fm0 = a;
-    fm1 = c;
-  } else {
-    if (mgi * 2 >= nu) return;  // uniform: no block of this pair
-    fm0 = __builtin_amdgcn_readfirstlane(ublk[mgi * 2]);
-    fm1 = mgi * 2 + 1 < nu ? __builtin_amdgcn_readfirstlane(ublk[mgi * 2 + 1]) : -1;
-  }
-  if (wv == 0) {
-    int cnt = 0;
-    for (int base = 0; base < nu; base += 64) {
-      const int ki = base + lane;
-      unsigned pm = 0;
-      if (ki < nu) {
-        const int fk = ublk[ki];
-#pragma unroll
-        for (int sb = 0; sb < GBT; ++sb) {
-          if (sb >= nsub || tab[sb][fk] < 0) continue;
-          if (tab[sb][fm0] >= 0) pm |= 1u << sb;
-          if (fm1 >= 0 && tab[sb][fm1] >= 0) pm |= 1u << (GBT + sb);
-        }
+  int sl0 = -1, sl1 = -1;  // one batch: the slots of fm0, fm1
+  if constexpr (GBT == 1) {
+    // one batch: the steps are the batch's own blocks (ascending), its slots
+    // lo .. hi - 1, and both blocks of the pair take part in every step
+    const int lo = b_off[bg], hi = b_off[bg + 1], n = hi - lo;
+    if (ABS) {  // blocks 2 mgi, 2 mgi + 1 of the batch's molecule, if live
+      if (n == 0) return;  // uniform: empty batch
+      const int f0 = b_fbl[lo];
+      const int blk0 = f0 - mt.blk_local[f0], nfm = mt.nfb[mt.blk_mol[f0]];
+      const int a = 2 * mgi < nfm ? blk0 + 2 * mgi : -1;
+      const int c = 2 * mgi + 1 < nfm ? blk0 + 2 * mgi + 1 : -1;
+      if (threadIdx.x < 2) pos_s[threadIdx.x] = -1;
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += 256) {
+        const int f = b_fbl[lo + i];
+        if (a >= 0 && f == a) pos_s[0] = i;
+        if (c >= 0 && f == c) pos_s[1] = i;
       }
-      const unsigned long long m = __ballot(pm != 0);
-      if (pm) {
-        const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        const int fk = ublk[ki];
-        srec[i][0] = fk | (int)(pm << 16);
-#pragma unroll
-        for (int sb = 0; sb < SR - 1; ++sb)
-          srec[i][1 + sb] = sb < nsub ? tab[sb < GBT ? sb : 0][fk] : -1;
-      }
-      cnt += __popcll(m);
+      __syncthreads();
+      const int pa = __builtin_amdgcn_readfirstlane(pos_s[0]);
+      const int pc = __builtin_amdgcn_readfirstlane(pos_s[1]);
+      if (pa < 0 && pc < 0) return;  // uniform: both blocks dead here
+      fm0 = pa >= 0 ? a : c;
+      sl0 = lo + (pa >= 0 ? pa : pc);
+      fm1 = pa >= 0 && pc >= 0 ? c : -1;
+      sl1 = fm1 >= 0 ? lo + pc : -1;
+    } else {
+      fm0 = __builtin_amdgcn_readfirstlane(b_fbl[lo + 2 * mgi]);
+      sl0 = lo + 2 * mgi;
+      fm1 = 2 * mgi + 1 < n ? __builtin_amdgcn_readfirstlane(b_fbl[lo + 2 * mgi + 1]) : -1;
+      sl1 = fm1 >= 0 ? sl0 + 1 : -1;
     }
-    // bit 7 of a step's mask: the last step of its exponent group (blocks
-    // with the same local index / KG of one molecule; the steps are in
-    // ascending block order, so a group is a run of steps)
-    for (int i = lane; i < cnt; i += 64) {
-      const int f = srec[i][0] & 0xFFFF;
+    // records; bit 7: the last step of its exponent group (blocks with the
+    // same local index / KG of one molecule, a run of steps)
+    const unsigned pmask = 1u | (fm1 >= 0 ? 2u : 0u);
+    for (int i = threadIdx.x; i < n; i += 256) {
+      const int f = b_fbl[lo + i];
       const int key = f - mt.blk_local[f] % KG;
-      bool last = i + 1 == cnt;
+      bool last = i + 1 == n;
       if (!last) {
-        const int f2 = srec[i + 1][0] & 0xFFFF;
+        const int f2 = b_fbl[lo + i + 1];
         last = f2 - mt.blk_local[f2] % KG != key;
       }
-      if (last) srec[i][0] |= 0x80 << 16;
+      srec[i][0] = f | (int)((pmask | (last ? 0x80u : 0u)) << 16);
+      srec[i][1] = lo + i;
     }
-    if (lane == 0) nst_s = cnt;
+    if (threadIdx.x == 0) nst_s = n;
+    __syncthreads();
+  } else {
+    for (int e = threadIdx.x; e < GBT * nfb; e += 256) tab[e / nfb][e % nfb] = -1;
+    __syncthreads();
+    for (int sb = 0; sb < nsub; ++sb) {
+      const int lo = b_off[bg + sb], hi = b_off[bg + sb + 1];
+      for (int k = lo + (int)threadIdx.x; k < hi; k += 256) tab[sb][b_fbl[k]] = k;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      int cnt = 0;
+      for (int base = 0; base < nfb; base += 64) {
+        const int f = base + lane;
+        bool live = false;
+        if (f < nfb)
+          for (int sb = 0; sb < GBT; ++sb) live = live || tab[sb][f] >= 0;
+        const unsigned long long m = __ballot(live);
+        if (live) ublk[cnt + __popcll(m & ((1ull << lane) - 1ull))] = f;
+        cnt += __popcll(m);
+      }
+      if (lane == 0) nu_s = cnt;
+    }
+    __syncthreads();
+    const int nu = __builtin_amdgcn_readfirstlane(nu_s);
+    if (ABS) {  // blocks 2 mgi, 2 mgi + 1 of the batch's molecule, if live
+      const int lo = b_off[bg];
+      if (lo == b_off[bg + 1]) return;  // uniform: empty batch
+      const int f0 = b_fbl[lo];
+      const int blk0 = f0 - mt.blk_local[f0], nfm = mt.nfb[mt.blk_mol[f0]];
+      int a = 2 * mgi < nfm ? blk0 + 2 * mgi : -1;
+      int c = 2 * mgi + 1 < nfm ? blk0 + 2 * mgi + 1 : -1;
+      if (a >= 0 && tab[0][a] < 0) a = -1;
+      if (c >= 0 && tab[0][c] < 0) c = -1;
+      if (a < 0) {
+        a = c;
+        c = -1;
+      }
+      if (a < 0) return;  // uniform: both blocks dead here
+      fm0 = a;
+      fm1 = c;
+    } else {
+      if (mgi * 2 >= nu) return;  // uniform: no block of this pair
+      fm0 = __builtin_amdgcn_readfirstlane(ublk[mgi * 2]);
+      fm1 = mgi * 2 + 1 < nu ? __builtin_amdgcn_readfirstlane(ublk[mgi * 2 + 1]) : -1;
+    }
+    if (wv == 0) {
+      int cnt = 0;
+      for (int base = 0; base < nu; base += 64) {
+        const int ki = base + lane;
+        unsigned pm = 0;
+        if (ki < nu) {
+          const int fk = ublk[ki];
+  #pragma unroll
+          for (int sb = 0; sb < GBT; ++sb) {
+            if (sb >= nsub || tab[sb][fk] < 0) continue;
+            if (tab[sb][fm0] >= 0) pm |= 1u << sb;
+            if (fm1 >= 0 && tab[sb][fm1] >= 0) pm |= 1u << (GBT + sb);
+          }
+        }
+        const unsigned long long m = __ballot(pm != 0);
+        if (pm) {
+          const int i = cnt + __popcll(m & ((1ull << lane) - 1ull));
+          const int fk = ublk[ki];
+          srec[i][0] = fk | (int)(pm << 16);
+  #pragma unroll
+          for (int sb = 0; sb < SR - 1; ++sb)
+            srec[i][1 + sb] = sb < nsub ? tab[sb < GBT ? sb : 0][fk] : -1;
+        }
+        cnt += __popcll(m);
+      }
+      // bit 7 of a step's mask: the last step of its exponent group (blocks
+      // with the same local index / KG of one molecule; the steps are in
+      // ascending block order, so a group is a run of steps)
+      for (int i = lane; i < cnt; i += 64) {
+        const int f = srec[i][0] & 0xFFFF;
+        const int key = f - mt.blk_local[f] % KG;
+        bool last = i + 1 == cnt;
+        if (!last) {
+          const int f2 = srec[i + 1][0] & 0xFFFF;
+          last = f2 - mt.blk_local[f2] % KG != key;
+        }
+        if (last) srec[i][0] |= 0x80 << 16;
+      }
+      if (lane == 0) nst_s = cnt;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int nst = __builtin_amdgcn_readfirstlane(nst_s);
   const int nq = gga ? 4 : 1;
   const int fm = mb ? fm1 : fm0;
@@ -524,7 +576,8 @@ __global__ void __launch_bounds__(256) density_x2_i8(
 #pragma unroll 1
     for (int m = 0; m < 2; ++m) {
       const int fmx = m ? fm1 : fm0;
-      const int sl = fmx >= 0 ? __builtin_amdgcn_readfirstlane(tab[sb][fmx]) : -1;
+      const int sl = GBT == 1 ? (m ? sl1 : sl0)
+                              : (fmx >= 0 ? __builtin_amdgcn_readfirstlane(tab[sb][fmx]) : -1);
       if (sl < 0) continue;  // uniform
       if (mb == m) {
 #pragma unroll
