@@ -15,9 +15,10 @@
 // and every inner sum runs on v_mfma_i32_16x16x64_i8 in exact int32
 // arithmetic (|sum| <= 4 * 64^3 < 2^31).  The ten slice products with
 // s + t <= 3 are kept; the dropped ones are below 2^-28 of the row-times-
-// column scale, of random sign.  The int32 sums of one tile product are
-// combined and scaled in FP64, per 64-function block (density) and per
-// 64-point batch (V).  The functional, rho, grad rho, E_xc, N_el and the V
+// column scale, of random sign.  The int32 sums are combined and scaled in
+// FP64 per exponent group of KG 64-function blocks (density: the P row and Pm
+// point exponents are shared over the group, the sums run across its steps)
+// and per 64-point batch (V).  The functional, rho, grad rho, E_xc, N_el and the V
 // accumulation are FP64; collocation gradients are FP32 (rounded once from
 // FP64).  Exact accumulation is the point of the integer form:
 // v_mfma_f32_*_bf16 rounds its FP32 sums relative to the largest term with a
@@ -164,7 +165,9 @@ __device__ __forceinline__ double z_value(double ph, double gx, double gy, doubl
 // section at dst (fragment order); one thread per (row fragment, lane) item,
 // 256 threads.  Inlined and specialised on TR: as a called function with a
 // run-time flag every element paid for the select and the call for its
-// register saves.  A row stored as zero slices 0 with exponent 0 (no branch).
+// register saves.  A row stored as zero (exponent -127) is sliced as 0.0
+// with exponent 0, so that slice4 takes no branch; its stored exponent stays
+// -127.
 template <bool TR>
 __device__ __forceinline__ void store_section(const double (*v)[65], const int *ex,
                                               signed char *dst) {
