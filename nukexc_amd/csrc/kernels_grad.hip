@@ -232,11 +232,38 @@ __device__ double becke_cell(int i, int natoms, const double *C, double x, doubl
   return Pi;
 }
 
-// sum over the 64 lanes of a wave (every lane receives it)
+// DPP move of a double (both halves) from the lane that CTRL selects inside the
+// same row of 16 lanes; every lane must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wave; every lane must be active, and every lane
+// receives the same value.  Inside each row of 16 lanes by DPP moves (quad
+// permutations for lane ^ 1 and lane ^ 2, then the half-row and row mirrors):
+// each step adds the same two partial sums in every lane (a + b and b + a are
+// the same double), so all lanes of a row hold the same row sum.  The four row
+// sums are read out of lanes 0, 16, 32 and 48 and added in a fixed order.
+// No LDS instruction is involved (lane shuffles through ds_bpermute would
+// put twelve dependent LDS round trips in every sum).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141>(v);  // row_half_mirror: lane 7 - i of each 8
+  v += dpp_move<0x140>(v);  // row_mirror: lane 15 - i of each 16
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 // One thread per sorted point, one wave per workgroup; per-atom sums in LDS
@@ -365,14 +392,24 @@ __global__ void __launch_bounds__(64) grad_weights_kernel(
     wpart[(int64_t)blockIdx.x * 3 * natoms + e] = acc[e];
 }
 
+// out[k] = sum over the rows of in[r][k]; one workgroup per column.  Thread t
+// sums the rows r = t (mod 256) in ascending order, then a fixed tree in LDS
+// adds the 256 partial sums, so the order of the additions is the same on
+// every run.
 __global__ void __launch_bounds__(256) colsum_kernel(int nrow, int ncol,
                                                      const double *__restrict__ in,
                                                      double *__restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ncol) return;
+  __shared__ double part[256];
+  const int k = blockIdx.x, t = threadIdx.x;
   double v = 0.0;
-  for (int r = 0; r < nrow; ++r) v += in[(int64_t)r * ncol + k];
-  out[k] = v;
+  for (int r = t; r < nrow; r += 256) v += in[(int64_t)r * ncol + k];
+  part[t] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) part[t] += part[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[k] = part[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -412,8 +449,7 @@ void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *center
 
 void vkg_colsum(hipStream_t s, int nrow, int ncol, const double *in, double *out) {
   if (ncol <= 0) return;
-  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, nrow,
-                     ncol, in, out);
+  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)ncol), dim3(256), 0, s, nrow, ncol, in, out);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

@@ -941,7 +941,10 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
     max_slots = std::max<int64_t>(max_slots, pl.b_off_h[b + 1] - pl.b_off_h[b]);
   const int64_t slots_alloc = std::min<int64_t>(max_slots, std::max<int64_t>(pl.nslots, 1));
   const int nseg = 32;
-  const int nblk = 4096;  // one-wave workgroups of the weight-derivative kernel
+  // one-wave workgroups of the weight-derivative kernel: one wave of points
+  // each up to 16384 workgroups, so that the dispatcher balances waves of
+  // unequal cost (points near many atoms cost more)
+  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(pl.npts_pad / 64, 1), 16384);
   const size_t psb = (size_t)mixed_p_bytes(pl.mol_nfb_h[0]);
   const size_t dsz = (size_t)slots_alloc * 4 * PT;
   if (mixed) {
