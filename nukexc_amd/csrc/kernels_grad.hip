@@ -213,18 +213,33 @@ __device__ __forceinline__ double gdist(double x, double y, double z, const doub
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// Per atom pair (i, j): 1/R~_ij and the factor of the mu (R_i - R_j) term of
+// dmu_ij/dR, [R~ = R_ij] / R_ij^2 (zero where the cap applies), with
+// R~ = min(R_ij, 5) (capped partition) or R_ij.  One thread per pair.
+__global__ void __launch_bounds__(256) pair_table_kernel(int natoms, const double *__restrict__ C,
+                                                         int cap, double2 *__restrict__ tab) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= natoms * natoms) return;
+  const int i = e / natoms, j = e % natoms;
+  const double dx = C[3 * i] - C[3 * j], dy = C[3 * i + 1] - C[3 * j + 1],
+               dz = C[3 * i + 2] - C[3 * j + 2];
+  const double R = sqrt(dx * dx + dy * dy + dz * dz);
+  const bool capped = cap && R >= 5.0;
+  tab[e] = i == j ? make_double2(0.0, 0.0)
+                  : make_double2(1.0 / (capped ? 5.0 : R), capped ? 0.0 : 1.0 / (R * R));
+}
+
 // Becke cell function P_i at (x, y, z); same screening and clamping as the
-// partition kernel.
-__device__ double becke_cell(int i, int natoms, const double *C, double x, double y, double z,
-                             double ri, bool cap) {
+// partition kernel, with mu_ij = (r_i - r_j) / R~_ij taken as a product with
+// the tabulated 1/R~_ij (tab = row i of the pair table).
+__device__ double becke_cell(int i, int natoms, const double *C, const double2 *tab, double x,
+                             double y, double z, double ri, bool cap) {
   double Pi = 1.0;
   for (int j = 0; j < natoms; ++j) {
     if (j == i) continue;
     const double rj = gdist(x, y, z, C + 3 * j);
     if (cap && ri - rj <= -5.0) continue;  // mu <= -1: factor 1
-    double Rij = gdist(C[3 * i], C[3 * i + 1], C[3 * i + 2], C + 3 * j);
-    if (cap) Rij = fmin(Rij, 5.0);
-    double mu = (ri - rj) / Rij;
+    double mu = (ri - rj) * tab[j].x;
     mu = mu > 1.0 ? 1.0 : (mu < -1.0 ? -1.0 : mu);
     Pi *= 0.5 * (1.0 - bp(bp(bp(mu))));
     if (Pi == 0.0) break;
@@ -279,7 +294,7 @@ __global__ void __launch_bounds__(64) grad_weights_kernel(
     int64_t npts_pad, int natoms, const double *__restrict__ C, const double *__restrict__ px,
     const double *__restrict__ py, const double *__restrict__ pz, const int *__restrict__ pown,
     const double *__restrict__ epoint, const double *__restrict__ tpt, int partition,
-    double *__restrict__ wpart) {
+    const double2 *__restrict__ ptab, double *__restrict__ wpart) {
   extern __shared__ double acc[];
   for (int e = threadIdx.x; e < 3 * natoms; e += blockDim.x) acc[e] = 0.0;
   __syncthreads();
@@ -305,7 +320,7 @@ __global__ void __launch_bounds__(64) grad_weights_kernel(
       for (int i = 0; i < natoms; ++i) {
         const double ri = gdist(x, y, z, C + 3 * i);
         if (cap && ri - rmin >= 5.0) continue;
-        const double Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
+        const double Pi = becke_cell(i, natoms, C, ptab + i * natoms, x, y, z, ri, cap);
         Z += Pi;
         if (i == B) PB = Pi;
       }
@@ -320,13 +335,15 @@ __global__ void __launch_bounds__(64) grad_weights_kernel(
           li = !(cap && ri - rmin >= 5.0);
         }
         if (li) {
-          Pi = becke_cell(i, natoms, C, x, y, z, ri, cap);
+          Pi = becke_cell(i, natoms, C, ptab + i * natoms, x, y, z, ri, cap);
           li = Pi != 0.0;
         }
         if (!__ballot(li)) continue;
         const double coefi = li ? wf * ((i == B ? 1.0 / PB : 0.0) - 1.0 / Z) * Pi : 0.0;
         const double *Ci = C + 3 * i;
-        const double ui[3] = {(x - Ci[0]) / ri, (y - Ci[1]) / ri, (z - Ci[2]) / ri};
+        const double2 *ti = ptab + i * natoms;
+        const double iri = 1.0 / ri;
+        const double ui[3] = {(x - Ci[0]) * iri, (y - Ci[1]) * iri, (z - Ci[2]) * iri};
         double gi[3] = {0.0, 0.0, 0.0};
         for (int j = 0; j < natoms; ++j) {
           if (j == i) continue;
@@ -340,20 +357,19 @@ __global__ void __launch_bounds__(64) grad_weights_kernel(
           }
           if (lj) {
             const double d[3] = {Ci[0] - Cj[0], Ci[1] - Cj[1], Ci[2] - Cj[2]};
-            const double Rij = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            const bool capped_here = cap && Rij >= 5.0;
-            const double Rt = capped_here ? 5.0 : Rij;
-            const double mu = (ri - rj) / Rt;
+            const double2 tij = ti[j];  // 1/R~, [not capped] / R^2
+            const double mu = (ri - rj) * tij.x;
             if (mu > -1.0 && mu < 1.0) {  // clamped: s' = 0
               const double p1 = bp(mu), p2 = bp(p1), p3 = bp(p2);
               const double sv = 0.5 * (1.0 - p3);
               const double ds = -0.5 * bdp(p2) * bdp(p1) * bdp(mu);
               const double f = coefi * ds / sv;  // s > 0 because P_i > 0
-              const double ct = capped_here ? 0.0 : mu / (Rij * Rij);
-              const double uj[3] = {(x - Cj[0]) / rj, (y - Cj[1]) / rj, (z - Cj[2]) / rj};
+              const double ct = mu * tij.y;
+              const double irj = 1.0 / rj;
+              const double uj[3] = {(x - Cj[0]) * irj, (y - Cj[1]) * irj, (z - Cj[2]) * irj};
               for (int k = 0; k < 3; ++k) {
-                gi[k] += f * (-ui[k] / Rt - ct * d[k]);
-                if (j != B) gj[k] = f * (uj[k] / Rt + ct * d[k]);
+                gi[k] += f * (-ui[k] * tij.x - ct * d[k]);
+                if (j != B) gj[k] = f * (uj[k] * tij.x + ct * d[k]);
               }
             } else {
               lj = false;
@@ -440,10 +456,15 @@ void vkg_owners(hipStream_t s, const Plan &pl, const int *owner, int *pown) {
 
 void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *centers,
                  const int *pown, const double *epoint, const double *tpt, int partition,
-                 int nblk, double *wpart) {
+                 int nblk, double *ptab, double *wpart) {
+  double2 *tab = reinterpret_cast<double2 *>(ptab);
+  const int npair = natoms * natoms;
+  hipLaunchKernelGGL(pair_table_kernel, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, s,
+                     natoms, centers, partition == 1 ? 1 : 0, tab);
+  NKX_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(grad_weights_kernel, dim3((unsigned)nblk), dim3(64),
                      (size_t)3 * natoms * sizeof(double), s, pl.npts_pad, natoms, centers,
-                     pl.px.p, pl.py.p, pl.pz.p, pown, epoint, tpt, partition, wpart);
+                     pl.px.p, pl.py.p, pl.pz.p, pown, epoint, tpt, partition, tab, wpart);
   NKX_HIP_CHECK(hipGetLastError());
 }
 

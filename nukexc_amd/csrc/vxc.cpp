@@ -959,13 +959,16 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
   ws.coef.reserve((size_t)4 * nspin * pl.npts_pad);
   ws.blockEN.reserve((size_t)3 * pl.nb);
   ws.red.reserve(3);
-  // gradient buffers: epoint, tpt, cpart, gacc, gfun, wpart, wsum
+  // gradient buffers: ptab, epoint, tpt, cpart, gacc, gfun, wpart, wsum
   const size_t n_ep = pl.npts_pad, n_tp = 3 * pl.npts_pad, n_cp = (size_t)slots_alloc * 3 * FB;
   const size_t n_ga = (size_t)nseg * pl.nbf_pad * 3, n_gf = (size_t)pl.nbf_pad * 3;
   const size_t n_wp = (size_t)nblk * 3 * natoms, n_ws = (size_t)3 * natoms;
-  ws.gbuf.reserve(n_ep + n_tp + n_cp + n_ga + n_gf + n_wp + n_ws);
-  double *epoint = ws.gbuf.p, *tpt = epoint + n_ep, *cpart = tpt + n_tp, *gacc = cpart + n_cp,
-         *gfun = gacc + n_ga, *wpart = gfun + n_gf, *wsum = wpart + n_wp;
+  // atom-pair table first, rounded to 256 bytes, so the others keep the
+  // alignment of the allocation
+  const size_t n_pt = ((size_t)2 * natoms * natoms + 31) & ~(size_t)31;
+  ws.gbuf.reserve(n_pt + n_ep + n_tp + n_cp + n_ga + n_gf + n_wp + n_ws);
+  double *ptab = ws.gbuf.p, *epoint = ptab + n_pt, *tpt = epoint + n_ep, *cpart = tpt + n_tp,
+         *gacc = cpart + n_cp, *gfun = gacc + n_ga, *wpart = gfun + n_gf, *wsum = wpart + n_wp;
   ws.gint.reserve(pl.npts_pad);
 
   for (int sp = 0; sp < nspin; ++sp) {
@@ -1020,7 +1023,8 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
   vkg_colsum(s, nseg, pl.nbf_pad * 3, gacc, gfun);
   vk_en_reduce(s, pl, ws.blockEN.p, ws.red.p);
   vkg_owners(s, pl, c.grid.owner.p, ws.gint.p);
-  vkg_weights(s, pl, natoms, c.grid.centers.p, ws.gint.p, epoint, tpt, partition, nblk, wpart);
+  vkg_weights(s, pl, natoms, c.grid.centers.p, ws.gint.p, epoint, tpt, partition, nblk, ptab,
+              wpart);
   vkg_colsum(s, nblk, 3 * natoms, wpart, wsum);
   std::vector<double> gf(n_gf), wsh(n_ws);
   double en[3];

@@ -123,10 +123,11 @@ void vkg_fold(hipStream_t s, int64_t b0, int64_t b1, int64_t s0, const Plan &pl,
 // pown[sp] = owner of sorted point sp (-1 for padding)
 void vkg_owners(hipStream_t s, const Plan &pl, const int *owner, int *pown);
 // point-motion and Becke weight-derivative terms per atom; partition 0 = fixed
-// weights, 1 = capped Becke, 2 = Becke.  wpart[block][3 natoms], nblk blocks.
+// weights, 1 = capped Becke, 2 = Becke.  wpart[block][3 natoms], nblk blocks;
+// ptab: 2 natoms^2 doubles of scratch for the per-pair table.
 void vkg_weights(hipStream_t s, const Plan &pl, int natoms, const double *centers,
                  const int *pown, const double *epoint, const double *tpt, int partition,
-                 int nblk, double *wpart);
+                 int nblk, double *ptab, double *wpart);
 // out[k] = sum over rows r < nrow of in[r * ncol + k], fixed order
 void vkg_colsum(hipStream_t s, int nrow, int ncol, const double *in, double *out);
 
