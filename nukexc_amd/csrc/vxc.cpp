@@ -943,8 +943,12 @@ void xc_grad(Ctx &c, int nspin, const double *const *P, const XcList &xl, bool g
   const int nseg = 32;
   // one-wave workgroups of the weight-derivative kernel: one wave of points
   // each up to 16384 workgroups, so that the dispatcher balances waves of
-  // unequal cost (points near many atoms cost more)
-  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(pl.npts_pad / 64, 1), 16384);
+  // unequal cost (points near many atoms cost more); fewer for large
+  // molecules, so that the per-workgroup partials stay within 2^25 doubles
+  const int64_t nblk_cap =
+      std::max<int64_t>(1024, ((int64_t)1 << 25) / std::max<int64_t>(3 * (int64_t)natoms, 1));
+  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(pl.npts_pad / 64, 1),
+                                          std::min<int64_t>(16384, nblk_cap));
   const size_t psb = (size_t)mixed_p_bytes(pl.mol_nfb_h[0]);
   const size_t dsz = (size_t)slots_alloc * 4 * PT;
   if (mixed) {
