@@ -284,12 +284,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One thread per sorted point, one wave per workgroup; per-atom sums in LDS
 // (3 natoms doubles).  Every loop over atoms runs in step across the wave
 // (lanes without a contribution add zero), so each per-atom term is summed
-// over the wave by shuffles and added to LDS by lane 0: all lanes of a wave
-// touch the same atoms, and one LDS atomic per lane would serialise on them.
-// With one writer per workgroup (no atomics) the sums, and so the gradient,
-// come out the same on every run.  The points of a wave are 64 consecutive sorted points
-// (npts_pad and the grid stride are multiples of 64), so every lane runs the
-// same trip count.
+// over the wave (wave_sum: every lane active) and added to LDS by lane 0: all
+// lanes of a wave touch the same atoms, and one LDS atomic per lane would
+// serialise on them.  With one writer per workgroup (no atomics) the sums, and
+// so the gradient, come out the same on every run.  The points of a wave are
+// 64 consecutive sorted points (npts_pad and the grid stride are multiples of
+// 64), so every lane runs the same trip count.  ptab: the atom-pair table of
+// pair_table_kernel.
 __global__ void __launch_bounds__(64) grad_weights_kernel(
     int64_t npts_pad, int natoms, const double *__restrict__ C, const double *__restrict__ px,
     const double *__restrict__ py, const double *__restrict__ pz, const int *__restrict__ pown,
