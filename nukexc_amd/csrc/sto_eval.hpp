@@ -109,9 +109,11 @@ __device__ __forceinline__ void sto_value_grad_hess(const FuncRec &f, const Harm
     Sh[4] += c * p0[0] * p1[1] * p1[2];
     Sh[5] += c * p0[0] * p0[1] * p2[2];
   }
-  const double q = f.k / r - f.zeta;
-  const double R1r = q * R / r;                                 // R' / r
-  const double A = ((q * q - f.k / (r * r)) * R - R1r) / (r * r);  // (R'' - R'/r) / r^2
+  // one reciprocal instead of four divisions by r and r^2
+  const double ir = 1.0 / r, ir2 = ir * ir;
+  const double q = f.k * ir - f.zeta;
+  const double R1r = q * R * ir;                              // R' / r
+  const double A = ((q * q - f.k * ir2) * R - R1r) * ir2;     // (R'' - R'/r) / r^2
   v = R * S;
   for (int i = 0; i < 3; ++i) g[i] = R * Sg[i] + S * d[i] * R1r;
   const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
